@@ -1,0 +1,717 @@
+// abi.hip -- the C-ABI of include/d3q19_abi.h over HIP.
+//
+// Replaces the reference's OpenCL plumbing: device discovery (analyse_platform,
+// D3Q19-OpenCL.c:847-971), buffer creation and upload (sim_main.c:115-206), kernel creation and
+// argument binding (create_LB_kernels D3Q19-OpenCL.c:465-556, sim_main.c:241-306), the per-step
+// enqueue sequence (sim_main.c:315-471) and error_check (D3Q19-OpenCL.c:1169-1246; here errors
+// are returned, never exit()).
+#include <stdio.h>
+#include <string.h>
+#include <cmath>
+#include <string>
+#include <vector>
+#include "internal.h"
+
+namespace d3q19 {
+
+static thread_local std::string g_last_error;
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+static int fail(int code, const std::string& msg) {
+    set_error(msg);
+    return code;
+}
+
+static size_t pop_size(int storage) {
+    switch (storage) {
+    case D3Q19_STORE_FP16_SHIFT:
+    case D3Q19_STORE_BF16_SHIFT: return 2;
+    case D3Q19_STORE_FP64: return 8;
+    default: return 4;
+    }
+}
+static size_t real_size(int storage) { return storage == D3Q19_STORE_FP64 ? 8 : 4; }
+
+static int validate(const int_param_struct* ip, const flp_param_struct* fp, const d3q19_options* opt) {
+    for (int d = 0; d < 3; d++) {
+        if (ip->LatticeSize[d] < 5) return fail(D3Q19_E_INVALID, "LatticeSize must be >= 5 on every axis");
+        if (ip->LatticeSize[d] - 2 > 4096) return fail(D3Q19_E_UNSUPPORTED, "interior extent above 4096 on an axis");
+        if (ip->BufferSize[d] != 1) return fail(D3Q19_E_INVALID, "BufferSize must be 1 (one-node shell)");
+        if (ip->BoundaryConds[d] != 0 && ip->BoundaryConds[d] != 1) return fail(D3Q19_E_INVALID, "BoundaryConds must be 0 or 1");
+        const int want = ip->BoundaryConds[d] == 0 ? ip->LatticeSize[d] - 2 : ip->LatticeSize[d] - 3;
+        if (ip->SystemSize[d] != want) return fail(D3Q19_E_INVALID, "SystemSize inconsistent with LatticeSize/BoundaryConds (D3Q19-OpenCL.c:119-128)");
+    }
+    if (ip->BoundaryConds[0] + ip->BoundaryConds[1] + ip->BoundaryConds[2] > 1)
+        return fail(D3Q19_E_INVALID, "more than one pair of velocity-boundary faces");
+    const long long nint = (long long)(ip->LatticeSize[0] - 2) * (ip->LatticeSize[1] - 2) * (ip->LatticeSize[2] - 2);
+    if (nint >= (1LL << 31)) return fail(D3Q19_E_UNSUPPORTED, "more than 2^31 interior nodes");
+    if (!(fp->NewtonianTau > 0.5f)) return fail(D3Q19_E_INVALID, "NewtonianTau must exceed 0.5");
+    if (ip->NumParticles < 0) return fail(D3Q19_E_INVALID, "NumParticles < 0");
+    if (ip->NumParticles > 0) {
+        if (ip->PointsPerParticle <= 0 || ip->PointsPerParticle > 4096) return fail(D3Q19_E_INVALID, "PointsPerParticle out of range");
+        if (!(fp->ParticleMass > 0.f) || !(fp->ParticleMomInertia > 0.f)) return fail(D3Q19_E_INVALID, "particle mass / inertia must be positive");
+        if ((long long)ip->NumParticles * ip->PointsPerParticle >= (1LL << 31) / 6) return fail(D3Q19_E_UNSUPPORTED, "too many markers");
+    }
+    if (opt) {
+        if (opt->collision != D3Q19_COLL_MRT && opt->collision != D3Q19_COLL_BGK) return fail(D3Q19_E_INVALID, "unknown collision operator");
+        if (opt->ibm_kernel != D3Q19_IBM_LINEAR_SMOOTHED && opt->ibm_kernel != D3Q19_IBM_PESKIN4) return fail(D3Q19_E_INVALID, "unknown IBM kernel");
+        if (opt->storage < 0 || opt->storage > 3) return fail(D3Q19_E_INVALID, "unknown population storage");
+        if (opt->block[0] < 0 || opt->block[1] < 0) return fail(D3Q19_E_INVALID, "negative block size");
+        if (opt->block[0] > 0 && (opt->block[0] * (opt->block[1] > 0 ? opt->block[1] : 1) > 256 ||
+                                   (opt->block[0] * (opt->block[1] > 0 ? opt->block[1] : 1)) % 64 != 0))
+            return fail(D3Q19_E_INVALID, "block must hold a multiple of 64 and at most 256 threads");
+    }
+    return 0;
+}
+
+static void geom_from(Sim* s) {
+    Geom& g = s->g;
+    g.nx = s->Nx - 2;
+    g.ny = s->Ny - 2;
+    g.nz = s->Nz - 2;
+    g.nint = g.nx * g.ny * g.nz;
+    g.qstride = g.nint;
+    g.wall = -1;
+    for (int d = 0; d < 3; d++) {
+        g.per[d] = s->ip.BoundaryConds[d] == 0 ? 1 : 0;
+        if (!g.per[d]) g.wall = d;
+        g.sys[d] = s->ip.SystemSize[d];
+    }
+    g.bnx = (g.nx + BRICK - 1) / BRICK;
+    g.bny = (g.ny + BRICK - 1) / BRICK;
+    g.bnz = (g.nz + BRICK - 1) / BRICK;
+    FluidParams& f = s->fl;
+    for (int d = 0; d < 3; d++) {
+        f.gconst[d] = s->fp.ConstBodyForce[d];
+        f.uw[0][d] = s->fp.VelLower[d];
+        f.uw[1][d] = s->fp.VelUpper[d];
+    }
+    f.tau0 = s->fp.NewtonianTau;
+    f.model = s->ip.ViscosityModel;
+    for (int k = 0; k < 4; k++) f.vp[k] = s->fp.ViscosityParams[k];
+    f.tau_min = s->opt.tau_min > 0.f ? s->opt.tau_min : 0.505f;
+    f.tau_max = s->opt.tau_max > 0.f ? s->opt.tau_max : 0.f;
+    f.calc_rho = s->opt.calc_rho;
+    f.tan_mask = 0;
+    if (g.wall >= 0)  // tangential passes only run when a velocity boundary exists (sim_main.c:383-405)
+        for (int d = 0; d < 3; d++)
+            if (s->opt.tangential_bc[d] == 1) f.tan_mask |= (1 << d);
+}
+
+static void default_block(Sim* s) {
+    int bx = s->opt.block[0], by = s->opt.block[1];
+    if (bx <= 0) {
+        const int nx = s->g.nx;
+        bx = nx >= 64 ? 64 : (nx > 32 ? 64 : 32);
+        by = 256 / bx;
+    }
+    if (by <= 0) by = 256 / bx;
+    s->block[0] = bx;
+    s->block[1] = by;
+    s->block[2] = 1;
+}
+
+static void* cur_f(Sim* s) { return s->parity == 0 ? s->fA : s->fB; }
+static void* nxt_f(Sim* s) { return s->parity == 0 ? s->fB : s->fA; }
+
+static void free_all(Sim* s) {
+    void* ptrs[] = {s->fA, s->fB, s->u, s->tau, s->facc, s->maskF, s->maskU, s->bcraw, s->pos, s->vel, s->quat,
+                    s->omega, s->ffluid, s->tfluid, s->fcont, s->tcont, s->pts, s->mforce, s->red};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    if (s->graph2) (void)hipGraphExecDestroy(s->graph2);
+    if (s->ev0) (void)hipEventDestroy(s->ev0);
+    if (s->ev1) (void)hipEventDestroy(s->ev1);
+    if (s->stream) (void)hipStreamDestroy(s->stream);
+}
+
+template <class T>
+static int dalloc(T** p, size_t bytes) {
+    if (bytes == 0) bytes = 8;
+    hipError_t e = hipMalloc((void**)p, bytes);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return fail(D3Q19_E_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    }
+    return 0;
+}
+
+// ---- step building blocks --------------------------------------------------------------------------
+static int do_collide_standalone(Sim* s, int flags) {
+    (void)flags;
+    int rc = launch_collide(s, cur_f(s), nxt_f(s), 1, 0, s->stream);
+    if (rc) return rc;
+    s->parity ^= 1;
+    return launch_clear_mask(s, s->maskF, s->stream);
+}
+
+// One step in sim_main.c:315-471 order.  particle_dynamics (CPU queue in the reference) runs
+// first: it reads only forces of the previous step, so the order relative to collide is free.
+static int enqueue_step(Sim* s, int flags, hipStream_t st) {
+    int rc;
+    if (s->np > 0) {
+        if ((rc = launch_dynamics(s, st))) return rc;
+        if ((rc = launch_clear_mask(s, s->maskU, st))) return rc;
+        if ((rc = launch_mark_bricks(s, s->maskU, st))) return rc;
+    }
+    if ((rc = launch_collide(s, cur_f(s), nxt_f(s), (flags & D3Q19_STEP_WRITE_U) ? 1 : 0, 1, st))) return rc;
+    s->parity ^= 1;
+    if (s->np > 0) {
+        if ((rc = launch_clear_mask(s, s->maskF, st))) return rc;
+        if ((rc = launch_ibm(s, st))) return rc;
+        uint8_t* t = s->maskF;
+        s->maskF = s->maskU;
+        s->maskU = t;
+        if ((rc = launch_contacts(s, st))) return rc;
+    }
+    return 0;
+}
+
+static int build_graph(Sim* s) {
+    if (s->graph2) {
+        (void)hipGraphExecDestroy(s->graph2);
+        s->graph2 = nullptr;
+    }
+    hipGraph_t graph;
+    HIP_OK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+    int rc = enqueue_step(s, 0, s->stream);
+    if (!rc) rc = enqueue_step(s, 0, s->stream);
+    hipError_t e = hipStreamEndCapture(s->stream, &graph);
+    if (rc) return rc;
+    if (e != hipSuccess) return fail(D3Q19_E_DEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    e = hipGraphInstantiate(&s->graph2, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) return fail(D3Q19_E_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    // capture enqueued nothing; the two swaps of parity and masks cancel
+    s->graph_f = cur_f(s);
+    s->graph_mask = s->maskF;
+    return 0;
+}
+
+}  // namespace d3q19
+
+using namespace d3q19;
+
+struct d3q19_sim : public Sim {};
+
+extern "C" {
+
+const char* d3q19_strerror(int code) {
+    switch (code) {
+    case D3Q19_OK: return "ok";
+    case D3Q19_E_INVALID: return "invalid argument";
+    case D3Q19_E_DEVICE: return "HIP runtime error";
+    case D3Q19_E_NOMEM: return "out of device memory";
+    case D3Q19_E_UNSUPPORTED: return "unsupported configuration";
+    case D3Q19_E_STATE: return "invalid state";
+    case D3Q19_E_NONFINITE: return "non-finite or non-positive density";
+    case D3Q19_E_CAPACITY: return "capacity exceeded";
+    default: return "unknown error";
+    }
+}
+
+const char* d3q19_last_error(void) { return g_last_error.c_str(); }
+
+int d3q19_device_count(int* count) {
+    if (!count) return fail(D3Q19_E_INVALID, "null count");
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess) {
+        *count = 0;
+        return fail(D3Q19_E_DEVICE, std::string("hipGetDeviceCount: ") + hipGetErrorString(e));
+    }
+    *count = n;
+    return 0;
+}
+
+int d3q19_device_name(int device, char* buf, size_t len) {
+    if (!buf || len == 0) return fail(D3Q19_E_INVALID, "null buffer");
+    hipDeviceProp_t p;
+    HIP_OK(hipGetDeviceProperties(&p, device));
+    snprintf(buf, len, "%s (%s, %d CUs)", p.name, p.gcnArchName, p.multiProcessorCount);
+    return 0;
+}
+
+int d3q19_footprint(const int_param_struct* ip, const d3q19_options* opt, size_t* bytes) {
+    if (!ip || !bytes) return fail(D3Q19_E_INVALID, "null argument");
+    const int st = opt ? opt->storage : 0;
+    const size_t nint = (size_t)(ip->LatticeSize[0] - 2) * (ip->LatticeSize[1] - 2) * (ip->LatticeSize[2] - 2);
+    const size_t np = ip->NumParticles > 0 ? ip->NumParticles : 0;
+    size_t b = 2 * Q * nint * pop_size(st) + 4 * nint * real_size(st);
+    if (np) b += 24 * nint + 2 * (nint / 64 + 1) + np * (22 * 8 + (size_t)ip->PointsPerParticle * 64);
+    *bytes = b;
+    return 0;
+}
+
+int d3q19_create(int device, const int_param_struct* ip, const flp_param_struct* fp, const d3q19_options* opt,
+                 const d3q19_float4* sphere_points, d3q19_sim** out) {
+    if (!ip || !fp || !out) return fail(D3Q19_E_INVALID, "null argument");
+    *out = nullptr;
+    int rc = validate(ip, fp, opt);
+    if (rc) return rc;
+    if (ip->NumParticles > 0 && !sphere_points) return fail(D3Q19_E_INVALID, "sphere_points required when NumParticles > 0");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev)
+        return fail(D3Q19_E_DEVICE, "no such HIP device");
+    HIP_OK(hipSetDevice(device));
+    d3q19_sim* s = new d3q19_sim();
+    s->device = device;
+    s->ip = *ip;
+    s->fp = *fp;
+    if (opt) s->opt = *opt;
+    else memset(&s->opt, 0, sizeof(s->opt));
+    s->Nx = ip->LatticeSize[0];
+    s->Ny = ip->LatticeSize[1];
+    s->Nz = ip->LatticeSize[2];
+    s->nn = (long long)s->Nx * s->Ny * s->Nz;
+    s->storage = s->opt.storage;
+    s->pop_bytes = pop_size(s->storage);
+    geom_from(s);
+    default_block(s);
+    s->np = ip->NumParticles;
+    s->ppp = s->np > 0 ? ip->PointsPerParticle : 0;
+    s->nfa = ip->WorkGroupsPerParticle > 0 ? ip->WorkGroupsPerParticle : 1;
+    const size_t nint = (size_t)s->g.nint;
+    const size_t rs = real_size(s->storage);
+    hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete s;
+        return fail(D3Q19_E_DEVICE, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+    }
+#define ALLOC(p, b)                    \
+    if ((rc = dalloc(&(p), (b)))) {    \
+        free_all(s);                   \
+        delete s;                      \
+        return rc;                     \
+    }
+    ALLOC(s->fA, (size_t)Q * nint * s->pop_bytes);
+    ALLOC(s->fB, (size_t)Q * nint * s->pop_bytes);
+    ALLOC(s->u, 3 * nint * rs);
+    ALLOC(s->tau, nint * rs);
+    const int wallA = s->g.wall;
+    s->bc_plane = wallA >= 0 ? (wallA == 0 ? s->g.ny * s->g.nz : (wallA == 1 ? s->g.nx * s->g.nz : s->g.nx * s->g.ny)) : 1;
+    ALLOC(s->bcraw, (size_t)10 * s->bc_plane * rs);
+    ALLOC(s->red, (size_t)(4 * 512 + 8) * sizeof(double));
+    s->nbricks = s->g.bnx * s->g.bny * s->g.bnz;
+    if (s->np > 0) {
+        ALLOC(s->facc, 3 * nint * sizeof(long long));
+        ALLOC(s->maskF, (size_t)s->nbricks);
+        ALLOC(s->maskU, (size_t)s->nbricks);
+        const size_t npd = (size_t)s->np;
+        ALLOC(s->pos, 3 * npd * 8);
+        ALLOC(s->vel, 3 * npd * 8);
+        ALLOC(s->quat, 4 * npd * 8);
+        ALLOC(s->omega, 3 * npd * 8);
+        ALLOC(s->ffluid, 3 * npd * 8);
+        ALLOC(s->tfluid, 3 * npd * 8);
+        ALLOC(s->fcont, 3 * npd * 8);
+        ALLOC(s->tcont, 3 * npd * 8);
+        ALLOC(s->pts, 4 * (size_t)s->ppp * sizeof(float));
+        ALLOC(s->mforce, 6 * npd * (size_t)s->ppp * 8);
+    }
+#undef ALLOC
+    // initial state: f = f_eq(1, 0), u = 0, tau = NewtonianTau (D3Q19-OpenCL.c:158-187)
+    rc = launch_fill_equilibrium(s, s->fA);
+    if (!rc) rc = launch_fill_equilibrium(s, s->fB);
+    if (rc) {
+        free_all(s);
+        delete s;
+        return rc;
+    }
+    e = hipMemsetAsync(s->u, 0, 3 * nint * rs, s->stream);
+    if (e == hipSuccess && s->np > 0) {
+        e = hipMemsetAsync(s->facc, 0, 3 * nint * sizeof(long long), s->stream);
+        if (e == hipSuccess) e = hipMemsetAsync(s->maskF, 0, s->nbricks, s->stream);
+        if (e == hipSuccess) e = hipMemsetAsync(s->maskU, 0, s->nbricks, s->stream);
+        double* zs[] = {s->pos, s->vel, s->omega, s->ffluid, s->tfluid, s->fcont, s->tcont};
+        for (double* z : zs)
+            if (e == hipSuccess) e = hipMemsetAsync(z, 0, 3 * (size_t)s->np * 8, s->stream);
+        if (e == hipSuccess) {
+            std::vector<double> q(4 * (size_t)s->np, 0.0);
+            for (int p = 0; p < s->np; p++) q[4 * p + 3] = 1.0;
+            e = hipMemcpyAsync(s->quat, q.data(), q.size() * 8, hipMemcpyHostToDevice, s->stream);
+            if (e == hipSuccess) e = hipMemcpyAsync(s->pts, sphere_points, 4 * (size_t)s->ppp * sizeof(float), hipMemcpyHostToDevice, s->stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+        }
+    }
+    if (e == hipSuccess) {
+        // tau field
+        std::vector<float> t(1, s->fp.NewtonianTau);
+        if (rs == 4) {
+            std::vector<float> tv(nint, s->fp.NewtonianTau);
+            e = hipMemcpy(s->tau, tv.data(), nint * 4, hipMemcpyHostToDevice);
+        } else {
+            std::vector<double> tv(nint, (double)s->fp.NewtonianTau);
+            e = hipMemcpy(s->tau, tv.data(), nint * 8, hipMemcpyHostToDevice);
+        }
+    }
+    if (e == hipSuccess) e = hipEventCreate(&s->ev0);
+    if (e == hipSuccess) e = hipEventCreate(&s->ev1);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    if (e != hipSuccess) {
+        free_all(s);
+        delete s;
+        return fail(D3Q19_E_DEVICE, std::string("initialisation: ") + hipGetErrorString(e));
+    }
+    *out = s;
+    return 0;
+}
+
+int d3q19_destroy(d3q19_sim* s) {
+    if (!s) return 0;
+    (void)hipSetDevice(s->device);
+    (void)hipStreamSynchronize(s->stream);
+    free_all(s);
+    delete s;
+    return 0;
+}
+
+#define CHECK_SIM(s)                                                   \
+    do {                                                               \
+        if (!(s)) return fail(D3Q19_E_INVALID, "null simulation");     \
+        HIP_OK(hipSetDevice((s)->device));                             \
+    } while (0)
+
+// staging copies: host buffer <-> temporary device buffer in reference layout
+static int stage_in(d3q19_sim* s, const void* host, size_t bytes, void** dev) {
+    int rc = dalloc((char**)dev, bytes);
+    if (rc) return rc;
+    HIP_OK(hipMemcpyAsync(*dev, host, bytes, hipMemcpyHostToDevice, s->stream));
+    return 0;
+}
+
+static int write_f_any(d3q19_sim* s, const void* f, int is_double) {
+    CHECK_SIM(s);
+    if (!f) return fail(D3Q19_E_INVALID, "null field");
+    void* d = nullptr;
+    const size_t bytes = (size_t)Q * s->nn * (is_double ? 8 : 4);
+    int rc = stage_in(s, f, bytes, &d);
+    if (!rc) rc = launch_import_f(s, d, is_double, cur_f(s));
+    hipError_t e = hipStreamSynchronize(s->stream);
+    if (d) (void)hipFree(d);
+    if (rc) return rc;
+    if (e != hipSuccess) return fail(D3Q19_E_DEVICE, hipGetErrorString(e));
+    return 0;
+}
+
+static int read_f_any(d3q19_sim* s, void* f, int as_double) {
+    CHECK_SIM(s);
+    if (!f) return fail(D3Q19_E_INVALID, "null field");
+    void* d = nullptr;
+    const size_t bytes = (size_t)Q * s->nn * (as_double ? 8 : 4);
+    int rc = dalloc((char**)&d, bytes);
+    if (rc) return rc;
+    hipError_t e = hipMemsetAsync(d, 0, bytes, s->stream);
+    if (e == hipSuccess) rc = launch_export_f(s, cur_f(s), d, as_double);
+    if (e == hipSuccess && !rc) e = hipMemcpyAsync(f, d, bytes, hipMemcpyDeviceToHost, s->stream);
+    hipError_t e2 = hipStreamSynchronize(s->stream);
+    (void)hipFree(d);
+    if (rc) return rc;
+    if (e != hipSuccess || e2 != hipSuccess) return fail(D3Q19_E_DEVICE, hipGetErrorString(e != hipSuccess ? e : e2));
+    return 0;
+}
+
+int d3q19_write_f(d3q19_sim* s, const float* f) { return write_f_any(s, f, 0); }
+int d3q19_read_f(d3q19_sim* s, float* f) { return read_f_any(s, f, 0); }
+int d3q19_write_f64(d3q19_sim* s, const double* f) { return write_f_any(s, f, 1); }
+int d3q19_read_f64(d3q19_sim* s, double* f) { return read_f_any(s, f, 1); }
+
+static int write_field(d3q19_sim* s, const float* h, int ncomp, void* dst) {
+    CHECK_SIM(s);
+    if (!h) return fail(D3Q19_E_INVALID, "null field");
+    void* d = nullptr;
+    int rc = stage_in(s, h, (size_t)ncomp * s->nn * 4, &d);
+    if (!rc) rc = launch_import_field(s, (const float*)d, ncomp, dst);
+    hipError_t e = hipStreamSynchronize(s->stream);
+    if (d) (void)hipFree(d);
+    if (rc) return rc;
+    if (e != hipSuccess) return fail(D3Q19_E_DEVICE, hipGetErrorString(e));
+    return 0;
+}
+
+static int read_field(d3q19_sim* s, float* h, int ncomp, const void* src, int is_force) {
+    CHECK_SIM(s);
+    if (!h) return fail(D3Q19_E_INVALID, "null field");
+    const size_t bytes = (size_t)ncomp * s->nn * 4;
+    if (is_force && !s->facc) {
+        memset(h, 0, bytes);
+        return 0;
+    }
+    void* d = nullptr;
+    int rc = dalloc((char**)&d, bytes);
+    if (rc) return rc;
+    hipError_t e = hipMemsetAsync(d, 0, bytes, s->stream);
+    if (e == hipSuccess) rc = is_force ? launch_export_force(s, (float*)d) : launch_export_field(s, src, ncomp, (float*)d);
+    if (e == hipSuccess && !rc) e = hipMemcpyAsync(h, d, bytes, hipMemcpyDeviceToHost, s->stream);
+    hipError_t e2 = hipStreamSynchronize(s->stream);
+    (void)hipFree(d);
+    if (rc) return rc;
+    if (e != hipSuccess || e2 != hipSuccess) return fail(D3Q19_E_DEVICE, hipGetErrorString(e != hipSuccess ? e : e2));
+    return 0;
+}
+
+int d3q19_write_u(d3q19_sim* s, const float* u) { return write_field(s, u, 3, s ? s->u : nullptr); }
+int d3q19_read_u(d3q19_sim* s, float* u) { return read_field(s, u, 3, s ? s->u : nullptr, 0); }
+int d3q19_write_tau(d3q19_sim* s, const float* t) { return write_field(s, t, 1, s ? s->tau : nullptr); }
+int d3q19_read_tau(d3q19_sim* s, float* t) { return read_field(s, t, 1, s ? s->tau : nullptr, 0); }
+int d3q19_read_force(d3q19_sim* s, float* g) { return read_field(s, g, 3, nullptr, 1); }
+
+int d3q19_write_particles(d3q19_sim* s, const d3q19_float4* pk, const d3q19_float4* pf, const d3q19_float4* pff) {
+    CHECK_SIM(s);
+    const int np = s->np;
+    if (np == 0) return 0;
+    HIP_OK(hipStreamSynchronize(s->stream));
+    std::vector<double> a(3 * (size_t)np), b(3 * (size_t)np), c(4 * (size_t)np), d(3 * (size_t)np);
+    if (pk) {
+        for (int p = 0; p < np; p++) {
+            const d3q19_float4 x = pk[p], v = pk[p + np], q = pk[p + 2 * np], w = pk[p + 3 * np];
+            a[3 * p] = x.x; a[3 * p + 1] = x.y; a[3 * p + 2] = x.z;
+            b[3 * p] = v.x; b[3 * p + 1] = v.y; b[3 * p + 2] = v.z;
+            c[4 * p] = q.x; c[4 * p + 1] = q.y; c[4 * p + 2] = q.z; c[4 * p + 3] = q.w;
+            d[3 * p] = w.x; d[3 * p + 1] = w.y; d[3 * p + 2] = w.z;
+        }
+        HIP_OK(hipMemcpy(s->pos, a.data(), a.size() * 8, hipMemcpyHostToDevice));
+        HIP_OK(hipMemcpy(s->vel, b.data(), b.size() * 8, hipMemcpyHostToDevice));
+        HIP_OK(hipMemcpy(s->quat, c.data(), c.size() * 8, hipMemcpyHostToDevice));
+        HIP_OK(hipMemcpy(s->omega, d.data(), d.size() * 8, hipMemcpyHostToDevice));
+    }
+    if (pf) {
+        for (int p = 0; p < np; p++) {
+            a[3 * p] = pf[p].x; a[3 * p + 1] = pf[p].y; a[3 * p + 2] = pf[p].z;
+            b[3 * p] = pf[p + np].x; b[3 * p + 1] = pf[p + np].y; b[3 * p + 2] = pf[p + np].z;
+        }
+        HIP_OK(hipMemcpy(s->fcont, a.data(), a.size() * 8, hipMemcpyHostToDevice));
+        HIP_OK(hipMemcpy(s->tcont, b.data(), b.size() * 8, hipMemcpyHostToDevice));
+    }
+    if (pff) {
+        // sum of the per-work-group partials (CPU_program.cl:218-223)
+        const int nfa = s->nfa;
+        for (int p = 0; p < np; p++) {
+            double F[3] = {0, 0, 0}, T[3] = {0, 0, 0};
+            for (int fa = 0; fa < nfa; fa++) {
+                const d3q19_float4 f = pff[p * nfa + fa], t = pff[(p + np) * nfa + fa];
+                F[0] += f.x; F[1] += f.y; F[2] += f.z;
+                T[0] += t.x; T[1] += t.y; T[2] += t.z;
+            }
+            for (int k = 0; k < 3; k++) { a[3 * p + k] = F[k]; b[3 * p + k] = T[k]; }
+        }
+        HIP_OK(hipMemcpy(s->ffluid, a.data(), a.size() * 8, hipMemcpyHostToDevice));
+        HIP_OK(hipMemcpy(s->tfluid, b.data(), b.size() * 8, hipMemcpyHostToDevice));
+    }
+    return 0;
+}
+
+int d3q19_read_particles(d3q19_sim* s, d3q19_float4* pk, d3q19_float4* pf, d3q19_float4* pff) {
+    CHECK_SIM(s);
+    const int np = s->np;
+    if (np == 0) return 0;
+    HIP_OK(hipStreamSynchronize(s->stream));
+    std::vector<double> a(3 * (size_t)np), b(3 * (size_t)np), c(4 * (size_t)np), d(3 * (size_t)np);
+    if (pk) {
+        HIP_OK(hipMemcpy(a.data(), s->pos, a.size() * 8, hipMemcpyDeviceToHost));
+        HIP_OK(hipMemcpy(b.data(), s->vel, b.size() * 8, hipMemcpyDeviceToHost));
+        HIP_OK(hipMemcpy(c.data(), s->quat, c.size() * 8, hipMemcpyDeviceToHost));
+        HIP_OK(hipMemcpy(d.data(), s->omega, d.size() * 8, hipMemcpyDeviceToHost));
+        for (int p = 0; p < np; p++) {
+            pk[p] = {(float)a[3 * p], (float)a[3 * p + 1], (float)a[3 * p + 2], 0.f};
+            pk[p + np] = {(float)b[3 * p], (float)b[3 * p + 1], (float)b[3 * p + 2], 0.f};
+            pk[p + 2 * np] = {(float)c[4 * p], (float)c[4 * p + 1], (float)c[4 * p + 2], (float)c[4 * p + 3]};
+            pk[p + 3 * np] = {(float)d[3 * p], (float)d[3 * p + 1], (float)d[3 * p + 2], 0.f};
+        }
+    }
+    if (pf) {
+        HIP_OK(hipMemcpy(a.data(), s->fcont, a.size() * 8, hipMemcpyDeviceToHost));
+        HIP_OK(hipMemcpy(b.data(), s->tcont, b.size() * 8, hipMemcpyDeviceToHost));
+        for (int p = 0; p < np; p++) {
+            pf[p] = {(float)a[3 * p], (float)a[3 * p + 1], (float)a[3 * p + 2], 0.f};
+            pf[p + np] = {(float)b[3 * p], (float)b[3 * p + 1], (float)b[3 * p + 2], 0.f};
+        }
+    }
+    if (pff) {
+        HIP_OK(hipMemcpy(a.data(), s->ffluid, a.size() * 8, hipMemcpyDeviceToHost));
+        HIP_OK(hipMemcpy(b.data(), s->tfluid, b.size() * 8, hipMemcpyDeviceToHost));
+        const int nfa = s->nfa;
+        for (int p = 0; p < np; p++)
+            for (int fa = 0; fa < nfa; fa++) {
+                pff[p * nfa + fa] = fa == 0 ? d3q19_float4{(float)a[3 * p], (float)a[3 * p + 1], (float)a[3 * p + 2], 0.f} : d3q19_float4{0, 0, 0, 0};
+                pff[(p + np) * nfa + fa] = fa == 0 ? d3q19_float4{(float)b[3 * p], (float)b[3 * p + 1], (float)b[3 * p + 2], 0.f} : d3q19_float4{0, 0, 0, 0};
+            }
+    }
+    return 0;
+}
+
+int d3q19_read_particles_f64(d3q19_sim* s, double* pos, double* vel, double* av, double* ff, double* tf) {
+    CHECK_SIM(s);
+    if (s->np == 0) return 0;
+    HIP_OK(hipStreamSynchronize(s->stream));
+    const size_t b = 3 * (size_t)s->np * 8;
+    if (pos) HIP_OK(hipMemcpy(pos, s->pos, b, hipMemcpyDeviceToHost));
+    if (vel) HIP_OK(hipMemcpy(vel, s->vel, b, hipMemcpyDeviceToHost));
+    if (av) HIP_OK(hipMemcpy(av, s->omega, b, hipMemcpyDeviceToHost));
+    if (ff) HIP_OK(hipMemcpy(ff, s->ffluid, b, hipMemcpyDeviceToHost));
+    if (tf) HIP_OK(hipMemcpy(tf, s->tfluid, b, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int d3q19_collide_stream(d3q19_sim* s, int flags) {
+    CHECK_SIM(s);
+    return do_collide_standalone(s, flags);
+}
+
+int d3q19_particle_dynamics(d3q19_sim* s) {
+    CHECK_SIM(s);
+    return launch_dynamics(s, s->stream);
+}
+
+int d3q19_particle_fluid_forces(d3q19_sim* s) {
+    CHECK_SIM(s);
+    if (s->np == 0) return 0;
+    int rc = launch_mark_bricks(s, s->maskF, s->stream);
+    if (rc) return rc;
+    return launch_ibm(s, s->stream);
+}
+
+int d3q19_particle_particle_forces(d3q19_sim* s) {
+    CHECK_SIM(s);
+    return launch_contacts(s, s->stream);
+}
+
+int d3q19_step(d3q19_sim* s, int t, int flags) {
+    CHECK_SIM(s);
+    (void)t;
+    return enqueue_step(s, flags, s->stream);
+}
+
+int d3q19_run(d3q19_sim* s, int t_first, int n, int flags) {
+    CHECK_SIM(s);
+    if (n <= 0) return 0;
+    int rc;
+    int done = 0;
+    // the first collide after an upload uses the raw wall unknowns: run it directly
+    if (s->bcraw_valid) {
+        if ((rc = enqueue_step(s, n == 1 ? flags : 0, s->stream))) return rc;
+        done = 1;
+    }
+    const int body = n - done - 1;  // the last step carries the flags
+    int pairs = body / 2;
+    if (pairs > 0) {
+        if (!s->graph2 || s->graph_f != cur_f(s) || s->graph_mask != s->maskF) {
+            if ((rc = build_graph(s))) return rc;
+        }
+        for (int i = 0; i < pairs; i++) HIP_OK(hipGraphLaunch(s->graph2, s->stream));
+        done += 2 * pairs;
+    }
+    while (done < n) {
+        if ((rc = enqueue_step(s, done == n - 1 ? flags : 0, s->stream))) return rc;
+        done++;
+    }
+    (void)t_first;
+    return 0;
+}
+
+int d3q19_sync(d3q19_sim* s) {
+    CHECK_SIM(s);
+    HIP_OK(hipStreamSynchronize(s->stream));
+    return 0;
+}
+
+int d3q19_plane_mean_u(d3q19_sim* s, double* means) {
+    CHECK_SIM(s);
+    if (!means) return fail(D3Q19_E_INVALID, "null output");
+    double* d = nullptr;
+    const size_t bytes = 3 * (size_t)s->Nz * 8;
+    int rc = dalloc(&d, bytes);
+    if (rc) return rc;
+    hipError_t e = hipMemsetAsync(d, 0, bytes, s->stream);
+    if (e == hipSuccess) rc = launch_plane_mean(s, d);
+    if (e == hipSuccess && !rc) e = hipMemcpyAsync(means, d, bytes, hipMemcpyDeviceToHost, s->stream);
+    hipError_t e2 = hipStreamSynchronize(s->stream);
+    (void)hipFree(d);
+    if (rc) return rc;
+    if (e != hipSuccess || e2 != hipSuccess) return fail(D3Q19_E_DEVICE, "plane_mean_u copy failed");
+    return 0;
+}
+
+int d3q19_moments(d3q19_sim* s, double* out4) {
+    CHECK_SIM(s);
+    if (!out4) return fail(D3Q19_E_INVALID, "null output");
+    double* d = s->red + 4 * 512;
+    int rc = launch_moments(s, cur_f(s), d);
+    if (rc) return rc;
+    HIP_OK(hipMemcpyAsync(out4, d, 4 * 8, hipMemcpyDeviceToHost, s->stream));
+    HIP_OK(hipStreamSynchronize(s->stream));
+    return 0;
+}
+
+int d3q19_check_finite(d3q19_sim* s) {
+    CHECK_SIM(s);
+    int* flag = nullptr;
+    int rc = dalloc(&flag, sizeof(int));
+    if (rc) return rc;
+    hipError_t e = hipMemsetAsync(flag, 0, sizeof(int), s->stream);
+    if (e == hipSuccess) rc = launch_check_finite(s, cur_f(s), flag);
+    int h = 0;
+    if (e == hipSuccess && !rc) e = hipMemcpyAsync(&h, flag, sizeof(int), hipMemcpyDeviceToHost, s->stream);
+    hipError_t e2 = hipStreamSynchronize(s->stream);
+    (void)hipFree(flag);
+    if (rc) return rc;
+    if (e != hipSuccess || e2 != hipSuccess) return fail(D3Q19_E_DEVICE, "check_finite failed");
+    if (h) return fail(D3Q19_E_NONFINITE, "non-finite or non-positive density in the population field");
+    return 0;
+}
+
+int d3q19_time_collide(d3q19_sim* s, int n, int flags, double* ms) {
+    CHECK_SIM(s);
+    if (n <= 0 || !ms) return fail(D3Q19_E_INVALID, "bad timing request");
+    HIP_OK(hipEventRecord(s->ev0, s->stream));
+    for (int i = 0; i < n; i++) {
+        int rc = launch_collide(s, cur_f(s), nxt_f(s), (flags & D3Q19_STEP_WRITE_U) ? 1 : 0, 1, s->stream);
+        if (rc) return rc;
+        s->parity ^= 1;
+    }
+    HIP_OK(hipEventRecord(s->ev1, s->stream));
+    HIP_OK(hipEventSynchronize(s->ev1));
+    float t = 0;
+    HIP_OK(hipEventElapsedTime(&t, s->ev0, s->ev1));
+    *ms = (double)t / n;
+    return 0;
+}
+
+int d3q19_time_steps(d3q19_sim* s, int t_first, int n, double* total_ms, double* collide_ms) {
+    CHECK_SIM(s);
+    if (n <= 0 || !total_ms) return fail(D3Q19_E_INVALID, "bad timing request");
+    HIP_OK(hipStreamSynchronize(s->stream));
+    HIP_OK(hipEventRecord(s->ev0, s->stream));
+    int rc = d3q19_run(s, t_first, n, 0);
+    if (rc) return rc;
+    HIP_OK(hipEventRecord(s->ev1, s->stream));
+    HIP_OK(hipEventSynchronize(s->ev1));
+    float t = 0;
+    HIP_OK(hipEventElapsedTime(&t, s->ev0, s->ev1));
+    *total_ms = (double)t;
+    if (collide_ms) {
+        // per-launch duration of the collide kernel alone, same stream, same state
+        std::vector<hipEvent_t> evs(2 * 16);
+        for (auto& ev : evs) HIP_OK(hipEventCreate(&ev));
+        double acc = 0;
+        for (int i = 0; i < 16; i++) {
+            HIP_OK(hipEventRecord(evs[2 * i], s->stream));
+            rc = launch_collide(s, cur_f(s), nxt_f(s), 0, 1, s->stream);
+            if (rc) break;
+            s->parity ^= 1;
+            HIP_OK(hipEventRecord(evs[2 * i + 1], s->stream));
+        }
+        HIP_OK(hipStreamSynchronize(s->stream));
+        for (int i = 0; i < 16 && !rc; i++) {
+            float dt = 0;
+            HIP_OK(hipEventElapsedTime(&dt, evs[2 * i], evs[2 * i + 1]));
+            acc += dt;
+        }
+        for (auto& ev : evs) (void)hipEventDestroy(ev);
+        if (rc) return rc;
+        *collide_ms = acc / 16;
+    }
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,180 @@
+// internal.h -- device layout, simulation object and kernel entry points of libd3q19.
+//
+// Device layout (interior nodes only; the reference's one-node buffer shell is not stored):
+//   node n = x + nx*(y + ny*z), 0 <= x < nx = LatticeSize[0]-2, etc.  Lattice index i_1D of the
+//   reference = (x+1) + Nx*((y+1) + Ny*(z+1)).
+//   populations  f[q*qstride + n], two lattices A/B (pull ping-pong).  The state between steps
+//                is the post-collision field; a step gathers f[q](n - c_q) (periodic wrap,
+//                moving-wall unknowns rebuilt from the knowns), collides and writes at n.
+//   u, tau       R[3*nint], R[nint]           (R = float, or double for the fp64 build)
+//   force        int64[3*nint] fixed point (2^-40 units), spread with integer atomics so the
+//                sum is independent of order; consumed and zeroed by the next collide.
+//   masks        uint8 per 4x4x4 brick: where force is pending / where u must be written.
+//   particles    double SoA.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <type_traits>
+#include <stdint.h>
+#include <string>
+#include "d3q19_abi.h"
+
+namespace d3q19 {
+
+constexpr int Q = 19;
+constexpr double FIX_SCALE = 1099511627776.0;       // 2^40
+constexpr double FIX_INV = 1.0 / 1099511627776.0;
+constexpr int BRICK = 4;                              // brick edge (nodes)
+constexpr float FP16_SCALE = 16384.0f;                // 2^14, relative-deviation scale of fp16 storage
+
+// ---- velocity set (reference order, D3Q19-OpenCL.c:9-13) ---------------------------------
+__host__ __device__ constexpr int cx(int q) {
+    return (q == 1 || q == 7 || q == 8 || q == 9 || q == 10) ? 1 : ((q == 2 || q == 11 || q == 12 || q == 13 || q == 14) ? -1 : 0);
+}
+__host__ __device__ constexpr int cy(int q) {
+    return (q == 3 || q == 7 || q == 11 || q == 15 || q == 16) ? 1 : ((q == 4 || q == 8 || q == 12 || q == 17 || q == 18) ? -1 : 0);
+}
+__host__ __device__ constexpr int cz(int q) {
+    return (q == 5 || q == 9 || q == 13 || q == 15 || q == 17) ? 1 : ((q == 6 || q == 10 || q == 14 || q == 16 || q == 18) ? -1 : 0);
+}
+__host__ __device__ constexpr int cc(int q, int a) { return a == 0 ? cx(q) : (a == 1 ? cy(q) : cz(q)); }
+__host__ __device__ constexpr double wq(int q) { return q == 0 ? 1.0 / 3.0 : (q < 7 ? 1.0 / 18.0 : 1.0 / 36.0); }
+
+// opposite direction computed from the velocity components
+__host__ __device__ constexpr int opposite(int q) {
+    int r = 0;
+    for (int p = 0; p < Q; p++)
+        if (cx(p) == -cx(q) && cy(p) == -cy(q) && cz(p) == -cz(q)) r = p;
+    return r;
+}
+
+// ---- d'Humieres moment basis, row order of GPU_program.cl:410-449 -----------------------------
+__host__ __device__ constexpr int mrow(int k, int q) {
+    int x = cx(q), y = cy(q), z = cz(q);
+    int c2 = x * x + y * y + z * z;
+    switch (k) {
+    case 0: return 1;
+    case 1: return 19 * c2 - 30;
+    case 2: return (21 * c2 * c2 - 53 * c2 + 24) / 2;
+    case 3: return x;
+    case 4: return (5 * c2 - 9) * x;
+    case 5: return y;
+    case 6: return (5 * c2 - 9) * y;
+    case 7: return z;
+    case 8: return (5 * c2 - 9) * z;
+    case 9: return 3 * x * x - c2;
+    case 10: return (3 * c2 - 5) * (3 * x * x - c2);
+    case 11: return y * y - z * z;
+    case 12: return (3 * c2 - 5) * (y * y - z * z);
+    case 13: return x * y;
+    case 14: return y * z;
+    case 15: return x * z;
+    case 16: return (y * y - z * z) * x;
+    case 17: return (z * z - x * x) * y;
+    default: return (x * x - y * y) * z;
+    }
+}
+__host__ __device__ constexpr int mnorm(int k) {
+    int s = 0;
+    for (int q = 0; q < Q; q++) s += mrow(k, q) * mrow(k, q);
+    return s;
+}
+
+// ---- lattice geometry passed to kernels --------------------------------------------------------
+struct Geom {
+    int nx, ny, nz;       // interior extents
+    int nint;             // nx*ny*nz
+    long long qstride;    // population stride
+    int per[3];           // 1 periodic, 0 wall
+    int wall;             // wall axis or -1
+    int bnx, bny, bnz;    // brick grid
+    int sys[3];           // SystemSize
+};
+
+struct FluidParams {
+    float gconst[3];
+    float uw[2][3];       // [lower, upper] wall velocity
+    float tau0;           // NewtonianTau
+    int model;            // ViscosityModel
+    float vp[4];          // ViscosityParams
+    float tau_min, tau_max;
+    int calc_rho;
+    int tan_mask;         // bit a: tangential velocity BC on axis a
+};
+
+// ---- simulation object ------------------------------------------------------------------------
+struct Sim {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int_param_struct ip;
+    flp_param_struct fp;
+    d3q19_options opt;
+    Geom g;
+    FluidParams fl;
+    int Nx, Ny, Nz;
+    long long nn;         // reference N_C
+    int storage = 0;
+    size_t pop_bytes = 0; // bytes per population value
+    void* fA = nullptr;
+    void* fB = nullptr;
+    void* u = nullptr;    // R[3*nint]
+    void* tau = nullptr;  // R[nint]
+    long long* facc = nullptr;
+    uint8_t* maskF = nullptr;  // force pending
+    uint8_t* maskU = nullptr;  // u needed by the next IBM pass
+    int nbricks = 0;
+    // raw wall unknowns of an uploaded pre-collision field: [2 sides][5][plane]
+    void* bcraw = nullptr;
+    int bcraw_valid = 0;
+    int bc_plane = 0;
+    // particles
+    int np = 0, ppp = 0, nfa = 1;
+    double* pos = nullptr;     // 3*np
+    double* vel = nullptr;     // 3*np
+    double* quat = nullptr;    // 4*np
+    double* omega = nullptr;   // 3*np
+    double* ffluid = nullptr;  // 3*np
+    double* tfluid = nullptr;  // 3*np
+    double* fcont = nullptr;   // 3*np contact force
+    double* tcont = nullptr;   // 3*np contact torque slot
+    float* pts = nullptr;      // 4*ppp marker offsets
+    double* mforce = nullptr;  // 6*np*ppp per-marker force, torque
+    double* red = nullptr;     // reduction scratch
+    // step graph
+    hipGraphExec_t graph2 = nullptr;   // two steps (ping-pong closes)
+    const void* graph_f = nullptr;     // state the graph was captured from
+    const uint8_t* graph_mask = nullptr;
+    int parity = 0;            // 0: current post-collision field in fA
+    int block[3] = {64, 4, 1};
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+// ---- launchers (defined in the .hip files) --------------------------------------------------------
+int launch_collide(Sim* s, const void* fin, void* fout, int write_u_all, int use_maskU, hipStream_t st);
+int launch_import_f(Sim* s, const void* ref_f, int is_double, void* fdst);
+int launch_export_f(Sim* s, const void* fsrc, void* ref_out, int as_double);
+int launch_import_field(Sim* s, const float* ref, int ncomp, void* dst);
+int launch_export_field(Sim* s, const void* src, int ncomp, float* ref_out);
+int launch_export_force(Sim* s, float* ref_out);
+int launch_plane_mean(Sim* s, double* dev_out);
+int launch_moments(Sim* s, const void* fsrc, double* dev_out);
+int launch_check_finite(Sim* s, const void* fsrc, int* dev_flag);
+int launch_clear_mask(Sim* s, uint8_t* m, hipStream_t st);
+int launch_mark_bricks(Sim* s, uint8_t* m, hipStream_t st);
+int launch_ibm(Sim* s, hipStream_t st);
+int launch_dynamics(Sim* s, hipStream_t st);
+int launch_contacts(Sim* s, hipStream_t st);
+int launch_fill_equilibrium(Sim* s, void* f);
+
+// error reporting
+void set_error(const std::string& msg);
+
+}  // namespace d3q19
+
+#define HIP_OK(expr)                                                                       \
+    do {                                                                                   \
+        hipError_t e_ = (expr);                                                            \
+        if (e_ != hipSuccess) {                                                            \
+            ::d3q19::set_error(std::string(#expr) + ": " + hipGetErrorString(e_));         \
+            return D3Q19_E_DEVICE;                                                         \
+        }                                                                                  \
+    } while (0)

@@ -1,0 +1,763 @@
+// lbm.hip -- fused collide+stream for gfx950 and the lattice-field utility kernels.
+//
+// One thread per interior node.  Pull form: the thread gathers the 19 post-collision
+// populations of its upstream neighbours (periodic axes wrap; on the wet wall layers the
+// unknown populations are rebuilt from the knowns, GPU_program.cl:737-865), adds the pending
+// immersed-boundary force, computes rho and u (GPU_program.cl:343-394), relaxes (MRT with the
+// reference's rates, GPU_program.cl:396-623, or BGK; optional per-node tau from the strain rate,
+// GPU_program.cl:457-564, 1041-1128) and writes its own 19 populations.  The reference's separate
+// periodic copy and wall kernels (GPU_program.cl:626-865) are thereby part of the same pass.
+//
+// Algorithmic HBM traffic per lattice update: 19 loads + 19 stores of the storage type
+// (152 B fp32, 76 B fp16/bf16, 304 B fp64); +8 B when tau is per node; +24 B read (+24 B
+// written) only at nodes of bricks holding pending spread force; +12 B u stores only at nodes
+// the next immersed-boundary pass reads (or everywhere when requested).
+#include "internal.h"
+
+namespace d3q19 {
+
+// ---- population storage formats --------------------------------------------------------------
+__device__ __forceinline__ float bf16_to_f32(uint16_t b) { return __uint_as_float(((uint32_t)b) << 16); }
+__device__ __forceinline__ uint16_t f32_to_bf16(float f) {
+    uint32_t u = __float_as_uint(f);
+    u += 0x7FFFu + ((u >> 16) & 1u);  // round to nearest even (finite inputs)
+    return (uint16_t)(u >> 16);
+}
+
+template <int ST> struct Pop;
+template <> struct Pop<D3Q19_STORE_FP32> {
+    using T = float;
+    using R = float;
+    template <int QI> __device__ static R ld(const T* p, long long off) { return p[off]; }
+    template <int QI> __device__ static void st(T* p, long long off, R v) { p[off] = v; }
+};
+template <> struct Pop<D3Q19_STORE_FP64> {
+    using T = double;
+    using R = double;
+    template <int QI> __device__ static R ld(const T* p, long long off) { return p[off]; }
+    template <int QI> __device__ static void st(T* p, long long off, R v) { p[off] = v; }
+};
+// binary16 of the relative deviation (f_q - w_q)/w_q scaled by 2^14: deviations down to ~1e-8
+// stay normal numbers and the headroom reaches |dev| = 4.
+template <> struct Pop<D3Q19_STORE_FP16_SHIFT> {
+    using T = _Float16;
+    using R = float;
+    template <int QI> __device__ static R ld(const T* p, long long off) {
+        constexpr float w = (float)wq(QI);
+        constexpr float sc = (float)(wq(QI) / 16384.0);
+        return fmaf((float)p[off], sc, w);
+    }
+    template <int QI> __device__ static void st(T* p, long long off, R v) {
+        constexpr float w = (float)wq(QI);
+        constexpr float isc = (float)(16384.0 / wq(QI));
+        p[off] = (_Float16)((v - w) * isc);
+    }
+};
+// bfloat16 of f_q - w_q
+template <> struct Pop<D3Q19_STORE_BF16_SHIFT> {
+    using T = uint16_t;
+    using R = float;
+    template <int QI> __device__ static R ld(const T* p, long long off) {
+        constexpr float w = (float)wq(QI);
+        return w + bf16_to_f32(p[off]);
+    }
+    template <int QI> __device__ static void st(T* p, long long off, R v) {
+        constexpr float w = (float)wq(QI);
+        p[off] = f32_to_bf16(v - w);
+    }
+};
+
+// compile-time loop
+template <int I, int N> struct Unroll {
+    template <class F> __device__ __forceinline__ static void run(F&& f) {
+        f(std::integral_constant<int, I>{});
+        Unroll<I + 1, N>::run(f);
+    }
+};
+template <int N> struct Unroll<N, N> {
+    template <class F> __device__ __forceinline__ static void run(F&&) {}
+};
+
+struct CollideArgs {
+    const void* fin;
+    void* fout;
+    void* u;
+    void* tau;
+    long long* facc;
+    const uint8_t* fmask;
+    const uint8_t* umask;
+    const void* bcraw;
+    Geom g;
+    FluidParams fl;
+    int write_u_all;
+    int use_umask;
+    int use_bcraw;
+    int has_force;
+    int bx, by;            // block tile
+    int nbx, nby;          // tiles per plane
+    int nblk;
+};
+
+// ---- moving-wall unknowns (GPU_program.cl:737-865) -----------------------------------------------
+// Axis A, side S (0: lower wall, inward +A; 1: upper wall, inward -A).  Knowns: populations with
+// zero or outward A-component.  Unknown q = its opposite + rho*(c_q . U)/3 (axis) or /6
+// (diagonal); U = wall velocity with the A-component replaced by the inward normal velocity u_n,
+// obtained from the knowns with rho = 1 (calc_rho = 0, sim_main.c:216) or, with calc_rho = 1,
+// rho from the knowns and the prescribed u_n.  Once after an upload the raw uploaded unknowns
+// are used instead (layout [side][k][plane], k = rank of q among the unknowns).
+// Unknown values are selected, not branched on, so every store to f[] has a fixed index.
+template <int A, int S, class R>
+__device__ __forceinline__ void wall_layer(R* f, const float* uw, int calc_rho, int use_raw, const R* raw, int pidx, int psz) {
+    constexpr int inward = S == 0 ? 1 : -1;
+    constexpr int a1 = A == 0 ? 1 : 0;
+    constexpr int a2 = A == 2 ? 1 : 2;
+    R sum0 = 0, sumout = 0;
+    Unroll<0, Q>::run([&](auto qc) __attribute__((always_inline)) {
+        constexpr int q = decltype(qc)::value;
+        if constexpr (cc(q, A) == 0) sum0 += f[q];
+        if constexpr (cc(q, A) == -inward) sumout += f[q];
+    });
+    R un, rho;
+    if (calc_rho) {
+        un = (R)(S == 0 ? uw[A] : -uw[A]);
+        rho = (sum0 + (R)2 * sumout) / ((R)1 - un);
+    } else {
+        rho = (R)1;
+        un = (R)1 - (sum0 + (R)2 * sumout);
+    }
+    const R ua1 = (R)uw[a1], ua2 = (R)uw[a2];
+    int k = 0;
+    Unroll<0, Q>::run([&](auto qc) __attribute__((always_inline)) {
+        constexpr int q = decltype(qc)::value;
+        if constexpr (cc(q, A) == inward) {
+            constexpr int o = opposite(q);
+            constexpr bool axial = (cc(q, a1) == 0 && cc(q, a2) == 0);
+            const R cu = un + (R)cc(q, a1) * ua1 + (R)cc(q, a2) * ua2;
+            R v = f[o] + rho * cu * (axial ? (R)(1.0 / 3.0) : (R)(1.0 / 6.0));
+            if (use_raw) v = raw[((size_t)(S * 5 + k)) * psz + pidx];
+            f[q] = v;
+            k++;
+        }
+    });
+}
+
+__host__ __device__ inline int plane_size(const Geom& g, int A) {
+    return A == 0 ? g.ny * g.nz : (A == 1 ? g.nx * g.nz : g.nx * g.ny);
+}
+
+// Gather the pre-collision populations of node (x,y,z): pull from the post-collision field with
+// periodic wrap, then the wall unknowns (rebuilt, or the raw uploaded values once after an upload),
+// then the optional tangential-wall passes (sim_main.c:388-404).
+// WALLZ = true: specialisation for the configuration of every shipped input and benchmark
+// (periodic x and y, moving walls normal to z, no tangential passes), which keeps f[] in
+// registers; WALLZ = false handles any wall axis, none, and the tangential passes.
+template <int ST, bool WALLZ>
+__device__ __forceinline__ void gather_pre(const typename Pop<ST>::T* __restrict__ fin, const Geom& g, const FluidParams& fl,
+                                           const void* bcraw, int use_raw, int x, int y, int z,
+                                           typename Pop<ST>::R* f) {
+    using P = Pop<ST>;
+    using R = typename P::R;
+    int xm = x - 1, xp = x + 1, ym = y - 1, yp = y + 1, zm = z - 1, zp = z + 1;
+    if (xm < 0) xm = g.per[0] ? g.nx - 1 : -1;
+    if (xp >= g.nx) xp = g.per[0] ? 0 : -1;
+    if (ym < 0) ym = g.per[1] ? g.ny - 1 : -1;
+    if (yp >= g.ny) yp = g.per[1] ? 0 : -1;
+    if (zm < 0) zm = g.per[2] ? g.nz - 1 : -1;
+    if (zp >= g.nz) zp = g.per[2] ? 0 : -1;
+    const long long qs = g.qstride;
+    Unroll<0, Q>::run([&](auto qc) __attribute__((always_inline)) {
+        constexpr int q = decltype(qc)::value;
+        const int sx = cx(q) == 0 ? x : (cx(q) > 0 ? xm : xp);
+        const int sy = cy(q) == 0 ? y : (cy(q) > 0 ? ym : yp);
+        const int sz = cz(q) == 0 ? z : (cz(q) > 0 ? zm : zp);
+        R v = (R)0;
+        if ((sx | sy | sz) >= 0) v = P::template ld<q>(fin, (long long)q * qs + (sx + g.nx * (sy + g.ny * sz)));
+        f[q] = v;
+    });
+    if constexpr (WALLZ) {
+        if (z == 0) wall_layer<2, 0, R>(f, fl.uw[0], fl.calc_rho, use_raw, (const R*)bcraw, x + g.nx * y, g.nx * g.ny);
+        if (z == g.nz - 1) wall_layer<2, 1, R>(f, fl.uw[1], fl.calc_rho, use_raw, (const R*)bcraw, x + g.nx * y, g.nx * g.ny);
+        return;
+    }
+    const R* rw = (const R*)bcraw;
+    const int cr = fl.calc_rho;
+    if (g.wall == 0) {
+        if (x == 0) wall_layer<0, 0, R>(f, fl.uw[0], cr, use_raw, rw, y + g.ny * z, g.ny * g.nz);
+        if (x == g.nx - 1) wall_layer<0, 1, R>(f, fl.uw[1], cr, use_raw, rw, y + g.ny * z, g.ny * g.nz);
+    }
+    if (g.wall == 1) {
+        if (y == 0) wall_layer<1, 0, R>(f, fl.uw[0], cr, use_raw, rw, x + g.nx * z, g.nx * g.nz);
+        if (y == g.ny - 1) wall_layer<1, 1, R>(f, fl.uw[1], cr, use_raw, rw, x + g.nx * z, g.nx * g.nz);
+    }
+    if (g.wall == 2) {
+        if (z == 0) wall_layer<2, 0, R>(f, fl.uw[0], cr, use_raw, rw, x + g.nx * y, g.nx * g.ny);
+        if (z == g.nz - 1) wall_layer<2, 1, R>(f, fl.uw[1], cr, use_raw, rw, x + g.nx * y, g.nx * g.ny);
+    }
+    // tangential passes in axis order after the wall pass (sim_main.c:388-404); skipped while the
+    // raw uploaded unknowns are in use, since the periodic gather then returns the uploaded values
+    if (fl.tan_mask && !use_raw) {
+        if ((fl.tan_mask & 1) && x == 0) wall_layer<0, 0, R>(f, fl.uw[0], 0, 0, rw, 0, 1);
+        if ((fl.tan_mask & 1) && x == g.nx - 1) wall_layer<0, 1, R>(f, fl.uw[1], 0, 0, rw, 0, 1);
+        if ((fl.tan_mask & 2) && y == 0) wall_layer<1, 0, R>(f, fl.uw[0], 0, 0, rw, 0, 1);
+        if ((fl.tan_mask & 2) && y == g.ny - 1) wall_layer<1, 1, R>(f, fl.uw[1], 0, 0, rw, 0, 1);
+        if ((fl.tan_mask & 4) && z == 0) wall_layer<2, 0, R>(f, fl.uw[0], 0, 0, rw, 0, 1);
+        if ((fl.tan_mask & 4) && z == g.nz - 1) wall_layer<2, 1, R>(f, fl.uw[1], 0, 0, rw, 0, 1);
+    }
+}
+
+// ---- relaxation time law (compute_tau, GPU_program.cl:1041-1128) ----------------------------------
+// Clamp: tau_min (reference 0.505), optional tau_max.  Casson with ViscosityParams[2] = m > 0 is
+// Papanastasiou-regularised: nu = (sqrt(eta) + sqrt(tauY/g)(1 - exp(-sqrt(m g))))^2.
+template <class R>
+__device__ __forceinline__ R tau_law(const FluidParams& fl, R srt) {
+    R tau = (R)fl.tau0;
+    const int model = fl.model;
+    if (model != D3Q19_VISC_POWER_LAW && model != D3Q19_VISC_CASSON && model != D3Q19_VISC_HB) return tau;
+    srt = srt > (R)1e-8 ? srt : (R)1e-8;
+    R nu;
+    if (model == D3Q19_VISC_POWER_LAW) {
+        const R k = (R)fl.vp[0], n = (R)fl.vp[1];
+        if (n == (R)0.5) nu = k / sqrt(srt);
+        else if (n == (R)1.0) nu = k;
+        else if (n == (R)2.0) nu = k * srt;
+        else nu = k * pow(srt, n - (R)1);
+    } else if (model == D3Q19_VISC_CASSON) {
+        const R ty = (R)fl.vp[0], eta = (R)fl.vp[1], m = (R)fl.vp[2];
+        R a = sqrt(ty / srt);
+        if (m > (R)0) a = a * ((R)1 - exp(-sqrt(m * srt)));
+        const R b = a + sqrt(eta);
+        nu = b * b;
+    } else {
+        const R ty = (R)fl.vp[0], k = (R)fl.vp[1], n = (R)fl.vp[2];
+        if (n == (R)0.5) nu = ty / srt + k / sqrt(srt);
+        else if (n == (R)1.0) nu = ty / srt + k;
+        else if (n == (R)2.0) nu = ty / srt + k * srt;
+        else nu = ty / srt + k * pow(srt, n - (R)1);
+    }
+    tau = (R)3 * nu + (R)0.5;
+    if (tau < (R)fl.tau_min) tau = (R)fl.tau_min;
+    if (fl.tau_max > 0.f && tau > (R)fl.tau_max) tau = (R)fl.tau_max;
+    return tau;
+}
+
+// MRT relaxation of v: out = M^-1 S M v with M^-1 = M^T diag(1/|row|^2).
+// Rates of GPU_program.cl:453 with 1/tau on moments 8, 9, 10, 14, 15 (GPU_program.cl:567-568).
+template <class R>
+__device__ __forceinline__ void mrt_relax(const R* v, R itau, R* out) {
+    R a[Q];
+    Unroll<0, Q>::run([&](auto kc) __attribute__((always_inline)) {
+        constexpr int k = decltype(kc)::value;
+        R m = (R)0;
+        Unroll<0, Q>::run([&](auto qc) __attribute__((always_inline)) {
+            constexpr int q = decltype(qc)::value;
+            constexpr int c = mrow(k, q);
+            if constexpr (c == 1) m += v[q];
+            else if constexpr (c == -1) m -= v[q];
+            else if constexpr (c != 0) m += (R)c * v[q];
+        });
+        constexpr double rate = (k == 0) ? 0.0 : (k == 1) ? 1.19 : (k == 2 || k == 12 || k == 13) ? 1.40
+                              : (k == 4 || k == 6 || k == 11) ? 1.20 : (k >= 16) ? 1.98 : 1.0;
+        constexpr bool tk = (k == 8 || k == 9 || k == 10 || k == 14 || k == 15);
+        constexpr double inv_norm = 1.0 / (double)mnorm(k);
+        if constexpr (k == 0) a[k] = (R)0;
+        else if constexpr (tk) a[k] = m * itau * (R)inv_norm;
+        else a[k] = m * (R)(rate * inv_norm);
+    });
+    Unroll<0, Q>::run([&](auto qc) __attribute__((always_inline)) {
+        constexpr int q = decltype(qc)::value;
+        R o = (R)0;
+        Unroll<0, Q>::run([&](auto kc) __attribute__((always_inline)) {
+            constexpr int k = decltype(kc)::value;
+            constexpr int c = mrow(k, q);
+            if constexpr (c == 1) o += a[k];
+            else if constexpr (c == -1) o -= a[k];
+            else if constexpr (c != 0) o += (R)c * a[k];
+        });
+        out[q] = o;
+    });
+}
+
+// strain-rate invariant from the collision change ss (GPU_program.cl:520-557)
+template <class R>
+__device__ __forceinline__ R strain_rate(const R* ss, R ux, R uy, R uz, R gx, R gy, R gz, R inv_rho) {
+    R s[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    Unroll<1, Q>::run([&](auto qc) __attribute__((always_inline)) {
+        constexpr int q = decltype(qc)::value;
+        Unroll<0, 3>::run([&](auto ac) __attribute__((always_inline)) {
+            constexpr int a = decltype(ac)::value;
+            Unroll<a, 3>::run([&](auto bc) __attribute__((always_inline)) {
+                constexpr int b = decltype(bc)::value;
+                constexpr int w = cc(q, a) * cc(q, b);
+                if constexpr (w == 1) s[a][b] += ss[q];
+                else if constexpr (w == -1) s[a][b] -= ss[q];
+            });
+        });
+    });
+    const R u[3] = {ux, uy, uz}, g[3] = {gx, gy, gz};
+    R sum = 0;
+    for (int a = 0; a < 3; a++)
+        for (int b = a; b < 3; b++) {
+            const R v = s[a][b] - (u[a] * g[b] + u[b] * g[a]);
+            sum += (a == b) ? v * v : (R)2 * v * v;
+        }
+    return sqrt(sum) * (R)2.1213203435596424 * inv_rho;
+}
+
+template <int ST, int COLL, int VT, bool WZ>
+__global__ void __launch_bounds__(256) collide_kernel(CollideArgs a) {
+    using P = Pop<ST>;
+    using R = typename P::R;
+    using T = typename P::T;
+    const Geom& g = a.g;
+    // XCD-aware block order: the 8 XCDs receive blocks round-robin, so logical tiles are dealt
+    // out in contiguous runs per XCD to keep the y/z halo rows inside one L2.
+    const int P_ = blockIdx.x;
+    const int qn = a.nblk >> 3, rn = a.nblk & 7;
+    const int xcd = P_ & 7, j = P_ >> 3;
+    const int L = xcd * qn + (xcd < rn ? xcd : rn) + j;
+    const int per_plane = a.nbx * a.nby;
+    const int z = L / per_plane;
+    const int rem = L - z * per_plane;
+    const int tyb = rem / a.nbx;
+    const int txb = rem - tyb * a.nbx;
+    const int x = txb * a.bx + (int)(threadIdx.x % a.bx);
+    const int y = tyb * a.by + (int)(threadIdx.x / a.bx);
+    if (x >= g.nx || y >= g.ny || z >= g.nz) return;
+    const int n = x + g.nx * (y + g.ny * z);
+
+    R f[Q];
+    gather_pre<ST, WZ>((const T*)a.fin, g, a.fl, a.bcraw, a.use_bcraw, x, y, z, f);
+
+    R gx = (R)a.fl.gconst[0], gy = (R)a.fl.gconst[1], gz = (R)a.fl.gconst[2];
+    const int brick = (x >> 2) + g.bnx * ((y >> 2) + g.bny * (z >> 2));
+    if (a.has_force && a.fmask[brick]) {
+        long long* fa = a.facc + n;
+        const long long vx = fa[0], vy = fa[g.nint], vz = fa[2 * (long long)g.nint];
+        if ((vx | vy | vz) != 0) {
+            gx += (R)((double)vx * FIX_INV);
+            gy += (R)((double)vy * FIX_INV);
+            gz += (R)((double)vz * FIX_INV);
+            fa[0] = 0;
+            fa[g.nint] = 0;
+            fa[2 * (long long)g.nint] = 0;
+        }
+    }
+
+    R rho = (R)0;
+    Unroll<0, Q>::run([&](auto qc) __attribute__((always_inline)) { rho += f[decltype(qc)::value]; });
+    const R inv_rho = (R)1 / rho;
+    const R ux = (f[1] - f[2] + f[7] + f[8] + f[9] + f[10] - f[11] - f[12] - f[13] - f[14] + (R)0.5 * gx) * inv_rho;
+    const R uy = (f[3] - f[4] + f[7] - f[8] + f[11] - f[12] + f[15] + f[16] - f[17] - f[18] + (R)0.5 * gy) * inv_rho;
+    const R uz = (f[5] - f[6] + f[9] - f[10] + f[13] - f[14] + f[15] - f[16] + f[17] - f[18] + (R)0.5 * gz) * inv_rho;
+    if (a.write_u_all || (a.use_umask && a.umask[brick])) {
+        R* u = (R*)a.u;
+        u[n] = ux;
+        u[n + g.nint] = uy;
+        u[n + 2 * (long long)g.nint] = uz;
+    }
+
+    // equilibrium (GPU_program.cl:978-1007), non-equilibrium n = feq - f, Guo term (1010-1038)
+    const R usq = ux * ux + uy * uy + uz * uz;
+    const R udg = ux * gx + uy * gy + uz * gz;
+    R ne[Q], fg[Q];
+    Unroll<0, Q>::run([&](auto qc) __attribute__((always_inline)) {
+        constexpr int q = decltype(qc)::value;
+        const R cu = (R)cx(q) * ux + (R)cy(q) * uy + (R)cz(q) * uz;
+        const R cg = (R)cx(q) * gx + (R)cy(q) * gy + (R)cz(q) * gz;
+        R feq;
+        if constexpr (q == 0) {
+            feq = rho * (R)(1.0 / 3.0) * ((R)1 - (R)1.5 * usq);
+            fg[q] = -udg;
+        } else {
+            feq = rho * (R)wq(q) * ((R)1 + (R)3 * cu + (R)4.5 * cu * cu - (R)1.5 * usq);
+            fg[q] = (cg - udg + (R)3 * cu * cg) * (R)(q < 7 ? 1.0 / 6.0 : 1.0 / 12.0);
+        }
+        ne[q] = feq - f[q];
+    });
+
+    R tau = (R)a.fl.tau0;
+    R* taup = (R*)a.tau;
+    if constexpr (VT) {
+        const R told = taup[n];
+        const R itold = (R)1 / told;
+        R ss[Q];
+        if constexpr (COLL == D3Q19_COLL_BGK) {
+            Unroll<0, Q>::run([&](auto qc) __attribute__((always_inline)) {
+                constexpr int q = decltype(qc)::value;
+                ss[q] = ne[q] * itold + fg[q] - (R)0.5 * fg[q] * itold;
+            });
+        } else {
+            R v[Q];
+            Unroll<0, Q>::run([&](auto qc) __attribute__((always_inline)) {
+                constexpr int q = decltype(qc)::value;
+                v[q] = ne[q] - (R)0.5 * fg[q];
+            });
+            mrt_relax<R>(v, itold, ss);
+            Unroll<0, Q>::run([&](auto qc) __attribute__((always_inline)) { ss[decltype(qc)::value] += fg[decltype(qc)::value]; });
+        }
+        const R srt = strain_rate<R>(ss, ux, uy, uz, gx, gy, gz, inv_rho);
+        tau = tau_law<R>(a.fl, srt);
+        taup[n] = tau;
+    }
+    const R itau = (R)1 / tau;
+    R fo[Q];
+    if constexpr (COLL == D3Q19_COLL_BGK) {
+        const R pf = (R)1 - (R)0.5 * itau;
+        Unroll<0, Q>::run([&](auto qc) __attribute__((always_inline)) {
+            constexpr int q = decltype(qc)::value;
+            fo[q] = f[q] + ne[q] * itau + pf * fg[q];
+        });
+    } else {
+        R v[Q], o[Q];
+        Unroll<0, Q>::run([&](auto qc) __attribute__((always_inline)) {
+            constexpr int q = decltype(qc)::value;
+            v[q] = ne[q] - (R)0.5 * fg[q];
+        });
+        mrt_relax<R>(v, itau, o);
+        Unroll<0, Q>::run([&](auto qc) __attribute__((always_inline)) {
+            constexpr int q = decltype(qc)::value;
+            fo[q] = f[q] + o[q] + fg[q];
+        });
+    }
+    T* out = (T*)a.fout;
+    Unroll<0, Q>::run([&](auto qc) __attribute__((always_inline)) {
+        constexpr int q = decltype(qc)::value;
+        P::template st<q>(out, (long long)q * g.qstride + n, fo[q]);
+    });
+}
+
+template <int ST, int COLL, int VT>
+static void launch_one(const CollideArgs& a, int threads, hipStream_t st) {
+    const bool wz = a.g.wall == 2 && a.g.per[0] && a.g.per[1] && a.fl.tan_mask == 0;
+    if (wz) hipLaunchKernelGGL((collide_kernel<ST, COLL, VT, true>), dim3(a.nblk), dim3(threads), 0, st, a);
+    else hipLaunchKernelGGL((collide_kernel<ST, COLL, VT, false>), dim3(a.nblk), dim3(threads), 0, st, a);
+}
+
+template <int ST>
+static void dispatch_coll(const CollideArgs& a, int coll, int vt, int threads, hipStream_t st) {
+    if (coll == D3Q19_COLL_BGK) {
+        if (vt) launch_one<ST, D3Q19_COLL_BGK, 1>(a, threads, st);
+        else launch_one<ST, D3Q19_COLL_BGK, 0>(a, threads, st);
+    } else {
+        if (vt) launch_one<ST, D3Q19_COLL_MRT, 1>(a, threads, st);
+        else launch_one<ST, D3Q19_COLL_MRT, 0>(a, threads, st);
+    }
+}
+
+int launch_collide(Sim* s, const void* fin, void* fout, int write_u_all, int use_maskU, hipStream_t st) {
+    CollideArgs a;
+    a.fin = fin;
+    a.fout = fout;
+    a.u = s->u;
+    a.tau = s->tau;
+    a.facc = s->facc;
+    a.fmask = s->maskF;
+    a.umask = s->maskU;
+    a.bcraw = s->bcraw;
+    a.g = s->g;
+    a.fl = s->fl;
+    a.write_u_all = write_u_all;
+    a.use_umask = use_maskU;
+    a.use_bcraw = s->bcraw_valid;
+    a.has_force = s->np > 0 ? 1 : 0;
+    a.bx = s->block[0];
+    a.by = s->block[1];
+    a.nbx = (s->g.nx + a.bx - 1) / a.bx;
+    a.nby = (s->g.ny + a.by - 1) / a.by;
+    a.nblk = a.nbx * a.nby * s->g.nz;
+    const int threads = a.bx * a.by;
+    switch (s->storage) {
+    case D3Q19_STORE_FP16_SHIFT: dispatch_coll<D3Q19_STORE_FP16_SHIFT>(a, s->opt.collision, s->opt.variable_viscosity, threads, st); break;
+    case D3Q19_STORE_BF16_SHIFT: dispatch_coll<D3Q19_STORE_BF16_SHIFT>(a, s->opt.collision, s->opt.variable_viscosity, threads, st); break;
+    case D3Q19_STORE_FP64: dispatch_coll<D3Q19_STORE_FP64>(a, s->opt.collision, s->opt.variable_viscosity, threads, st); break;
+    default: dispatch_coll<D3Q19_STORE_FP32>(a, s->opt.collision, s->opt.variable_viscosity, threads, st); break;
+    }
+    HIP_OK(hipGetLastError());
+    s->bcraw_valid = 0;
+    return 0;
+}
+
+// ---- import / export of the reference-layout population field ------------------------------------
+// import: the pre-collision field f_pre (reference layout, shell included) becomes the stored
+// post-collision field so that the next gather reproduces it: f_post[q](r) = f_pre[q](r + c_q)
+// (periodic wrap).  Wall unknowns of f_pre cannot be represented that way; they are kept in the
+// raw side buffer and used by the next gather instead of the rebuilt values.
+template <int ST, class IN>
+__global__ void import_kernel(const IN* __restrict__ ref, Geom g, int Nx, int Ny, long long nn,
+                              typename Pop<ST>::T* __restrict__ dst, typename Pop<ST>::R* raw) {
+    using P = Pop<ST>;
+    using R = typename P::R;
+    const long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (idx >= g.nint) return;
+    const int x = (int)(idx % g.nx);
+    const int y = (int)((idx / g.nx) % g.ny);
+    const int z = (int)(idx / ((long long)g.nx * g.ny));
+    Unroll<0, Q>::run([&](auto qc) __attribute__((always_inline)) {
+        constexpr int q = decltype(qc)::value;
+        int s3[3] = {x + cx(q), y + cy(q), z + cz(q)};
+        const int nd[3] = {g.nx, g.ny, g.nz};
+        bool ok = true;
+        for (int d = 0; d < 3; d++) {
+            if (s3[d] < 0 || s3[d] >= nd[d]) {
+                if (g.per[d]) s3[d] = (s3[d] + nd[d]) % nd[d];
+                else ok = false;
+            }
+        }
+        if (!ok) { s3[0] = x; s3[1] = y; s3[2] = z; }
+        const long long ri = (s3[0] + 1) + (long long)Nx * ((s3[1] + 1) + (long long)Ny * (s3[2] + 1));
+        P::template st<q>(dst, (long long)q * g.qstride + idx, (R)ref[q * nn + ri]);
+    });
+    if (g.wall >= 0 && raw) {
+        const int A = g.wall;
+        const int c = A == 0 ? x : (A == 1 ? y : z);
+        const int n = A == 0 ? g.nx : (A == 1 ? g.ny : g.nz);
+        const long long ri = (x + 1) + (long long)Nx * ((y + 1) + (long long)Ny * (z + 1));
+        const int psz = plane_size(g, A);
+        const int pidx = A == 0 ? y + g.ny * z : (A == 1 ? x + g.nx * z : x + g.nx * y);
+        for (int S = 0; S < 2; S++) {
+            if (!((S == 0 && c == 0) || (S == 1 && c == n - 1))) continue;
+            const int inward = S == 0 ? 1 : -1;
+            int k = 0;
+            for (int q = 1; q < Q; q++) {
+                if (cc(q, A) != inward) continue;
+                raw[((size_t)(S * 5 + k)) * psz + pidx] = (R)ref[q * nn + ri];
+                k++;
+            }
+        }
+    }
+}
+
+template <int ST, class OUT>
+__global__ void export_kernel(const typename Pop<ST>::T* __restrict__ fin, Geom g, FluidParams fl, const void* raw, int use_raw,
+                              int Nx, int Ny, long long nn, OUT* __restrict__ ref) {
+    using R = typename Pop<ST>::R;
+    const long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (idx >= g.nint) return;
+    const int x = (int)(idx % g.nx);
+    const int y = (int)((idx / g.nx) % g.ny);
+    const int z = (int)(idx / ((long long)g.nx * g.ny));
+    R f[Q];
+    gather_pre<ST, false>(fin, g, fl, raw, use_raw, x, y, z, f);
+    const long long ri = (x + 1) + (long long)Nx * ((y + 1) + (long long)Ny * (z + 1));
+    Unroll<0, Q>::run([&](auto qc) __attribute__((always_inline)) { ref[decltype(qc)::value * nn + ri] = (OUT)f[decltype(qc)::value]; });
+}
+
+template <int ST>
+static void import_st(Sim* s, const void* ref, int is_double, void* dst) {
+    using R = typename Pop<ST>::R;
+    const int th = 256;
+    const int blocks = (int)((s->g.nint + th - 1) / th);
+    R* raw = (R*)s->bcraw;
+    if (is_double)
+        hipLaunchKernelGGL((import_kernel<ST, double>), dim3(blocks), dim3(th), 0, s->stream, (const double*)ref, s->g, s->Nx, s->Ny, s->nn,
+                           (typename Pop<ST>::T*)dst, raw);
+    else
+        hipLaunchKernelGGL((import_kernel<ST, float>), dim3(blocks), dim3(th), 0, s->stream, (const float*)ref, s->g, s->Nx, s->Ny, s->nn,
+                           (typename Pop<ST>::T*)dst, raw);
+}
+
+int launch_import_f(Sim* s, const void* ref, int is_double, void* dst) {
+    switch (s->storage) {
+    case D3Q19_STORE_FP16_SHIFT: import_st<D3Q19_STORE_FP16_SHIFT>(s, ref, is_double, dst); break;
+    case D3Q19_STORE_BF16_SHIFT: import_st<D3Q19_STORE_BF16_SHIFT>(s, ref, is_double, dst); break;
+    case D3Q19_STORE_FP64: import_st<D3Q19_STORE_FP64>(s, ref, is_double, dst); break;
+    default: import_st<D3Q19_STORE_FP32>(s, ref, is_double, dst); break;
+    }
+    HIP_OK(hipGetLastError());
+    s->bcraw_valid = (s->g.wall >= 0) ? 1 : 0;
+    return 0;
+}
+
+template <int ST>
+static void export_st(Sim* s, const void* fsrc, void* ref, int as_double) {
+    const int th = 256;
+    const int blocks = (int)((s->g.nint + th - 1) / th);
+    if (as_double)
+        hipLaunchKernelGGL((export_kernel<ST, double>), dim3(blocks), dim3(th), 0, s->stream, (const typename Pop<ST>::T*)fsrc, s->g, s->fl,
+                           s->bcraw, s->bcraw_valid, s->Nx, s->Ny, s->nn, (double*)ref);
+    else
+        hipLaunchKernelGGL((export_kernel<ST, float>), dim3(blocks), dim3(th), 0, s->stream, (const typename Pop<ST>::T*)fsrc, s->g, s->fl,
+                           s->bcraw, s->bcraw_valid, s->Nx, s->Ny, s->nn, (float*)ref);
+}
+
+int launch_export_f(Sim* s, const void* fsrc, void* ref, int as_double) {
+    switch (s->storage) {
+    case D3Q19_STORE_FP16_SHIFT: export_st<D3Q19_STORE_FP16_SHIFT>(s, fsrc, ref, as_double); break;
+    case D3Q19_STORE_BF16_SHIFT: export_st<D3Q19_STORE_BF16_SHIFT>(s, fsrc, ref, as_double); break;
+    case D3Q19_STORE_FP64: export_st<D3Q19_STORE_FP64>(s, fsrc, ref, as_double); break;
+    default: export_st<D3Q19_STORE_FP32>(s, fsrc, ref, as_double); break;
+    }
+    HIP_OK(hipGetLastError());
+    return 0;
+}
+
+// ---- scalar/vector fields between reference layout (float) and the interior layout ---------------
+template <class R>
+__global__ void field_in_kernel(const float* __restrict__ ref, Geom g, int Nx, int Ny, long long nn, int ncomp, R* __restrict__ dst) {
+    const long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (idx >= g.nint) return;
+    const int x = (int)(idx % g.nx), y = (int)((idx / g.nx) % g.ny), z = (int)(idx / ((long long)g.nx * g.ny));
+    const long long ri = (x + 1) + (long long)Nx * ((y + 1) + (long long)Ny * (z + 1));
+    for (int c = 0; c < ncomp; c++) dst[c * (long long)g.nint + idx] = (R)ref[c * nn + ri];
+}
+template <class R>
+__global__ void field_out_kernel(const R* __restrict__ src, Geom g, int Nx, int Ny, long long nn, int ncomp, float* __restrict__ ref) {
+    const long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (idx >= g.nint) return;
+    const int x = (int)(idx % g.nx), y = (int)((idx / g.nx) % g.ny), z = (int)(idx / ((long long)g.nx * g.ny));
+    const long long ri = (x + 1) + (long long)Nx * ((y + 1) + (long long)Ny * (z + 1));
+    for (int c = 0; c < ncomp; c++) ref[c * nn + ri] = (float)src[c * (long long)g.nint + idx];
+}
+__global__ void force_out_kernel(const long long* __restrict__ facc, Geom g, int Nx, int Ny, long long nn, float* __restrict__ ref) {
+    const long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (idx >= g.nint) return;
+    const int x = (int)(idx % g.nx), y = (int)((idx / g.nx) % g.ny), z = (int)(idx / ((long long)g.nx * g.ny));
+    const long long ri = (x + 1) + (long long)Nx * ((y + 1) + (long long)Ny * (z + 1));
+    for (int c = 0; c < 3; c++) ref[c * nn + ri] = (float)((double)facc[c * (long long)g.nint + idx] * FIX_INV);
+}
+
+int launch_import_field(Sim* s, const float* ref, int ncomp, void* dst) {
+    const int th = 256, blocks = (int)((s->g.nint + th - 1) / th);
+    if (s->storage == D3Q19_STORE_FP64)
+        hipLaunchKernelGGL(field_in_kernel<double>, dim3(blocks), dim3(th), 0, s->stream, ref, s->g, s->Nx, s->Ny, s->nn, ncomp, (double*)dst);
+    else
+        hipLaunchKernelGGL(field_in_kernel<float>, dim3(blocks), dim3(th), 0, s->stream, ref, s->g, s->Nx, s->Ny, s->nn, ncomp, (float*)dst);
+    HIP_OK(hipGetLastError());
+    return 0;
+}
+int launch_export_field(Sim* s, const void* src, int ncomp, float* ref) {
+    const int th = 256, blocks = (int)((s->g.nint + th - 1) / th);
+    if (s->storage == D3Q19_STORE_FP64)
+        hipLaunchKernelGGL(field_out_kernel<double>, dim3(blocks), dim3(th), 0, s->stream, (const double*)src, s->g, s->Nx, s->Ny, s->nn, ncomp, ref);
+    else
+        hipLaunchKernelGGL(field_out_kernel<float>, dim3(blocks), dim3(th), 0, s->stream, (const float*)src, s->g, s->Nx, s->Ny, s->nn, ncomp, ref);
+    HIP_OK(hipGetLastError());
+    return 0;
+}
+int launch_export_force(Sim* s, float* ref) {
+    const int th = 256, blocks = (int)((s->g.nint + th - 1) / th);
+    hipLaunchKernelGGL(force_out_kernel, dim3(blocks), dim3(th), 0, s->stream, s->facc, s->g, s->Nx, s->Ny, s->nn, ref);
+    HIP_OK(hipGetLastError());
+    return 0;
+}
+
+// ---- initial field ---------------------------------------------------------------------------------
+template <int ST>
+__global__ void fill_eq_kernel(typename Pop<ST>::T* f, Geom g) {
+    using P = Pop<ST>;
+    using R = typename P::R;
+    const long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (idx >= g.nint) return;
+    Unroll<0, Q>::run([&](auto qc) __attribute__((always_inline)) {
+        constexpr int q = decltype(qc)::value;
+        P::template st<q>(f, (long long)q * g.qstride + idx, (R)wq(q));
+    });
+}
+int launch_fill_equilibrium(Sim* s, void* f) {
+    const int th = 256, blocks = (int)((s->g.nint + th - 1) / th);
+    switch (s->storage) {
+    case D3Q19_STORE_FP16_SHIFT: hipLaunchKernelGGL(fill_eq_kernel<D3Q19_STORE_FP16_SHIFT>, dim3(blocks), dim3(th), 0, s->stream, (_Float16*)f, s->g); break;
+    case D3Q19_STORE_BF16_SHIFT: hipLaunchKernelGGL(fill_eq_kernel<D3Q19_STORE_BF16_SHIFT>, dim3(blocks), dim3(th), 0, s->stream, (uint16_t*)f, s->g); break;
+    case D3Q19_STORE_FP64: hipLaunchKernelGGL(fill_eq_kernel<D3Q19_STORE_FP64>, dim3(blocks), dim3(th), 0, s->stream, (double*)f, s->g); break;
+    default: hipLaunchKernelGGL(fill_eq_kernel<D3Q19_STORE_FP32>, dim3(blocks), dim3(th), 0, s->stream, (float*)f, s->g); break;
+    }
+    HIP_OK(hipGetLastError());
+    return 0;
+}
+
+// ---- observables ------------------------------------------------------------------------------------
+// plane means of u over each interior z plane, FP64, one block per plane, fixed reduction order.
+template <class R>
+__global__ void plane_mean_kernel(const R* __restrict__ u, Geom g, double* out, int Nz) {
+    const int z = blockIdx.x;
+    __shared__ double sh[3][256];
+    double acc[3] = {0, 0, 0};
+    const int plane = g.nx * g.ny;
+    for (int i = threadIdx.x; i < plane; i += blockDim.x) {
+        const long long n = (long long)z * plane + i;
+        for (int c = 0; c < 3; c++) acc[c] += (double)u[c * (long long)g.nint + n];
+    }
+    for (int c = 0; c < 3; c++) sh[c][threadIdx.x] = acc[c];
+    __syncthreads();
+    for (int st = blockDim.x / 2; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st)
+            for (int c = 0; c < 3; c++) sh[c][threadIdx.x] += sh[c][threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int c = 0; c < 3; c++) out[c * Nz + z + 1] = sh[c][0] / (double)plane;
+}
+int launch_plane_mean(Sim* s, double* dout) {
+    if (s->storage == D3Q19_STORE_FP64)
+        hipLaunchKernelGGL(plane_mean_kernel<double>, dim3(s->g.nz), dim3(256), 0, s->stream, (const double*)s->u, s->g, dout, s->Nz);
+    else
+        hipLaunchKernelGGL(plane_mean_kernel<float>, dim3(s->g.nz), dim3(256), 0, s->stream, (const float*)s->u, s->g, dout, s->Nz);
+    HIP_OK(hipGetLastError());
+    return 0;
+}
+
+// total mass and momentum of the pre-collision field; block partials then one fixed-order pass
+template <int ST>
+__global__ void moments_kernel(const typename Pop<ST>::T* __restrict__ fin, Geom g, FluidParams fl, const void* raw, int use_raw,
+                               double* partial, int* bad) {
+    using R = typename Pop<ST>::R;
+    __shared__ double sh[4][256];
+    double acc[4] = {0, 0, 0, 0};
+    int badl = 0;
+    for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < g.nint; idx += (long long)gridDim.x * blockDim.x) {
+        const int x = (int)(idx % g.nx), y = (int)((idx / g.nx) % g.ny), z = (int)(idx / ((long long)g.nx * g.ny));
+        R f[Q];
+        gather_pre<ST, false>(fin, g, fl, raw, use_raw, x, y, z, f);
+        double r = 0, jx = 0, jy = 0, jz = 0;
+        Unroll<0, Q>::run([&](auto qc) __attribute__((always_inline)) {
+            constexpr int q = decltype(qc)::value;
+            r += (double)f[q];
+            jx += cx(q) * (double)f[q];
+            jy += cy(q) * (double)f[q];
+            jz += cz(q) * (double)f[q];
+        });
+        if (!(r > 0.0) || !isfinite(r) || !isfinite(jx) || !isfinite(jy) || !isfinite(jz)) badl = 1;
+        acc[0] += r; acc[1] += jx; acc[2] += jy; acc[3] += jz;
+    }
+    if (badl && bad) atomicOr(bad, 1);
+    for (int c = 0; c < 4; c++) sh[c][threadIdx.x] = acc[c];
+    __syncthreads();
+    for (int st = blockDim.x / 2; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st)
+            for (int c = 0; c < 4; c++) sh[c][threadIdx.x] += sh[c][threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int c = 0; c < 4; c++) partial[4 * blockIdx.x + c] = sh[c][0];
+}
+__global__ void sum_partials_kernel(const double* partial, int nb, double* out) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        double a[4] = {0, 0, 0, 0};
+        for (int b = 0; b < nb; b++)
+            for (int c = 0; c < 4; c++) a[c] += partial[4 * b + c];
+        for (int c = 0; c < 4; c++) out[c] = a[c];
+    }
+}
+
+template <int ST>
+static void moments_st(Sim* s, const void* fsrc, double* partial, int nb, int* bad) {
+    hipLaunchKernelGGL((moments_kernel<ST>), dim3(nb), dim3(256), 0, s->stream, (const typename Pop<ST>::T*)fsrc, s->g, s->fl, s->bcraw,
+                       s->bcraw_valid, partial, bad);
+}
+static int moments_any(Sim* s, const void* fsrc, double* dout, int* bad) {
+    const int nb = 512;
+    double* partial = s->red;  // 4*512 doubles + 4
+    switch (s->storage) {
+    case D3Q19_STORE_FP16_SHIFT: moments_st<D3Q19_STORE_FP16_SHIFT>(s, fsrc, partial, nb, bad); break;
+    case D3Q19_STORE_BF16_SHIFT: moments_st<D3Q19_STORE_BF16_SHIFT>(s, fsrc, partial, nb, bad); break;
+    case D3Q19_STORE_FP64: moments_st<D3Q19_STORE_FP64>(s, fsrc, partial, nb, bad); break;
+    default: moments_st<D3Q19_STORE_FP32>(s, fsrc, partial, nb, bad); break;
+    }
+    if (dout) hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(64), 0, s->stream, partial, nb, dout);
+    HIP_OK(hipGetLastError());
+    return 0;
+}
+int launch_moments(Sim* s, const void* fsrc, double* dout) { return moments_any(s, fsrc, dout, nullptr); }
+int launch_check_finite(Sim* s, const void* fsrc, int* flag) { return moments_any(s, fsrc, nullptr, flag); }
+
+}  // namespace d3q19
