@@ -50,9 +50,17 @@ __global__ void __launch_bounds__(256) ibm_marker_kernel(IbmArgs a) {
     const double r0[3] = {(double)a.pts[4 * m], (double)a.pts[4 * m + 1], (double)a.pts[4 * m + 2]};
     const int nd[3] = {g.nx, g.ny, g.nz};
     double rpp[3];
+    bool finite = true;
     for (int d = 0; d < 3; d++) {
         const double L = (double)g.sys[d];
         rpp[d] = fmod((a.pos[3 * p + d] + r0[d]) + L, L);  // GPU_program.cl:141-144
+        if (!(rpp[d] >= 0.0 && rpp[d] < L)) finite = false;  // NaN or out of range
+    }
+    if (!finite) {
+        // a sphere whose state is no longer finite contributes nothing (the watchdog reports it)
+        double* mf = a.mforce + 6 * (long long)gid;
+        for (int k = 0; k < 6; k++) mf[k] = 0.0 / 0.0;
+        return;
     }
     const R* u = (const R*)a.u;
     const long long nint = g.nint;

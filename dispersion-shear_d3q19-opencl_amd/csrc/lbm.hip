@@ -457,9 +457,10 @@ int launch_collide(Sim* s, const void* fin, void* fout, int write_u_all, int use
     a.g = s->g;
     a.fl = s->fl;
     a.write_u_all = write_u_all;
-    a.use_umask = use_maskU;
+    // the brick masks and the force accumulator exist only when there are spheres
+    a.use_umask = (use_maskU && s->maskU != nullptr) ? 1 : 0;
     a.use_bcraw = s->bcraw_valid;
-    a.has_force = s->np > 0 ? 1 : 0;
+    a.has_force = (s->np > 0 && s->maskF != nullptr && s->facc != nullptr) ? 1 : 0;
     a.bx = s->block[0];
     a.by = s->block[1];
     a.nbx = (s->g.nx + a.bx - 1) / a.bx;
