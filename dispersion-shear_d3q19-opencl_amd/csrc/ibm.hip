@@ -24,6 +24,7 @@ struct IbmArgs {
     const void* u;
     long long* facc;
     double* mforce;
+    uint32_t* epochs;
 };
 
 __device__ __forceinline__ double peskin4(double r) {
@@ -41,10 +42,27 @@ __device__ __forceinline__ void spread_add(long long* facc, long long nint, int 
     }
 }
 
+__device__ __forceinline__ double wave_sum(double v) {
+    // fixed butterfly order over the 64 lanes: deterministic
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// One 64-lane wavefront per marker; lane k = (kx, ky, kz) in [0,4)^3 owns one node of the
+// marker's 4x4x4 support.  Interpolation is a wave reduction of the lanes' weighted velocities;
+// spreading is three integer atomics per lane.
 template <class R, int MODE>
 __global__ void __launch_bounds__(256) ibm_marker_kernel(IbmArgs a) {
-    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
-    if (gid >= a.np * a.ppp) return;
+    const int lane = threadIdx.x & 63;
+    const int gid = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (gid >= a.np * a.ppp) return;  // uniform per wavefront
+    if (gid == 0 && lane == 0) {
+        // the step's epoch (read as epochs[0] + 1 by the kernels before this one) is committed,
+        // and its spread force is now the pending one
+        const uint32_t e = a.epochs[0] + 1u;
+        a.epochs[0] = e;
+        a.epochs[1] = e;
+    }
     const Geom& g = a.g;
     const int p = gid / a.ppp, m = gid - p * a.ppp;
     const double r0[3] = {(double)a.pts[4 * m], (double)a.pts[4 * m + 1], (double)a.pts[4 * m + 2]};
@@ -58,202 +76,87 @@ __global__ void __launch_bounds__(256) ibm_marker_kernel(IbmArgs a) {
     }
     if (!finite) {
         // a sphere whose state is no longer finite contributes nothing (the watchdog reports it)
-        double* mf = a.mforce + 6 * (long long)gid;
-        for (int k = 0; k < 6; k++) mf[k] = 0.0 / 0.0;
+        if (lane < 6) a.mforce[6 * (long long)gid + lane] = 0.0 / 0.0;
         return;
+    }
+    const int kk[3] = {lane & 3, (lane >> 2) & 3, lane >> 4};
+    int sn[3];       // spread node of this lane, per axis
+    double sw[3];    // spread weight factor per axis
+    bool sok = true;
+    double iw = 0.0; // interpolation weight of this lane (product over axes)
+    int inode = 0;
+    bool iok;
+    if (MODE == D3Q19_IBM_LINEAR_SMOOTHED) {
+        // interpolation: trilinear corners (lanes with kx,ky,kz in {0,1}), GPU_program.cl:150-196
+        // spreading: trilinear pair convolved with the (1/4,1/2,1/4) smoothing, interior targets
+        iok = (kk[0] < 2 && kk[1] < 2 && kk[2] < 2);
+        double wi = 1.0;
+        int ic[3];
+        for (int d = 0; d < 3; d++) {
+            const double fl = floor(rpp[d]);
+            const int ia = (int)fl;
+            const int ib = (ia == nd[d] - 1) ? 0 : ia + 1;  // GPU_program.cl:160-162
+            const double A = 1.0 - (rpp[d] - fl), B = 1.0 - A;
+            const int k = kk[d];
+            ic[d] = (k & 1) ? ib : ia;
+            wi *= (k & 1) ? B : A;
+            int node;
+            double w;
+            if (ib == ia + 1) {
+                node = ia - 1 + k;
+                w = k == 0 ? 0.25 * A : (k == 1 ? 0.5 * A + 0.25 * B : (k == 2 ? 0.25 * A + 0.5 * B : 0.25 * B));
+            } else {
+                node = k == 0 ? ia - 1 : (k == 1 ? ia : (k == 2 ? ib : ib + 1));
+                w = k == 0 ? 0.25 * A : (k == 1 ? 0.5 * A : (k == 2 ? 0.5 * B : 0.25 * B));
+            }
+            sn[d] = node;
+            sw[d] = w;
+            if (node < 0 || node >= nd[d]) sok = false;
+        }
+        iw = iok ? wi : 0.0;
+        inode = ic[0] + g.nx * (ic[1] + g.ny * ic[2]);
+    } else {
+        double w3 = 1.0;
+        for (int d = 0; d < 3; d++) {
+            const int base = (int)floor(rpp[d]);
+            int node = base - 1 + kk[d];
+            const double w = peskin4(rpp[d] - (double)node);
+            if (g.per[d]) node = ((node % nd[d]) + nd[d]) % nd[d];
+            else if (node < 0 || node >= nd[d]) sok = false;
+            sn[d] = node;
+            sw[d] = w;
+            w3 *= w;
+        }
+        iok = sok;
+        iw = sok ? w3 : 0.0;
+        inode = sok ? sn[0] + g.nx * (sn[1] + g.ny * sn[2]) : 0;
     }
     const R* u = (const R*)a.u;
     const long long nint = g.nint;
-    double up[3] = {0, 0, 0};
-    double F[3], T[3];
-    if (MODE == D3Q19_IBM_LINEAR_SMOOTHED) {
-        int ia[3], ib[3];
-        double wa[3];
-        for (int d = 0; d < 3; d++) {
-            const double fl = floor(rpp[d]);
-            ia[d] = (int)fl;
-            ib[d] = (ia[d] == nd[d] - 1) ? 0 : ia[d] + 1;  // GPU_program.cl:160-162
-            wa[d] = 1.0 - (rpp[d] - fl);
-        }
-        // trilinear interpolation, corner order of GPU_program.cl:164, 171-178
-        for (int c = 0; c < 8; c++) {
-            const int bx = (c == 1 || c == 4 || c == 5 || c == 7), by = (c == 2 || c == 4 || c == 6 || c == 7),
-                      bz = (c == 3 || c == 5 || c == 6 || c == 7);
-            const double w = (bx ? 1.0 - wa[0] : wa[0]) * (by ? 1.0 - wa[1] : wa[1]) * (bz ? 1.0 - wa[2] : wa[2]);
-            const int node = (bx ? ib[0] : ia[0]) + g.nx * ((by ? ib[1] : ia[1]) + g.ny * (bz ? ib[2] : ia[2]));
-            for (int k = 0; k < 3; k++) up[k] += w * (double)u[k * nint + node];
-        }
-        double wr[3];
-        const double* om = a.omega + 3 * p;
-        wr[0] = om[1] * r0[2] - om[2] * r0[1];
-        wr[1] = om[2] * r0[0] - om[0] * r0[2];
-        wr[2] = om[0] * r0[1] - om[1] * r0[0];
-        for (int k = 0; k < 3; k++) F[k] = (up[k] - (a.vel[3 * p + k] + wr[k])) * a.dA;
-        // spread targets per axis: trilinear pair convolved with (1/4,1/2,1/4), interior only
-        int ti[3][4];
-        double tw[3][4];
-        int tn[3];
-        for (int d = 0; d < 3; d++) {
-            const double A = wa[d], B = 1.0 - wa[d];
-            int cnt = 0;
-            if (ib[d] == ia[d] + 1) {
-                const int base = ia[d] - 1;
-                const double w4[4] = {0.25 * A, 0.5 * A + 0.25 * B, 0.25 * A + 0.5 * B, 0.25 * B};
-                for (int k = 0; k < 4; k++) {
-                    const int node = base + k;
-                    if (node >= 0 && node < nd[d]) { ti[d][cnt] = node; tw[d][cnt] = w4[k]; cnt++; }
-                }
-            } else {
-                const int nodes[4] = {ia[d] - 1, ia[d], ib[d], ib[d] + 1};
-                const double w4[4] = {0.25 * A, 0.5 * A, 0.5 * B, 0.25 * B};
-                for (int k = 0; k < 4; k++)
-                    if (nodes[k] >= 0 && nodes[k] < nd[d]) { ti[d][cnt] = nodes[k]; tw[d][cnt] = w4[k]; cnt++; }
-            }
-            tn[d] = cnt;
-        }
-        for (int kz = 0; kz < tn[2]; kz++)
-            for (int ky = 0; ky < tn[1]; ky++)
-                for (int kx = 0; kx < tn[0]; kx++) {
-                    const int node = ti[0][kx] + g.nx * (ti[1][ky] + g.ny * ti[2][kz]);
-                    spread_add(a.facc, nint, node, tw[0][kx] * tw[1][ky] * tw[2][kz], F);
-                }
-    } else {
-        int ti[3][4], ok[3][4];
-        double tw[3][4];
-        for (int d = 0; d < 3; d++) {
-            const int base = (int)floor(rpp[d]);
-            for (int k = 0; k < 4; k++) {
-                int node = base - 1 + k;
-                tw[d][k] = peskin4(rpp[d] - (double)node);
-                if (g.per[d]) {
-                    node = ((node % nd[d]) + nd[d]) % nd[d];
-                    ok[d][k] = 1;
-                } else {
-                    ok[d][k] = (node >= 0 && node < nd[d]);
-                }
-                ti[d][k] = node;
-            }
-        }
-        for (int kz = 0; kz < 4; kz++)
-            for (int ky = 0; ky < 4; ky++)
-                for (int kx = 0; kx < 4; kx++) {
-                    if (!(ok[0][kx] && ok[1][ky] && ok[2][kz])) continue;
-                    const double w = tw[0][kx] * tw[1][ky] * tw[2][kz];
-                    const int node = ti[0][kx] + g.nx * (ti[1][ky] + g.ny * ti[2][kz]);
-                    for (int c = 0; c < 3; c++) up[c] += w * (double)u[c * nint + node];
-                }
-        double wr[3];
-        const double* om = a.omega + 3 * p;
-        wr[0] = om[1] * r0[2] - om[2] * r0[1];
-        wr[1] = om[2] * r0[0] - om[0] * r0[2];
-        wr[2] = om[0] * r0[1] - om[1] * r0[0];
-        for (int k = 0; k < 3; k++) F[k] = (up[k] - (a.vel[3 * p + k] + wr[k])) * a.dA;
-        for (int kz = 0; kz < 4; kz++)
-            for (int ky = 0; ky < 4; ky++)
-                for (int kx = 0; kx < 4; kx++) {
-                    if (!(ok[0][kx] && ok[1][ky] && ok[2][kz])) continue;
-                    const int node = ti[0][kx] + g.nx * (ti[1][ky] + g.ny * ti[2][kz]);
-                    spread_add(a.facc, nint, node, tw[0][kx] * tw[1][ky] * tw[2][kz], F);
-                }
+    double c0 = 0, c1 = 0, c2 = 0;
+    if (iok) {
+        c0 = iw * (double)u[inode];
+        c1 = iw * (double)u[nint + inode];
+        c2 = iw * (double)u[2 * nint + inode];
     }
-    T[0] = r0[1] * F[2] - r0[2] * F[1];
-    T[1] = r0[2] * F[0] - r0[0] * F[2];
-    T[2] = r0[0] * F[1] - r0[1] * F[0];
-    double* mf = a.mforce + 6 * (long long)gid;
-    for (int k = 0; k < 3; k++) {
-        mf[k] = F[k];
-        mf[3 + k] = T[k];
+    const double up[3] = {wave_sum(c0), wave_sum(c1), wave_sum(c2)};
+    const double* om = a.omega + 3 * p;
+    const double wr[3] = {om[1] * r0[2] - om[2] * r0[1], om[2] * r0[0] - om[0] * r0[2], om[0] * r0[1] - om[1] * r0[0]};
+    double F[3];
+    for (int k = 0; k < 3; k++) F[k] = (up[k] - (a.vel[3 * p + k] + wr[k])) * a.dA;
+    if (sok) {
+        const int node = sn[0] + g.nx * (sn[1] + g.ny * sn[2]);
+        spread_add(a.facc, nint, node, sw[0] * sw[1] * sw[2], F);
     }
-}
-
-// per-sphere FP64 sums of the marker forces and torques in a fixed order
-__global__ void __launch_bounds__(256) ibm_reduce_kernel(const double* __restrict__ mforce, int ppp, double* ff, double* tf) {
-    const int p = blockIdx.x;
-    __shared__ double sh[6][256];
-    double acc[6] = {0, 0, 0, 0, 0, 0};
-    for (int m = threadIdx.x; m < ppp; m += blockDim.x) {
-        const double* mf = mforce + 6 * ((long long)p * ppp + m);
-        for (int k = 0; k < 6; k++) acc[k] += mf[k];
+    if (lane == 0) {
+        double* mf = a.mforce + 6 * (long long)gid;
+        mf[0] = F[0];
+        mf[1] = F[1];
+        mf[2] = F[2];
+        mf[3] = r0[1] * F[2] - r0[2] * F[1];
+        mf[4] = r0[2] * F[0] - r0[0] * F[2];
+        mf[5] = r0[0] * F[1] - r0[1] * F[0];
     }
-    for (int k = 0; k < 6; k++) sh[k][threadIdx.x] = acc[k];
-    __syncthreads();
-    for (int st = blockDim.x / 2; st > 0; st >>= 1) {
-        if ((int)threadIdx.x < st)
-            for (int k = 0; k < 6; k++) sh[k][threadIdx.x] += sh[k][threadIdx.x + st];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0)
-        for (int k = 0; k < 3; k++) {
-            ff[3 * p + k] = sh[k][0];
-            tf[3 * p + k] = sh[3 + k][0];
-        }
-}
-
-// ---- brick masks -----------------------------------------------------------------------------------
-__global__ void clear_mask_kernel(uint8_t* m, int n) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) m[i] = 0;
-}
-
-// Marks every brick that a marker stencil of sphere p can touch: for each axis, marker
-// coordinates span [c-R, c+R]; they are wrapped like the markers (fmod over SystemSize) and
-// widened by the stencil reach (-2..+3 nodes); periodic axes wrap, wall axes clip.
-__global__ void mark_kernel(const double* __restrict__ pos, Geom g, double radius, uint8_t* mask) {
-    const int p = blockIdx.x;
-    __shared__ int lists[3][64];
-    __shared__ int counts[3];
-    __shared__ uint8_t flags[3][1024];
-    const int nd[3] = {g.nx, g.ny, g.nz};
-    const int bn[3] = {g.bnx, g.bny, g.bnz};
-    for (int d = 0; d < 3; d++)
-        for (int i = threadIdx.x; i < bn[d] && i < 1024; i += blockDim.x) flags[d][i] = 0;
-    __syncthreads();
-    if (threadIdx.x < 3) {
-        const int d = threadIdx.x;
-        const double c = pos[3 * p + d];
-        const double L = (double)g.sys[d];
-        const bool finite = (c > -1e7 && c < 1e7);  // false for NaN: such a sphere marks nothing
-        const int lo = finite ? (int)floor(c - radius) - 1 : 0, hi = finite ? (int)floor(c + radius) + 1 : -1;
-        for (int v = lo; v <= hi; v++) {
-            const double w = fmod((double)v + 0.5 + L, L);  // wrap like the markers
-            const int wn = (int)floor(w);
-            for (int k = -2; k <= 3; k++) {
-                int node = wn + k;
-                if (g.per[d]) node = ((node % nd[d]) + nd[d]) % nd[d];
-                else if (node < 0 || node >= nd[d]) continue;
-                const int b = node / BRICK;
-                if (b < 1024) flags[d][b] = 1;
-            }
-        }
-        int cnt = 0;
-        for (int b = 0; b < bn[d] && b < 1024; b++)
-            if (flags[d][b] && cnt < 64) lists[d][cnt++] = b;
-        counts[d] = cnt;
-    }
-    __syncthreads();
-    const int tot = counts[0] * counts[1] * counts[2];
-    for (int i = threadIdx.x; i < tot; i += blockDim.x) {
-        const int kx = i % counts[0];
-        const int ky = (i / counts[0]) % counts[1];
-        const int kz = i / (counts[0] * counts[1]);
-        mask[lists[0][kx] + g.bnx * (lists[1][ky] + g.bny * lists[2][kz])] = 1;
-    }
-}
-
-int launch_clear_mask(Sim* s, uint8_t* m, hipStream_t st) {
-    if (!m) return 0;
-    hipLaunchKernelGGL(clear_mask_kernel, dim3((s->nbricks + 255) / 256), dim3(256), 0, st, m, s->nbricks);
-    HIP_OK(hipGetLastError());
-    return 0;
-}
-
-int launch_mark_bricks(Sim* s, uint8_t* m, hipStream_t st) {
-    if (s->np == 0) return 0;
-    // marker offsets are scaled unit vectors (|r0|^2 within 1% of 1, D3Q19-OpenCL.c:597-601)
-    const double rmax = 0.5 * (double)s->fp.ParticleDiam * 1.01 + 1e-3;
-    hipLaunchKernelGGL(mark_kernel, dim3(s->np), dim3(64), 0, st, s->pos, s->g, rmax, m);
-    HIP_OK(hipGetLastError());
-    return 0;
 }
 
 int launch_ibm(Sim* s, hipStream_t st) {
@@ -271,8 +174,9 @@ int launch_ibm(Sim* s, hipStream_t st) {
     a.u = s->u;
     a.facc = s->facc;
     a.mforce = s->mforce;
+    a.epochs = s->epochs;
     const int total = s->np * s->ppp;
-    const int th = 256, blocks = (total + th - 1) / th;
+    const int th = 256, blocks = (total + 3) / 4;  // four markers (wavefronts) per block
     const bool dbl = s->storage == D3Q19_STORE_FP64;
     if (a.mode == D3Q19_IBM_PESKIN4) {
         if (dbl) hipLaunchKernelGGL((ibm_marker_kernel<double, D3Q19_IBM_PESKIN4>), dim3(blocks), dim3(th), 0, st, a);
@@ -281,7 +185,6 @@ int launch_ibm(Sim* s, hipStream_t st) {
         if (dbl) hipLaunchKernelGGL((ibm_marker_kernel<double, D3Q19_IBM_LINEAR_SMOOTHED>), dim3(blocks), dim3(th), 0, st, a);
         else hipLaunchKernelGGL((ibm_marker_kernel<float, D3Q19_IBM_LINEAR_SMOOTHED>), dim3(blocks), dim3(th), 0, st, a);
     }
-    hipLaunchKernelGGL(ibm_reduce_kernel, dim3(s->np), dim3(256), 0, st, s->mforce, s->ppp, s->ffluid, s->tfluid);
     HIP_OK(hipGetLastError());
     return 0;
 }

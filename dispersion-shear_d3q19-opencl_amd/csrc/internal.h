@@ -9,7 +9,10 @@
 //   u, tau       R[3*nint], R[nint]           (R = float, or double for the fp64 build)
 //   force        int64[3*nint] fixed point (2^-40 units), spread with integer atomics so the
 //                sum is independent of order; consumed and zeroed by the next collide.
-//   masks        uint8 per 4x4x4 brick: where force is pending / where u must be written.
+//   masks        uint32 epoch stamp per 4x4x4 brick, two arrays by epoch parity: a brick is
+//                marked for the step whose epoch it holds (no per-step clearing).  The kernels of a
+//                step use e = epochs[0] + 1; the spreading kernel commits epochs[0] = e and records
+//                epochs[1] = e as the epoch whose spread force is pending.
 //   particles    double SoA.
 #pragma once
 #include <hip/hip_runtime.h>
@@ -119,8 +122,8 @@ struct Sim {
     void* u = nullptr;    // R[3*nint]
     void* tau = nullptr;  // R[nint]
     long long* facc = nullptr;
-    uint8_t* maskF = nullptr;  // force pending
-    uint8_t* maskU = nullptr;  // u needed by the next IBM pass
+    uint32_t* mask[2] = {nullptr, nullptr};  // brick epoch stamps, by epoch parity
+    uint32_t* epochs = nullptr;              // [current epoch, epoch of the pending spread force]
     int nbricks = 0;
     // raw wall unknowns of an uploaded pre-collision field: [2 sides][5][plane]
     void* bcraw = nullptr;
@@ -142,7 +145,6 @@ struct Sim {
     // step graph
     hipGraphExec_t graph2 = nullptr;   // two steps (ping-pong closes)
     const void* graph_f = nullptr;     // state the graph was captured from
-    const uint8_t* graph_mask = nullptr;
     int parity = 0;            // 0: current post-collision field in fA
     int block[3] = {64, 4, 1};
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -158,11 +160,9 @@ int launch_export_force(Sim* s, float* ref_out);
 int launch_plane_mean(Sim* s, double* dev_out);
 int launch_moments(Sim* s, const void* fsrc, double* dev_out);
 int launch_check_finite(Sim* s, const void* fsrc, int* dev_flag);
-int launch_clear_mask(Sim* s, uint8_t* m, hipStream_t st);
-int launch_mark_bricks(Sim* s, uint8_t* m, hipStream_t st);
 int launch_ibm(Sim* s, hipStream_t st);
-int launch_dynamics(Sim* s, hipStream_t st);
-int launch_contacts(Sim* s, hipStream_t st);
+int launch_sphere_pre(Sim* s, int do_dyn, int do_mark, hipStream_t st);       // dynamics, brick marks
+int launch_sphere_post(Sim* s, int do_reduce, int do_contacts, hipStream_t st); // force sums, contacts
 int launch_fill_equilibrium(Sim* s, void* f);
 
 // error reporting

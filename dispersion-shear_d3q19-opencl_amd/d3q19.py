@@ -148,7 +148,7 @@ def lib():
     """libd3q19.so, the HIP library.  Raises if it is not built."""
     global _lib
     if _lib is None:
-        path = os.path.join(LIBDIR, "libd3q19.so")
+        path = os.environ.get("D3Q19_LIBRARY") or os.path.join(LIBDIR, "libd3q19.so")
         if not os.path.exists(path):
             raise RuntimeError("libd3q19.so not built; run build.py (no CPU fallback exists)")
         L = C.CDLL(path)
