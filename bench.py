@@ -21,6 +21,21 @@ sys.path.insert(0, os.path.join(ROOT, "dispersion-shear_d3q19-opencl_amd"))
 sys.path.insert(0, os.path.join(ROOT, "tests"))
 
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "hbm_traffic_collide_c2_powerlaw.md")
+
+
+def measured_traffic(key):
+    """HBM bytes per collide launch from the committed counter-pass summary (FETCH_SIZE and
+    WRITE_SIZE, calibrated with a copy kernel of known size), or None when absent."""
+    try:
+        for line in open(TRAFFIC_SUMMARY):
+            if line.startswith(key + ":"):
+                return int(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
 METRIC = "MLUPS & %HBM roofline, FP32 vs FP16-store; shear-cell solves/s + p99 latency"
 
 
@@ -94,16 +109,49 @@ def run(args):
                                "tau clamp [0.51,5]), 32 spheres r=6 x 192 markers, BGK, Peskin-4 IBM, fp32 populations",
                    "interior_nodes": nodes},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": measured_traffic("traffic_bytes_per_launch_fp32"),
+                     "traffic_source": "profiles/hbm_traffic_collide_c2_powerlaw.md (rocprofv3 FETCH_SIZE x2.0, WRITE_SIZE x1.0, "
+                                       "scaled by a 1 GiB copy)",
+                     "algorithmic_bytes": int(bytes_per_launch),
                      "kernel": "collide_kernel (fused collide+stream), 152 B/node algorithmic",
                      "kernel_ms": round(collide_ms, 5)},
         "fp16_store": {"mlups": round(mlups16, 1), "ms_per_step": round(t16 / args.steps, 5),
                        "collide_ms": round(col16, 5), "collide_achieved_GBs": round(ach16, 1),
+                       "collide_traffic": measured_traffic("traffic_bytes_per_launch_fp16"),
                        "speedup_vs_fp32": round(mlups16 / mlups, 3)},
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cpu_steps)
     print(json.dumps(out), flush=True)
+
+
+COPY_ELEMS = 1 << 28  # float32 elements of the calibration copy (1 GiB read + 1 GiB written)
+
+
+def traffic_probe():
+    """Launches for the HBM-traffic counter passes (rocprofv3 --pmc FETCH_SIZE, then WRITE_SIZE):
+    5 direct collide+stream launches of the workload in fp32 and in fp16-shifted storage, then a
+    plain elementwise copy of known size (torch.neg over 1 GiB of float32) whose counter readings
+    calibrate the counters (expected 1 GiB fetched and 1 GiB written per launch)."""
+    import d3q19
+    import torch
+    for storage in (0, 1):
+        c = d3q19.Case(lines=c2_deck(storage=storage))
+        s = d3q19.Sim(c)
+        s.upload_case()
+        s.run(1, 4)
+        s.sync()
+        s.time_collide(5)
+        s.sync()
+        s.close()
+        print("probe: collide storage=%d done" % storage, flush=True)
+    src = torch.ones(COPY_ELEMS, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        torch.neg(src, out=dst)
+    torch.cuda.synchronize()
+    print("probe: copy of %d bytes read + %d bytes written, 3 launches" % (4 * COPY_ELEMS, 4 * COPY_ELEMS), flush=True)
 
 
 def main():
@@ -113,11 +161,16 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--cpu-steps", type=int, default=60)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--traffic-probe", action="store_true",
+                    help="only the launches profiled by the HBM-traffic counter passes")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (a cell is not split across GPUs); use --gpus 1",
               file=sys.stderr)
         sys.exit(2)
+    if args.traffic_probe:
+        traffic_probe()
+        return
     run(args)
 
 
