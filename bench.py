@@ -39,13 +39,7 @@ def measured_traffic(key):
 METRIC = "MLUPS & %HBM roofline, FP32 vs FP16-store; shear-cell solves/s + p99 latency"
 
 
-def c2_deck(storage=0, steps=None):
-    from cases import deck
-    n, tau, U, H = 0.7, 0.9, 0.02, 128.0
-    nu = (tau - 0.5) / 3.0
-    K = nu * (2 * U / H) ** (1 - n)
-    return deck(size=128, walls=(0, 0, 1), tau=tau, uw=U, np_=32, diam=12.0, markers=192, dist=4, seed=2024, gap=2.0,
-                coll=1, ibm=1, visc=2, vparams=(K, n, 0, 0), local=1, clamp=(0.51, 5.0), storage=storage)
+from workloads import c2_deck, c3_deck, c4_decks, c5_requests, poisson_arrivals, precision_study  # noqa: E402
 
 
 def cpu_baseline(sample_steps):
@@ -126,6 +120,111 @@ def run(args):
     print(json.dumps(out), flush=True)
 
 
+def pct(a, q):
+    import numpy as np
+    return float(np.percentile(np.asarray(a, dtype=np.float64), q)) if len(a) else None
+
+
+def ensemble(args):
+    """64 independent shear cells (uneven edges, sphere sets and rheologies) in fp16-shifted storage,
+    stepped together by the batched persistent kernels with the autotuned launch shape; compared with
+    the same cells stepped one after another as single simulations (graph-replayed too)."""
+    import d3q19
+    decks = c4_decks(storage=1)
+    cases = [d3q19.Case(lines=d) for d in decks]
+    nodes = sum((c.N[0] - 2) * (c.N[1] - 2) * (c.N[2] - 2) for c in cases)
+    b = d3q19.Batch(storage=1, collision=d3q19.COLL_BGK, ibm=d3q19.IBM_PESKIN4, arena_bytes=16 << 30)
+    ids = [b.add(c) for c in cases]
+    print("ensemble: %d cells, %d nodes, %d spheres" % (len(ids), nodes, sum(c.np for c in cases)), flush=True)
+    b.run(args.ens_warmup)
+    b.sync()
+    t = b.autotune()
+    print("ensemble: autotuned %s" % t.as_dict(), flush=True)
+    total_ms, col_ms = b.time_steps(args.ens_steps)
+    mlups = nodes * args.ens_steps / (total_ms * 1e-3) / 1e6
+    mem = b.memory()
+    b.close()
+    seq_steps = args.ens_seq_steps
+    seq_ms = 0.0
+    for c in cases:
+        sm = d3q19.Sim(c)
+        if c.np:
+            sm.write_particles(c.parKin)
+        sm.run(1, 4)
+        sm.sync()
+        ms, _ = sm.time_steps(5, seq_steps)
+        seq_ms += ms
+        sm.close()
+    mlups_seq = nodes * seq_steps / (seq_ms * 1e-3) / 1e6
+    return {"cells": len(ids), "nodes": nodes, "steps": args.ens_steps, "storage": "fp16_shift",
+            "mlups": round(mlups, 1), "ms_per_step": round(total_ms / args.ens_steps, 4),
+            "collide_ms": round(col_ms, 4), "collide_achieved_GBs": round(76.0 * nodes / (col_ms * 1e-3) / 1e9, 1),
+            "tuning": t.as_dict(), "arena_used_bytes": mem[1],
+            "sequential_single_mlups": round(mlups_seq, 1), "sequential_steps_per_cell": seq_steps,
+            "speedup_vs_sequential": round(mlups / mlups_seq, 3)}
+
+
+def serving(args):
+    """Poisson arrivals of shear-cell solve requests into the continuous-batching server (fp32
+    populations); then the same requests as sequential plain fp32 solves (single simulations, one
+    after another, same convergence rule).  Latency = completion - scheduled arrival."""
+    import d3q19
+    decks = c5_requests(count=args.requests, edge_range=(32, args.max_edge), storage=0)
+    cases = [d3q19.Case(lines=d) for d in decks]
+    arr = poisson_arrivals(args.rate, len(cases))
+    sp = d3q19.solve_params(tolerance=1e-4, check_interval=args.check_interval, min_steps=10 * args.check_interval,
+                            max_steps=args.max_steps)
+    srv = d3q19.Server(storage=0, collision=d3q19.COLL_BGK, ibm=d3q19.IBM_PESKIN4, arena_bytes=args.arena_gb << 30)
+    t0 = d3q19.clock()
+    nxt = 0
+    rid_of = {}
+    done = {}
+    while len(done) < len(cases):
+        now = d3q19.clock() - t0
+        while nxt < len(cases) and arr[nxt] <= now:
+            rid_of[srv.submit(cases[nxt], sp)] = nxt
+            nxt += 1
+        for r in srv.progress():
+            done[rid_of[r.id]] = r
+            print("serving: request %d done (%s, status %d, %d steps)" % (rid_of[r.id], cases[rid_of[r.id]].N, r.status, r.steps),
+                  flush=True)
+        q, res, _ = srv.status()
+        if q == 0 and res == 0 and nxt < len(cases):
+            time.sleep(max(0.0, arr[nxt] - (d3q19.clock() - t0)))
+    srv.close()
+    lat = [done[i].t_finish - (t0 + arr[i]) for i in range(len(cases))]
+    span = max(done[i].t_finish for i in done) - (t0 + arr[0])
+    # sequential plain fp32 solves of the same requests
+    service = []
+    seq = {}
+    for i, c in enumerate(cases):
+        a = time.perf_counter()
+        seq[i] = d3q19.solve(c, sp)
+        service.append(time.perf_counter() - a)
+    fin, lat_seq = 0.0, []
+    for i in range(len(cases)):
+        fin = max(fin, arr[i]) + service[i]
+        lat_seq.append(fin - arr[i])
+    agree = max(abs(done[i].obs.wall_stress - seq[i].obs.wall_stress) / max(abs(seq[i].obs.wall_stress), 1e-300)
+                for i in range(len(cases)))
+    steps_equal = sum(1 for i in range(len(cases)) if done[i].steps == seq[i].steps)
+    return {"requests": len(cases), "rate_per_s": args.rate, "edges": [c.N[0] - 2 for c in cases],
+            "spheres": [c.np for c in cases], "check_interval": args.check_interval, "tolerance": 1e-4,
+            "max_steps": args.max_steps, "converged": sum(1 for r in done.values() if r.status == 0),
+            "batched": {"solves_per_s": round(len(cases) / span, 4), "p50_latency_s": round(pct(lat, 50), 4),
+                        "p99_latency_s": round(pct(lat, 99), 4), "makespan_s": round(span, 3)},
+            "sequential": {"solves_per_s": round(len(cases) / sum(service), 4), "p50_latency_s": round(pct(lat_seq, 50), 4),
+                           "p99_latency_s": round(pct(lat_seq, 99), 4), "busy_s": round(sum(service), 3)},
+            "speedup_solves_per_s": round((len(cases) / span) / (len(cases) / sum(service)), 3),
+            "max_rel_diff_wall_stress_vs_sequential": agree, "same_step_count": steps_equal}
+
+
+def precision(args):
+    """The 256x256x128 Casson precision workload: fp32 / fp16-shifted / bf16-shifted populations
+    against the fp64 build, errors of u, drag coefficient and wall stress."""
+    return precision_study(c3_deck, args.prec_steps)
+
+
 COPY_ELEMS = 1 << 28  # float32 elements of the calibration copy (1 GiB read + 1 GiB written)
 
 
@@ -163,6 +262,19 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="only the launches profiled by the HBM-traffic counter passes")
+    ap.add_argument("--mode", default="step", choices=["step", "ensemble", "serving", "precision"],
+                    help="step: the per-step benchmark line (default); ensemble / serving / precision: the batched, "
+                         "serving and storage-precision workloads, one JSON line each")
+    ap.add_argument("--ens-steps", type=int, default=1000)
+    ap.add_argument("--ens-warmup", type=int, default=10)
+    ap.add_argument("--ens-seq-steps", type=int, default=100)
+    ap.add_argument("--requests", type=int, default=24)
+    ap.add_argument("--rate", type=float, default=4.0)
+    ap.add_argument("--max-edge", type=int, default=160)
+    ap.add_argument("--check-interval", type=int, default=100)
+    ap.add_argument("--max-steps", type=int, default=40000)
+    ap.add_argument("--arena-gb", type=int, default=64)
+    ap.add_argument("--prec-steps", type=int, default=3000)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (a cell is not split across GPUs); use --gpus 1",
@@ -171,7 +283,14 @@ def main():
     if args.traffic_probe:
         traffic_probe()
         return
-    run(args)
+    if args.mode == "ensemble":
+        print(json.dumps({"workload": "batched ensemble", **ensemble(args)}), flush=True)
+    elif args.mode == "serving":
+        print(json.dumps({"workload": "serving", **serving(args)}), flush=True)
+    elif args.mode == "precision":
+        print(json.dumps({"workload": "precision study", **precision(args)}), flush=True)
+    else:
+        run(args)
 
 
 if __name__ == "__main__":

@@ -117,15 +117,60 @@ static void default_block(Sim* s) {
 static void* cur_f(Sim* s) { return s->parity == 0 ? s->fA : s->fB; }
 static void* nxt_f(Sim* s) { return s->parity == 0 ? s->fB : s->fA; }
 
-static void free_all(Sim* s) {
-    void* ptrs[] = {s->fA, s->fB, s->u, s->tau, s->facc, s->mask[0], s->mask[1], s->epochs, s->bcraw, s->pos, s->vel, s->quat,
-                    s->omega, s->ffluid, s->tfluid, s->fcont, s->tcont, s->pts, s->mforce, s->red};
+void sim_release(Sim* s) {
+    void* ptrs[] = {s->fA, s->fB, s->u, s->tau, s->facc, s->mask[0], s->mask[1], s->bcraw, s->pos, s->vel, s->quat,
+                    s->omega, s->ffluid, s->tfluid, s->fcont, s->tcont, s->pts, s->mforce, s->red, s->d_desc, s->d_obs};
     for (void* p : ptrs)
-        if (p) (void)hipFree(p);
+        if (p) {
+            if (s->arena) s->arena->release(p);
+            else (void)hipFree(p);
+        }
+    if (s->epochs && !s->shares_epochs) (void)hipFree(s->epochs);
     if (s->graph2) (void)hipGraphExecDestroy(s->graph2);
     if (s->ev0) (void)hipEventDestroy(s->ev0);
     if (s->ev1) (void)hipEventDestroy(s->ev1);
-    if (s->stream) (void)hipStreamDestroy(s->stream);
+    if (s->stream && !s->shares_stream) (void)hipStreamDestroy(s->stream);
+}
+static void free_all(Sim* s) { sim_release(s); }
+
+// ---- arena ---------------------------------------------------------------------------------------------
+void* Arena::alloc(size_t bytes) {
+    const size_t need = (bytes + 255) & ~(size_t)255;
+    for (auto it = free_.begin(); it != free_.end(); ++it) {
+        if (it->second < need) continue;
+        const size_t off = it->first, len = it->second;
+        free_.erase(it);
+        if (len > need) free_[off + need] = len - need;
+        live_[off] = need;
+        used += need;
+        return base + off;
+    }
+    return nullptr;
+}
+
+bool Arena::release(void* p) {
+    const size_t off = (size_t)((char*)p - base);
+    auto it = live_.find(off);
+    if (it == live_.end()) return false;
+    size_t len = it->second;
+    live_.erase(it);
+    used -= len;
+    size_t start = off;
+    auto nx = free_.lower_bound(off);
+    if (nx != free_.end() && nx->first == off + len) {  // merge with the following hole
+        len += nx->second;
+        nx = free_.erase(nx);
+    }
+    if (nx != free_.begin()) {
+        auto pv = std::prev(nx);
+        if (pv->first + pv->second == off) {  // merge with the preceding hole
+            start = pv->first;
+            len += pv->second;
+            free_.erase(pv);
+        }
+    }
+    free_[start] = len;
+    return true;
 }
 
 template <class T>
@@ -137,6 +182,51 @@ static int dalloc(T** p, size_t bytes) {
         return fail(D3Q19_E_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
     }
     return 0;
+}
+
+// device memory of a simulation: from its arena (batch members) or hipMalloc
+template <class T>
+static int salloc(Sim* s, T** p, size_t bytes) {
+    if (!s->arena) return dalloc(p, bytes);
+    if (bytes == 0) bytes = 8;
+    *p = (T*)s->arena->alloc(bytes);
+    if (!*p) return fail(D3Q19_E_CAPACITY, "batch arena exhausted");
+    return 0;
+}
+
+CellDesc make_cell_desc(Sim* s) {
+    CellDesc d;
+    memset(&d, 0, sizeof(d));
+    d.c = make_collide_args(s, nullptr, nullptr, 0, 1);
+    d.i = make_ibm_args(s);
+    d.p = make_part_args(s);
+    d.f[0] = s->fA;
+    d.f[1] = s->fB;
+    d.par = s->parity;
+    d.nint = s->g.nint;
+    d.nx = s->g.nx;
+    d.nxy = s->g.nx * s->g.ny;
+    d.inv_nx = 1.0f / (float)d.nx;
+    d.inv_nxy = 1.0f / (float)d.nxy;
+    ObsArgs& o = d.o;
+    o.axisA = s->g.wall;
+    o.coll = s->opt.collision;
+    o.rate_fixed = 0.f;
+    o.axisF = -1;
+    if (o.axisA >= 0) {
+        double best = -1.0;
+        for (int a = 0; a < 3; a++) {
+            if (a == o.axisA) continue;
+            const double du = fabs((double)s->fp.VelUpper[a] - (double)s->fp.VelLower[a]);
+            if (du > best) {
+                best = du;
+                o.axisF = a;
+            }
+        }
+        // the reference's MRT relaxes the x-y stress moment at the fixed rate 1.4 (GPU_program.cl:453)
+        if (o.coll == D3Q19_COLL_MRT && o.axisA + o.axisF == 1) o.rate_fixed = 1.4f;
+    }
+    return d;
 }
 
 // ---- step building blocks --------------------------------------------------------------------------
@@ -186,9 +276,214 @@ static int build_graph(Sim* s) {
 
 using namespace d3q19;
 
-struct d3q19_sim : public Sim {};
+namespace d3q19 {
+// Fields of a simulation derived from its parameter structs (no device work).
+static void sim_setup(Sim* s, const int_param_struct* ip, const flp_param_struct* fp, const d3q19_options* opt) {
+    s->ip = *ip;
+    s->fp = *fp;
+    if (opt) s->opt = *opt;
+    else memset(&s->opt, 0, sizeof(s->opt));
+    s->Nx = ip->LatticeSize[0];
+    s->Ny = ip->LatticeSize[1];
+    s->Nz = ip->LatticeSize[2];
+    s->nn = (long long)s->Nx * s->Ny * s->Nz;
+    s->storage = s->opt.storage;
+    s->pop_bytes = pop_size(s->storage);
+    geom_from(s);
+    default_block(s);
+    s->np = ip->NumParticles;
+    s->ppp = s->np > 0 ? ip->PointsPerParticle : 0;
+    s->nfa = ip->WorkGroupsPerParticle > 0 ? ip->WorkGroupsPerParticle : 1;
+    const int wallA = s->g.wall;
+    s->bc_plane = wallA >= 0 ? (wallA == 0 ? s->g.ny * s->g.nz : (wallA == 1 ? s->g.nx * s->g.nz : s->g.nx * s->g.ny)) : 1;
+    s->nbricks = s->g.bnx * s->g.bny * s->g.bnz;
+}
+
+struct AllocItem {
+    void** ptr;
+    size_t bytes;
+};
+template <class T>
+static AllocItem item(T** p, size_t bytes) {
+    return AllocItem{(void**)p, bytes};
+}
+
+// Every device array of a simulation, in allocation order.
+static void alloc_list(Sim* s, std::vector<AllocItem>& v) {
+    const size_t nint = (size_t)s->g.nint;
+    const size_t rs = real_size(s->storage);
+    v.push_back(item(&s->fA, (size_t)Q * nint * s->pop_bytes));
+    v.push_back(item(&s->fB, (size_t)Q * nint * s->pop_bytes));
+    v.push_back(item(&s->u, 3 * nint * rs));
+    v.push_back(item(&s->tau, nint * rs));
+    v.push_back(item(&s->bcraw, (size_t)10 * s->bc_plane * rs));
+    v.push_back(item(&s->red, (size_t)(4 * 512 + 8) * sizeof(double)));
+    v.push_back(item(&s->d_desc, sizeof(CellDesc)));
+    v.push_back(item(&s->d_obs, (size_t)OBS_STRIDE * sizeof(double)));
+    if (s->np > 0) {
+        const size_t npd = (size_t)s->np;
+        v.push_back(item(&s->facc, 3 * nint * sizeof(long long)));
+        v.push_back(item(&s->mask[0], (size_t)s->nbricks * sizeof(uint32_t)));
+        v.push_back(item(&s->mask[1], (size_t)s->nbricks * sizeof(uint32_t)));
+        v.push_back(item(&s->pos, 3 * npd * 8));
+        v.push_back(item(&s->vel, 3 * npd * 8));
+        v.push_back(item(&s->quat, 4 * npd * 8));
+        v.push_back(item(&s->omega, 3 * npd * 8));
+        v.push_back(item(&s->ffluid, 3 * npd * 8));
+        v.push_back(item(&s->tfluid, 3 * npd * 8));
+        v.push_back(item(&s->fcont, 3 * npd * 8));
+        v.push_back(item(&s->tcont, 3 * npd * 8));
+        v.push_back(item(&s->pts, 4 * (size_t)s->ppp * sizeof(float)));
+        v.push_back(item(&s->mforce, 6 * npd * (size_t)s->ppp * 8));
+    }
+}
+
+// Device bytes of one simulation's state, each array rounded to the arena's 256-byte granule.
+size_t sim_state_bytes(const int_param_struct* ip, const flp_param_struct* fp, const d3q19_options* opt) {
+    Sim t;
+    sim_setup(&t, ip, fp, opt);
+    std::vector<AllocItem> items;
+    alloc_list(&t, items);
+    size_t b = 0;
+    for (const AllocItem& it : items) b += ((it.bytes ? it.bytes : 8) + 255) & ~(size_t)255;
+    return b;
+}
+
+int sim_validate(const int_param_struct* ip, const flp_param_struct* fp, const d3q19_options* opt) { return validate(ip, fp, opt); }
+
+// Allocation and initial state of a simulation (d3q19_create) or of a batch member, whose memory
+// comes from the batch arena and which uses the batch's stream and epoch counter.
+int sim_init(Sim* s, int device, const int_param_struct* ip, const flp_param_struct* fp, const d3q19_options* opt,
+             const d3q19_float4* sphere_points, Arena* arena, hipStream_t shared_stream, uint32_t* shared_epochs) {
+    int rc = validate(ip, fp, opt);
+    if (rc) return rc;
+    if (ip->NumParticles > 0 && !sphere_points) return fail(D3Q19_E_INVALID, "sphere_points required when NumParticles > 0");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev)
+        return fail(D3Q19_E_DEVICE, "no such HIP device");
+    HIP_OK(hipSetDevice(device));
+    s->device = device;
+    s->arena = arena;
+    sim_setup(s, ip, fp, opt);
+    const size_t nint = (size_t)s->g.nint;
+    const size_t rs = real_size(s->storage);
+    s->nbricks = s->g.bnx * s->g.bny * s->g.bnz;
+    hipError_t e = hipSuccess;
+    if (shared_stream) {
+        s->stream = shared_stream;
+        s->shares_stream = true;
+    } else {
+        e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+        if (e != hipSuccess) return fail(D3Q19_E_DEVICE, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+    }
+    {
+        std::vector<AllocItem> items;
+        alloc_list(s, items);
+        for (const AllocItem& it : items)
+            if ((rc = salloc(s, it.ptr, it.bytes))) {
+                free_all(s);
+                return rc;
+            }
+    }
+    if (s->np > 0) {
+        if (shared_epochs) {
+            s->epochs = shared_epochs;
+            s->shares_epochs = true;
+        } else if ((rc = dalloc(&s->epochs, 2 * sizeof(uint32_t)))) {
+            free_all(s);
+            return rc;
+        }
+    }
+    // initial state: f = f_eq(1, 0), u = 0, tau = NewtonianTau (D3Q19-OpenCL.c:158-187)
+    rc = launch_fill_equilibrium(s, s->fA);
+    if (!rc) rc = launch_fill_equilibrium(s, s->fB);
+    if (rc) {
+        free_all(s);
+        return rc;
+    }
+    e = hipMemsetAsync(s->u, 0, 3 * nint * rs, s->stream);
+    if (e == hipSuccess && s->np > 0) {
+        e = hipMemsetAsync(s->facc, 0, 3 * nint * sizeof(long long), s->stream);
+        if (e == hipSuccess) e = hipMemsetAsync(s->mask[0], 0, s->nbricks * sizeof(uint32_t), s->stream);
+        if (e == hipSuccess) e = hipMemsetAsync(s->mask[1], 0, s->nbricks * sizeof(uint32_t), s->stream);
+        static const uint32_t ep0[2] = {0u, 0u};  // next step has epoch 1; no spread force pending
+        if (e == hipSuccess && !s->shares_epochs) e = hipMemcpyAsync(s->epochs, ep0, sizeof(ep0), hipMemcpyHostToDevice, s->stream);
+        double* zs[] = {s->pos, s->vel, s->omega, s->ffluid, s->tfluid, s->fcont, s->tcont};
+        for (double* z : zs)
+            if (e == hipSuccess) e = hipMemsetAsync(z, 0, 3 * (size_t)s->np * 8, s->stream);
+        if (e == hipSuccess) {
+            std::vector<double> q(4 * (size_t)s->np, 0.0);
+            for (int p = 0; p < s->np; p++) q[4 * p + 3] = 1.0;
+            e = hipMemcpyAsync(s->quat, q.data(), q.size() * 8, hipMemcpyHostToDevice, s->stream);
+            if (e == hipSuccess) e = hipMemcpyAsync(s->pts, sphere_points, 4 * (size_t)s->ppp * sizeof(float), hipMemcpyHostToDevice, s->stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+        }
+    }
+    if (e == hipSuccess && launch_fill_tau(s)) e = hipErrorLaunchFailure;  // tau = NewtonianTau
+    if (e == hipSuccess) e = hipEventCreate(&s->ev0);
+    if (e == hipSuccess) e = hipEventCreate(&s->ev1);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    if (e != hipSuccess) {
+        free_all(s);
+        return fail(D3Q19_E_DEVICE, std::string("initialisation: ") + hipGetErrorString(e));
+    }
+    return 0;
+}
+
+// The fluid's kinematic viscosity at shear rate gdot: NewtonianTau, or the tau law with its clamps
+// when tau is per node (the law of the collide kernel, evaluated in double).
+static double fluid_nu(const Sim* s, double gdot) {
+    const FluidParams& fl = s->fl;
+    if (!s->opt.variable_viscosity) return ((double)fl.tau0 - 0.5) / 3.0;
+    const int model = fl.model;
+    double tau = (double)fl.tau0;
+    if (model == D3Q19_VISC_POWER_LAW || model == D3Q19_VISC_CASSON || model == D3Q19_VISC_HB) {
+        const double g = gdot > 1e-8 ? gdot : 1e-8;
+        double nu;
+        if (model == D3Q19_VISC_POWER_LAW) {
+            nu = (double)fl.vp[0] * pow(g, (double)fl.vp[1] - 1.0);
+        } else if (model == D3Q19_VISC_CASSON) {
+            double a = sqrt((double)fl.vp[0] / g);
+            if (fl.vp[2] > 0.f) a *= 1.0 - exp(-sqrt((double)fl.vp[2] * g));
+            const double b = a + sqrt((double)fl.vp[1]);
+            nu = b * b;
+        } else {
+            nu = (double)fl.vp[0] / g + (double)fl.vp[1] * pow(g, (double)fl.vp[2] - 1.0);
+        }
+        tau = 3.0 * nu + 0.5;
+        if (tau < (double)fl.tau_min) tau = (double)fl.tau_min;
+        if (fl.tau_max > 0.f && tau > (double)fl.tau_max) tau = (double)fl.tau_max;
+    }
+    return (tau - 0.5) / 3.0;
+}
+
+void derive_observables(const Sim* s, const double* r, d3q19_observables* o) {
+    memset(o, 0, sizeof(*o));
+    const CellDesc d = make_cell_desc(const_cast<Sim*>(s));
+    const int A = d.o.axisA, F = d.o.axisF;
+    o->wall_axis = A;
+    o->flow_axis = F;
+    if (r[5] > 0) o->mean_fluid_force = r[4] / r[5];
+    if (A < 0 || F < 0) return;
+    o->wall_stress_lower = r[1] > 0 ? r[0] / r[1] : 0.0;
+    o->wall_stress_upper = r[3] > 0 ? r[2] / r[3] : 0.0;
+    o->wall_stress = 0.5 * (o->wall_stress_lower + o->wall_stress_upper);
+    const int nA = A == 0 ? s->g.nx : (A == 1 ? s->g.ny : s->g.nz);
+    const double du = (double)s->fp.VelUpper[F] - (double)s->fp.VelLower[F];
+    o->shear_rate = du / (double)(nA - 1);
+    o->fluid_viscosity = fluid_nu(s, fabs(o->shear_rate));
+    if (o->shear_rate != 0.0) o->apparent_viscosity = o->wall_stress / o->shear_rate;
+    if (o->fluid_viscosity > 0.0) o->relative_viscosity = o->apparent_viscosity / o->fluid_viscosity;
+    if (r[5] > 0) {
+        const double U = 0.5 * fabs(du), a = 0.5 * (double)s->fp.ParticleDiam;
+        if (U > 0 && a > 0) o->drag_coefficient = o->mean_fluid_force / (0.5 * U * U * 3.14159265358979323846 * a * a);
+    }
+}
+}  // namespace d3q19
 
 extern "C" {
+
+
 
 const char* d3q19_strerror(int code) {
     switch (code) {
@@ -228,12 +523,9 @@ int d3q19_device_name(int device, char* buf, size_t len) {
 
 int d3q19_footprint(const int_param_struct* ip, const d3q19_options* opt, size_t* bytes) {
     if (!ip || !bytes) return fail(D3Q19_E_INVALID, "null argument");
-    const int st = opt ? opt->storage : 0;
-    const size_t nint = (size_t)(ip->LatticeSize[0] - 2) * (ip->LatticeSize[1] - 2) * (ip->LatticeSize[2] - 2);
-    const size_t np = ip->NumParticles > 0 ? ip->NumParticles : 0;
-    size_t b = 2 * Q * nint * pop_size(st) + 4 * nint * real_size(st);
-    if (np) b += 24 * nint + 2 * (nint / 64 + 1) + np * (22 * 8 + (size_t)ip->PointsPerParticle * 64);
-    *bytes = b;
+    flp_param_struct fp;
+    memset(&fp, 0, sizeof(fp));
+    *bytes = sim_state_bytes(ip, &fp, opt) + (ip->NumParticles > 0 ? 256 : 0);  // + epoch counter
     return 0;
 }
 
@@ -241,114 +533,11 @@ int d3q19_create(int device, const int_param_struct* ip, const flp_param_struct*
                  const d3q19_float4* sphere_points, d3q19_sim** out) {
     if (!ip || !fp || !out) return fail(D3Q19_E_INVALID, "null argument");
     *out = nullptr;
-    int rc = validate(ip, fp, opt);
-    if (rc) return rc;
-    if (ip->NumParticles > 0 && !sphere_points) return fail(D3Q19_E_INVALID, "sphere_points required when NumParticles > 0");
-    int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev)
-        return fail(D3Q19_E_DEVICE, "no such HIP device");
-    HIP_OK(hipSetDevice(device));
     d3q19_sim* s = new d3q19_sim();
-    s->device = device;
-    s->ip = *ip;
-    s->fp = *fp;
-    if (opt) s->opt = *opt;
-    else memset(&s->opt, 0, sizeof(s->opt));
-    s->Nx = ip->LatticeSize[0];
-    s->Ny = ip->LatticeSize[1];
-    s->Nz = ip->LatticeSize[2];
-    s->nn = (long long)s->Nx * s->Ny * s->Nz;
-    s->storage = s->opt.storage;
-    s->pop_bytes = pop_size(s->storage);
-    geom_from(s);
-    default_block(s);
-    s->np = ip->NumParticles;
-    s->ppp = s->np > 0 ? ip->PointsPerParticle : 0;
-    s->nfa = ip->WorkGroupsPerParticle > 0 ? ip->WorkGroupsPerParticle : 1;
-    const size_t nint = (size_t)s->g.nint;
-    const size_t rs = real_size(s->storage);
-    hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
-    if (e != hipSuccess) {
-        delete s;
-        return fail(D3Q19_E_DEVICE, std::string("hipStreamCreate: ") + hipGetErrorString(e));
-    }
-#define ALLOC(p, b)                    \
-    if ((rc = dalloc(&(p), (b)))) {    \
-        free_all(s);                   \
-        delete s;                      \
-        return rc;                     \
-    }
-    ALLOC(s->fA, (size_t)Q * nint * s->pop_bytes);
-    ALLOC(s->fB, (size_t)Q * nint * s->pop_bytes);
-    ALLOC(s->u, 3 * nint * rs);
-    ALLOC(s->tau, nint * rs);
-    const int wallA = s->g.wall;
-    s->bc_plane = wallA >= 0 ? (wallA == 0 ? s->g.ny * s->g.nz : (wallA == 1 ? s->g.nx * s->g.nz : s->g.nx * s->g.ny)) : 1;
-    ALLOC(s->bcraw, (size_t)10 * s->bc_plane * rs);
-    ALLOC(s->red, (size_t)(4 * 512 + 8) * sizeof(double));
-    s->nbricks = s->g.bnx * s->g.bny * s->g.bnz;
-    if (s->np > 0) {
-        ALLOC(s->facc, 3 * nint * sizeof(long long));
-        ALLOC(s->mask[0], (size_t)s->nbricks * sizeof(uint32_t));
-        ALLOC(s->mask[1], (size_t)s->nbricks * sizeof(uint32_t));
-        ALLOC(s->epochs, 2 * sizeof(uint32_t));
-        const size_t npd = (size_t)s->np;
-        ALLOC(s->pos, 3 * npd * 8);
-        ALLOC(s->vel, 3 * npd * 8);
-        ALLOC(s->quat, 4 * npd * 8);
-        ALLOC(s->omega, 3 * npd * 8);
-        ALLOC(s->ffluid, 3 * npd * 8);
-        ALLOC(s->tfluid, 3 * npd * 8);
-        ALLOC(s->fcont, 3 * npd * 8);
-        ALLOC(s->tcont, 3 * npd * 8);
-        ALLOC(s->pts, 4 * (size_t)s->ppp * sizeof(float));
-        ALLOC(s->mforce, 6 * npd * (size_t)s->ppp * 8);
-    }
-#undef ALLOC
-    // initial state: f = f_eq(1, 0), u = 0, tau = NewtonianTau (D3Q19-OpenCL.c:158-187)
-    rc = launch_fill_equilibrium(s, s->fA);
-    if (!rc) rc = launch_fill_equilibrium(s, s->fB);
+    const int rc = sim_init(s, device, ip, fp, opt, sphere_points, nullptr, nullptr, nullptr);
     if (rc) {
-        free_all(s);
         delete s;
         return rc;
-    }
-    e = hipMemsetAsync(s->u, 0, 3 * nint * rs, s->stream);
-    if (e == hipSuccess && s->np > 0) {
-        e = hipMemsetAsync(s->facc, 0, 3 * nint * sizeof(long long), s->stream);
-        if (e == hipSuccess) e = hipMemsetAsync(s->mask[0], 0, s->nbricks * sizeof(uint32_t), s->stream);
-        if (e == hipSuccess) e = hipMemsetAsync(s->mask[1], 0, s->nbricks * sizeof(uint32_t), s->stream);
-        const uint32_t ep0[2] = {0u, 0u};  // next step has epoch 1; no spread force pending
-        if (e == hipSuccess) e = hipMemcpyAsync(s->epochs, ep0, sizeof(ep0), hipMemcpyHostToDevice, s->stream);
-        double* zs[] = {s->pos, s->vel, s->omega, s->ffluid, s->tfluid, s->fcont, s->tcont};
-        for (double* z : zs)
-            if (e == hipSuccess) e = hipMemsetAsync(z, 0, 3 * (size_t)s->np * 8, s->stream);
-        if (e == hipSuccess) {
-            std::vector<double> q(4 * (size_t)s->np, 0.0);
-            for (int p = 0; p < s->np; p++) q[4 * p + 3] = 1.0;
-            e = hipMemcpyAsync(s->quat, q.data(), q.size() * 8, hipMemcpyHostToDevice, s->stream);
-            if (e == hipSuccess) e = hipMemcpyAsync(s->pts, sphere_points, 4 * (size_t)s->ppp * sizeof(float), hipMemcpyHostToDevice, s->stream);
-            if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
-        }
-    }
-    if (e == hipSuccess) {
-        // tau field
-        std::vector<float> t(1, s->fp.NewtonianTau);
-        if (rs == 4) {
-            std::vector<float> tv(nint, s->fp.NewtonianTau);
-            e = hipMemcpy(s->tau, tv.data(), nint * 4, hipMemcpyHostToDevice);
-        } else {
-            std::vector<double> tv(nint, (double)s->fp.NewtonianTau);
-            e = hipMemcpy(s->tau, tv.data(), nint * 8, hipMemcpyHostToDevice);
-        }
-    }
-    if (e == hipSuccess) e = hipEventCreate(&s->ev0);
-    if (e == hipSuccess) e = hipEventCreate(&s->ev1);
-    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
-    if (e != hipSuccess) {
-        free_all(s);
-        delete s;
-        return fail(D3Q19_E_DEVICE, std::string("initialisation: ") + hipGetErrorString(e));
     }
     *out = s;
     return 0;
@@ -638,6 +827,20 @@ int d3q19_moments(d3q19_sim* s, double* out4) {
     if (rc) return rc;
     HIP_OK(hipMemcpyAsync(out4, d, 4 * 8, hipMemcpyDeviceToHost, s->stream));
     HIP_OK(hipStreamSynchronize(s->stream));
+    return 0;
+}
+
+int d3q19_observables_get(d3q19_sim* s, d3q19_observables* out) {
+    CHECK_SIM(s);
+    if (!out) return fail(D3Q19_E_INVALID, "null output");
+    s->h_desc = make_cell_desc(s);
+    double raw[OBS_STRIDE];
+    HIP_OK(hipMemcpyAsync(s->d_desc, &s->h_desc, sizeof(CellDesc), hipMemcpyHostToDevice, s->stream));
+    int rc = launch_observables(s->d_desc, 1, s->storage, 0, s->d_obs, s->stream);
+    if (rc) return rc;
+    HIP_OK(hipMemcpyAsync(raw, s->d_obs, sizeof(raw), hipMemcpyDeviceToHost, s->stream));
+    HIP_OK(hipStreamSynchronize(s->stream));
+    derive_observables(s, raw, out);
     return 0;
 }
 

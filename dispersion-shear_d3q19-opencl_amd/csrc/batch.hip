@@ -1,0 +1,801 @@
+// batch.hip -- batched ensembles of independent shear cells, the serving scheduler (continuous
+// batching with admission by device-memory footprint) and the plain sequential solve.
+//
+// A batch owns one HIP stream, one device arena from which every member cell's state is carved
+// as a single block, one epoch counter shared by the members' brick marks, and the work lists of
+// the persistent kernels: a table of CellDesc (one slot per cell id), the collide tiles (cell,
+// first node), the spheres (cell, local index, first marker, markers per sphere) and a
+// marker -> sphere map.  The lists live in device memory and the persistent kernels read their
+// lengths from there, so the captured two-step graph stays valid while cells join and leave; it
+// is re-captured only when a list outgrows its allocation or the launch shape changes.
+//
+// Step parity: every member's current post-collision field is f[par ^ phase], where par is fixed
+// when the cell joins and phase is the batch's step parity.  Graph pairs leave phase unchanged; a
+// single direct step flips it.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <deque>
+#include <string>
+#include <vector>
+#include "internal.h"
+
+using namespace d3q19;
+
+namespace d3q19 {
+
+static int bfail(int code, const std::string& msg) {
+    set_error(msg);
+    return code;
+}
+
+struct Member {
+    d3q19_sim* sim = nullptr;
+    Arena sub;            // the member's own block of the batch arena
+    void* block = nullptr;
+    int par = 0;
+};
+
+struct Batch {
+    int device = 0;
+    int ncu = 256;
+    hipStream_t stream = nullptr;
+    d3q19_batch_options bo;
+    Arena arena;
+    uint32_t* epochs = nullptr;
+    int max_cells = 1024;
+    std::vector<Member> cells;  // index = cell id
+    int phase = 0;
+    // device work lists
+    CellDesc* d_cells = nullptr;
+    uint2* d_tiles = nullptr;
+    size_t cap_tiles = 0;
+    int4* d_sph = nullptr;
+    size_t cap_sph = 0;
+    int* d_msph = nullptr;
+    size_t cap_mark = 0;
+    BatchCounts* d_cnt = nullptr;
+    double* d_obs = nullptr;
+    // host mirrors, kept alive until the next rebuild (which first synchronises the stream)
+    std::vector<CellDesc> h_cells;
+    std::vector<uint2> h_tiles;
+    std::vector<int4> h_sph;
+    std::vector<int> h_msph;
+    BatchCounts h_cnt{};
+    bool dirty = true;
+    d3q19_tuning tuning{};
+    hipGraphExec_t graph2 = nullptr;
+    bool graph_valid = false;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+static BatchLaunch launch_of(const Batch* b) {
+    BatchLaunch L;
+    L.t.cells = b->d_cells;
+    L.t.tiles = b->d_tiles;
+    L.t.sph = b->d_sph;
+    L.t.msph = b->d_msph;
+    L.t.cnt = b->d_cnt;
+    L.t.epochs = b->epochs;
+    L.storage = b->bo.storage;
+    L.collision = b->bo.collision;
+    L.ibm_mode = b->bo.ibm_kernel;
+    L.tile = b->tuning.tile;
+    L.grid_collide = b->tuning.grid_collide;
+    L.grid_sphere = b->tuning.grid_sphere;
+    L.grid_ibm = b->tuning.grid_ibm;
+    return L;
+}
+
+static int active_slots(const Batch* b) {
+    int hi = 0;
+    for (int i = 0; i < (int)b->cells.size(); i++)
+        if (b->cells[i].sim) hi = i + 1;
+    return hi;
+}
+
+template <class T>
+static int grow(T** p, size_t* cap, size_t need, bool* moved) {
+    if (need <= *cap && *p) return 0;
+    size_t n = std::max<size_t>(need + need / 2, 1024);
+    if (*p) (void)hipFree(*p);
+    *p = nullptr;
+    hipError_t e = hipMalloc((void**)p, n * sizeof(T));
+    if (e != hipSuccess) {
+        *cap = 0;
+        return bfail(D3Q19_E_NOMEM, std::string("hipMalloc (batch work list): ") + hipGetErrorString(e));
+    }
+    *cap = n;
+    *moved = true;
+    return 0;
+}
+
+// Rebuild the work lists from the member set and upload them (stream ordered).
+static int upload_tables(Batch* b) {
+    HIP_OK(hipStreamSynchronize(b->stream));  // host mirrors may still be read by earlier copies
+    const int nslots = active_slots(b);
+    const int tile = b->tuning.tile;
+    b->h_tiles.clear();
+    b->h_sph.clear();
+    b->h_msph.clear();
+    for (int c = 0; c < b->max_cells; c++) {
+        Member& m = b->cells[c];
+        if (!m.sim) {
+            memset(&b->h_cells[c], 0, sizeof(CellDesc));
+            continue;
+        }
+        Sim* s = m.sim;
+        s->parity = m.par ^ b->phase;  // keep the single-simulation view consistent
+        CellDesc d = make_cell_desc(s);
+        d.par = m.par;
+        b->h_cells[c] = d;
+        for (int base = 0; base < s->g.nint; base += tile) b->h_tiles.push_back(make_uint2((unsigned)c, (unsigned)base));
+        for (int p = 0; p < s->np; p++) {
+            const int sidx = (int)b->h_sph.size();
+            b->h_sph.push_back(make_int4(c, p, (int)b->h_msph.size(), s->ppp));
+            for (int k = 0; k < s->ppp; k++) b->h_msph.push_back(sidx);
+        }
+    }
+    (void)nslots;
+    b->h_cnt.ntiles = (int)b->h_tiles.size();
+    b->h_cnt.nsph = (int)b->h_sph.size();
+    b->h_cnt.nmark = (int)b->h_msph.size();
+    b->h_cnt.ncells = active_slots(b);
+    bool moved = false;
+    int rc;
+    if ((rc = grow(&b->d_tiles, &b->cap_tiles, b->h_tiles.size(), &moved))) return rc;
+    if ((rc = grow(&b->d_sph, &b->cap_sph, b->h_sph.size(), &moved))) return rc;
+    if ((rc = grow(&b->d_msph, &b->cap_mark, b->h_msph.size(), &moved))) return rc;
+    if (moved) b->graph_valid = false;
+    HIP_OK(hipMemcpyAsync(b->d_cells, b->h_cells.data(), (size_t)b->max_cells * sizeof(CellDesc), hipMemcpyHostToDevice, b->stream));
+    if (!b->h_tiles.empty())
+        HIP_OK(hipMemcpyAsync(b->d_tiles, b->h_tiles.data(), b->h_tiles.size() * sizeof(uint2), hipMemcpyHostToDevice, b->stream));
+    if (!b->h_sph.empty())
+        HIP_OK(hipMemcpyAsync(b->d_sph, b->h_sph.data(), b->h_sph.size() * sizeof(int4), hipMemcpyHostToDevice, b->stream));
+    if (!b->h_msph.empty())
+        HIP_OK(hipMemcpyAsync(b->d_msph, b->h_msph.data(), b->h_msph.size() * sizeof(int), hipMemcpyHostToDevice, b->stream));
+    HIP_OK(hipMemcpyAsync(b->d_cnt, &b->h_cnt, sizeof(BatchCounts), hipMemcpyHostToDevice, b->stream));
+    b->dirty = false;
+    return 0;
+}
+
+// One step of every cell with step parity sp.
+static int enqueue_batch_step(Batch* b, int sp, int write_u_all, hipStream_t st) {
+    const BatchLaunch L = launch_of(b);
+    int rc;
+    if ((rc = launch_sphere_pre_batch(L, st))) return rc;
+    if ((rc = launch_collide_batch(L, sp, write_u_all, 0, st))) return rc;
+    if ((rc = launch_ibm_batch(L, st))) return rc;
+    return launch_sphere_post_batch(L, st);
+}
+
+static int capture_graph(Batch* b) {
+    if (b->graph2) {
+        (void)hipGraphExecDestroy(b->graph2);
+        b->graph2 = nullptr;
+    }
+    hipGraph_t g;
+    HIP_OK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    int rc = enqueue_batch_step(b, 0, 0, b->stream);
+    if (!rc) rc = enqueue_batch_step(b, 1, 0, b->stream);
+    hipError_t e = hipStreamEndCapture(b->stream, &g);
+    if (rc) return rc;
+    if (e != hipSuccess) return bfail(D3Q19_E_DEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    e = hipGraphInstantiate(&b->graph2, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return bfail(D3Q19_E_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    b->graph_valid = true;
+    return 0;
+}
+
+static int direct_step(Batch* b, int write_u_all) {
+    int rc = enqueue_batch_step(b, b->phase, write_u_all, b->stream);
+    if (rc) return rc;
+    b->phase ^= 1;
+    for (Member& m : b->cells)
+        if (m.sim) m.sim->parity = m.par ^ b->phase;
+    return 0;
+}
+
+static int batch_run(Batch* b, int n, int flags) {
+    if (n <= 0) return 0;
+    int rc;
+    if (b->dirty && (rc = upload_tables(b))) return rc;
+    if (b->h_cnt.ncells == 0) return 0;
+    const int wu = (flags & D3Q19_STEP_WRITE_U) ? 1 : 0;
+    int left = n;
+    if (b->phase == 1) {  // realign with the graph's parity
+        if ((rc = direct_step(b, (left == 1) ? wu : 0))) return rc;
+        left--;
+    }
+    const int body = wu ? left - 1 : left;
+    const int pairs = body > 0 ? body / 2 : 0;
+    if (pairs > 0) {
+        if (!b->graph_valid && (rc = capture_graph(b))) return rc;
+        for (int i = 0; i < pairs; i++) HIP_OK(hipGraphLaunch(b->graph2, b->stream));
+        left -= 2 * pairs;
+    }
+    while (left > 0) {
+        if ((rc = direct_step(b, (left == 1) ? wu : 0))) return rc;
+        left--;
+    }
+    return 0;
+}
+
+static void default_tuning(Batch* b) {
+    b->tuning.tile = 256;
+    b->tuning.grid_collide = 4 * b->ncu;
+    b->tuning.grid_sphere = 2 * b->ncu;
+    b->tuning.grid_ibm = 4 * b->ncu;
+    b->tuning.collide_ms = 0.0;
+}
+
+static int valid_tuning(const d3q19_tuning* t) {
+    if (t->tile != 64 && t->tile != 128 && t->tile != 256) return 0;
+    if (t->grid_collide < 1 || t->grid_sphere < 1 || t->grid_ibm < 1) return 0;
+    if (t->grid_collide > (1 << 20) || t->grid_sphere > (1 << 20) || t->grid_ibm > (1 << 20)) return 0;
+    return 1;
+}
+
+// mean duration of `reps` trial collide launches (state unchanged) under the current tuning
+static int time_probe(Batch* b, int reps, double* ms) {
+    const BatchLaunch L = launch_of(b);
+    int rc;
+    for (int i = 0; i < 2; i++)
+        if ((rc = launch_collide_batch(L, b->phase, 0, 1, b->stream))) return rc;
+    HIP_OK(hipEventRecord(b->ev0, b->stream));
+    for (int i = 0; i < reps; i++)
+        if ((rc = launch_collide_batch(L, b->phase, 0, 1, b->stream))) return rc;
+    HIP_OK(hipEventRecord(b->ev1, b->stream));
+    HIP_OK(hipEventSynchronize(b->ev1));
+    float t = 0;
+    HIP_OK(hipEventElapsedTime(&t, b->ev0, b->ev1));
+    *ms = (double)t / reps;
+    return 0;
+}
+
+static int autotune(Batch* b) {
+    int rc;
+    if (b->dirty && (rc = upload_tables(b))) return rc;
+    if (b->h_cnt.ncells == 0) return bfail(D3Q19_E_STATE, "autotune needs at least one cell");
+    const d3q19_tuning keep = b->tuning;
+    d3q19_tuning best = keep;
+    double best_ms = 1e30;
+    // candidates: tile of 64/128/256 nodes x k*256 threads per CU in flight (k blocks of 256
+    // threads per CU, or the same thread count in smaller tiles); k = 64 is close to one tile
+    // per block for batches of up to ~16 M nodes
+    const int tiles[3] = {64, 128, 256};
+    const int ks[6] = {1, 2, 4, 8, 16, 64};
+    for (int ti = 0; ti < 3; ti++) {
+        b->tuning.tile = tiles[ti];
+        b->dirty = true;
+        if ((rc = upload_tables(b))) return rc;
+        for (int pi = 0; pi < 6; pi++) {
+            const int grid = ks[pi] * b->ncu * (256 / tiles[ti]);
+            b->tuning.grid_collide = grid;
+            double ms = 0;
+            if ((rc = time_probe(b, 6, &ms))) return rc;
+            if (ms < best_ms) {
+                best_ms = ms;
+                best = b->tuning;
+            }
+        }
+    }
+    best.collide_ms = best_ms;
+    b->tuning = best;
+    b->dirty = true;
+    b->graph_valid = false;
+    return upload_tables(b);
+}
+
+static int batch_observables(Batch* b, const int* ids, int n, d3q19_observables* out) {
+    int rc;
+    if (b->dirty && (rc = upload_tables(b))) return rc;
+    const int nslots = active_slots(b);
+    if (nslots == 0) return 0;
+    if ((rc = launch_observables(b->d_cells, nslots, b->bo.storage, b->phase, b->d_obs, b->stream))) return rc;
+    std::vector<double> raw((size_t)OBS_STRIDE * nslots);
+    HIP_OK(hipMemcpyAsync(raw.data(), b->d_obs, raw.size() * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+    HIP_OK(hipStreamSynchronize(b->stream));
+    for (int i = 0; i < n; i++) {
+        const int c = ids[i];
+        if (c < 0 || c >= nslots || !b->cells[c].sim) return bfail(D3Q19_E_INVALID, "no such cell");
+        derive_observables(b->cells[c].sim, raw.data() + (size_t)OBS_STRIDE * c, &out[i]);
+    }
+    return 0;
+}
+
+static int batch_add(Batch* b, const int_param_struct* ip, const flp_param_struct* fp, const d3q19_options* opt,
+                     const d3q19_float4* pts, const d3q19_float4* kin, int* id) {
+    if (!ip || !fp || !id) return bfail(D3Q19_E_INVALID, "null argument");
+    d3q19_options o;
+    if (opt) o = *opt;
+    else memset(&o, 0, sizeof(o));
+    o.storage = b->bo.storage;
+    o.collision = b->bo.collision;
+    o.ibm_kernel = b->bo.ibm_kernel;
+    o.block[0] = o.block[1] = o.block[2] = 0;
+    int rc = sim_validate(ip, fp, &o);
+    if (rc) return rc;
+    if (ip->BoundaryConds[0] != 0 || ip->BoundaryConds[1] != 0 || ip->BoundaryConds[2] != 1)
+        return bfail(D3Q19_E_UNSUPPORTED, "batched cells are shear cells: periodic x and y, moving walls normal to z");
+    for (int d = 0; d < 3; d++)
+        if (o.tangential_bc[d] == 1) return bfail(D3Q19_E_UNSUPPORTED, "batched cells take no tangential wall passes");
+    const long long nint = (long long)(ip->LatticeSize[0] - 2) * (ip->LatticeSize[1] - 2) * (ip->LatticeSize[2] - 2);
+    if (nint >= (1LL << 24)) return bfail(D3Q19_E_UNSUPPORTED, "a batched cell holds fewer than 2^24 nodes");
+    if (ip->NumParticles > 0 && !pts) return bfail(D3Q19_E_INVALID, "sphere_points required when NumParticles > 0");
+    int slot = -1;
+    for (int i = 0; i < b->max_cells; i++)
+        if (!b->cells[i].sim) {
+            slot = i;
+            break;
+        }
+    if (slot < 0) return bfail(D3Q19_E_CAPACITY, "batch holds its maximum number of cells");
+    const size_t bytes = sim_state_bytes(ip, fp, &o);
+    HIP_OK(hipSetDevice(b->device));
+    HIP_OK(hipStreamSynchronize(b->stream));
+    void* block = b->arena.alloc(bytes);
+    if (!block) return bfail(D3Q19_E_CAPACITY, "batch arena cannot hold the cell");
+    Member& m = b->cells[slot];
+    m.block = block;
+    m.sub = Arena();
+    m.sub.base = (char*)block;
+    m.sub.size = bytes;
+    m.sub.free_[0] = bytes;
+    d3q19_sim* s = new d3q19_sim();
+    rc = sim_init(s, b->device, ip, fp, &o, pts, &m.sub, b->stream, b->epochs);
+    if (!rc && kin && s->np > 0) rc = d3q19_write_particles(s, kin, nullptr, nullptr);
+    if (rc) {
+        sim_release(s);
+        delete s;
+        b->arena.release(block);
+        m = Member();
+        return rc;
+    }
+    m.sim = s;
+    m.par = b->phase;  // f[par ^ phase] = fA, the initial state
+    s->parity = 0;
+    b->dirty = true;
+    *id = slot;
+    return 0;
+}
+
+static int batch_remove(Batch* b, int id) {
+    if (id < 0 || id >= b->max_cells || !b->cells[id].sim) return bfail(D3Q19_E_INVALID, "no such cell");
+    HIP_OK(hipSetDevice(b->device));
+    HIP_OK(hipStreamSynchronize(b->stream));
+    Member& m = b->cells[id];
+    sim_release(m.sim);
+    delete m.sim;
+    b->arena.release(m.block);
+    m = Member();
+    b->dirty = true;
+    return 0;
+}
+
+static void batch_free(Batch* b) {
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    for (Member& m : b->cells)
+        if (m.sim) {
+            sim_release(m.sim);
+            delete m.sim;
+        }
+    b->cells.clear();
+    void* ptrs[] = {b->arena.base, b->epochs, b->d_cells, b->d_tiles, b->d_sph, b->d_msph, b->d_cnt, b->d_obs};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    if (b->graph2) (void)hipGraphExecDestroy(b->graph2);
+    if (b->ev0) (void)hipEventDestroy(b->ev0);
+    if (b->ev1) (void)hipEventDestroy(b->ev1);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+}
+
+static int batch_create(int device, const d3q19_batch_options* bo, Batch** out) {
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) return bfail(D3Q19_E_DEVICE, "no such HIP device");
+    Batch* b = new Batch();
+    if (bo) b->bo = *bo;
+    else memset(&b->bo, 0, sizeof(b->bo));
+    if (b->bo.storage < 0 || b->bo.storage > 3 || (b->bo.collision != D3Q19_COLL_MRT && b->bo.collision != D3Q19_COLL_BGK) ||
+        (b->bo.ibm_kernel != D3Q19_IBM_LINEAR_SMOOTHED && b->bo.ibm_kernel != D3Q19_IBM_PESKIN4)) {
+        delete b;
+        return bfail(D3Q19_E_INVALID, "bad batch options");
+    }
+    b->device = device;
+    b->max_cells = b->bo.max_cells > 0 ? b->bo.max_cells : 1024;
+    if (b->max_cells > 65536) {
+        delete b;
+        return bfail(D3Q19_E_INVALID, "max_cells above 65536");
+    }
+    hipError_t e = hipSetDevice(device);
+    hipDeviceProp_t prop;
+    if (e == hipSuccess) e = hipGetDeviceProperties(&prop, device);
+    if (e == hipSuccess) b->ncu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+    size_t freeb = 0, totalb = 0;
+    if (e == hipSuccess) e = hipMemGetInfo(&freeb, &totalb);
+    size_t ab = b->bo.arena_bytes ? (size_t)b->bo.arena_bytes : freeb / 2;
+    ab &= ~(size_t)255;
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipMalloc((void**)&b->arena.base, ab);
+    if (e == hipSuccess) {
+        b->arena.size = ab;
+        b->arena.free_[0] = ab;
+    }
+    if (e == hipSuccess) e = hipMalloc((void**)&b->epochs, 2 * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMemsetAsync(b->epochs, 0, 2 * sizeof(uint32_t), b->stream);
+    if (e == hipSuccess) e = hipMalloc((void**)&b->d_cells, (size_t)b->max_cells * sizeof(CellDesc));
+    if (e == hipSuccess) e = hipMalloc((void**)&b->d_cnt, sizeof(BatchCounts));
+    if (e == hipSuccess) e = hipMalloc((void**)&b->d_obs, (size_t)OBS_STRIDE * b->max_cells * sizeof(double));
+    if (e == hipSuccess) e = hipEventCreate(&b->ev0);
+    if (e == hipSuccess) e = hipEventCreate(&b->ev1);
+    if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+    if (e != hipSuccess) {
+        const int code = (e == hipErrorOutOfMemory) ? D3Q19_E_NOMEM : D3Q19_E_DEVICE;
+        const std::string msg = std::string("batch creation: ") + hipGetErrorString(e);
+        batch_free(b);
+        delete b;
+        return bfail(code, msg);
+    }
+    b->cells.resize(b->max_cells);
+    b->h_cells.resize(b->max_cells);
+    default_tuning(b);
+    b->dirty = true;
+    *out = b;
+    return 0;
+}
+
+// ---- serving -----------------------------------------------------------------------------------------
+static double now_s() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+struct Request {
+    int64_t id = 0;
+    int_param_struct ip;
+    flp_param_struct fp;
+    d3q19_options opt;
+    std::vector<d3q19_float4> pts, kin;
+    d3q19_solve_params sp;
+    double t_submit = 0, t_admit = 0;
+    int cell = -1;
+    int steps = 0;
+    double last = 0.0;
+    bool have_last = false;
+};
+
+struct Server {
+    Batch* b = nullptr;
+    std::deque<Request*> queue;
+    std::vector<Request*> resident;
+    std::deque<d3q19_result> outbox;  // finished, not yet handed to the caller
+    int64_t next_id = 1;
+    int interval = 0;  // steps per scheduling round = the requests' check interval
+};
+
+// convergence rule shared by the server and the sequential solve; returns 1 when finished
+static int check_request(Request* r, const d3q19_observables& o, d3q19_result* res) {
+    const double sw = o.wall_stress;
+    int status = -99;
+    if (!std::isfinite(sw)) status = D3Q19_E_NONFINITE;
+    else if (r->have_last && r->steps >= r->sp.min_steps && fabs(sw - r->last) <= r->sp.tolerance * fabs(sw)) status = 0;
+    else if (r->steps >= r->sp.max_steps) status = 1;
+    r->last = sw;
+    r->have_last = true;
+    if (status == -99) return 0;
+    memset(res, 0, sizeof(*res));
+    res->id = r->id;
+    res->status = status;
+    res->steps = r->steps;
+    res->obs = o;
+    res->t_submit = r->t_submit;
+    res->t_admit = r->t_admit;
+    res->t_finish = now_s();
+    return 1;
+}
+
+static int valid_solve_params(const d3q19_solve_params* sp) {
+    return sp && sp->check_interval > 0 && (sp->check_interval % 2) == 0 && sp->max_steps > 0 && sp->tolerance >= 0.0 && sp->min_steps >= 0;
+}
+
+static void drain(Server* v, d3q19_result* done, int cap, int* n_done) {
+    while (!v->outbox.empty() && *n_done < cap) {
+        done[(*n_done)++] = v->outbox.front();
+        v->outbox.pop_front();
+    }
+}
+
+static int server_progress(Server* v, d3q19_result* done, int cap, int* n_done) {
+    *n_done = 0;
+    Batch* b = v->b;
+    int rc;
+    // admission in arrival order while the arena holds the next request
+    while (!v->queue.empty()) {
+        Request* r = v->queue.front();
+        int id = -1;
+        rc = batch_add(b, &r->ip, &r->fp, &r->opt, r->pts.empty() ? nullptr : r->pts.data(), r->kin.empty() ? nullptr : r->kin.data(), &id);
+        if (rc == D3Q19_E_CAPACITY) {
+            if (v->resident.empty()) return bfail(D3Q19_E_CAPACITY, "request larger than the whole batch arena");
+            break;
+        }
+        if (rc) return rc;
+        r->cell = id;
+        r->t_admit = now_s();
+        v->queue.pop_front();
+        v->resident.push_back(r);
+    }
+    if (v->resident.empty()) {
+        drain(v, done, cap, n_done);
+        return 0;
+    }
+    if ((rc = batch_run(b, v->interval, 0))) return rc;
+    std::vector<int> ids;
+    for (Request* r : v->resident) ids.push_back(r->cell);
+    std::vector<d3q19_observables> obs(ids.size());
+    if ((rc = batch_observables(b, ids.data(), (int)ids.size(), obs.data()))) return rc;
+    std::vector<Request*> keep;
+    for (size_t i = 0; i < v->resident.size(); i++) {
+        Request* r = v->resident[i];
+        r->steps += v->interval;
+        d3q19_result res;
+        if (check_request(r, obs[i], &res)) {
+            v->outbox.push_back(res);
+            rc = batch_remove(b, r->cell);
+            delete r;
+            if (rc) return rc;
+        } else {
+            keep.push_back(r);
+        }
+    }
+    v->resident.swap(keep);
+    drain(v, done, cap, n_done);
+    return 0;
+}
+
+}  // namespace d3q19
+
+struct d3q19_batch : public d3q19::Batch {};
+struct d3q19_server : public d3q19::Server {};
+
+#define CHECK_BATCH(b)                                                         \
+    do {                                                                       \
+        if (!(b)) return d3q19::bfail(D3Q19_E_INVALID, "null batch");          \
+        HIP_OK(hipSetDevice((b)->device));                                     \
+    } while (0)
+
+extern "C" {
+
+int d3q19_batch_create(int device, const d3q19_batch_options* bo, d3q19_batch** out) {
+    if (!out) return bfail(D3Q19_E_INVALID, "null output");
+    *out = nullptr;
+    Batch* b = nullptr;
+    int rc = batch_create(device, bo, &b);
+    if (rc) return rc;
+    *out = static_cast<d3q19_batch*>(b);
+    return 0;
+}
+
+int d3q19_batch_destroy(d3q19_batch* b) {
+    if (!b) return 0;
+    (void)hipSetDevice(b->device);
+    batch_free(b);
+    delete static_cast<Batch*>(b);
+    return 0;
+}
+
+int d3q19_batch_add(d3q19_batch* b, const int_param_struct* ip, const flp_param_struct* fp, const d3q19_options* opt,
+                    const d3q19_float4* pts, const d3q19_float4* kin, int* id) {
+    CHECK_BATCH(b);
+    return batch_add(b, ip, fp, opt, pts, kin, id);
+}
+
+int d3q19_batch_remove(d3q19_batch* b, int id) {
+    CHECK_BATCH(b);
+    return batch_remove(b, id);
+}
+
+int d3q19_batch_cell_bytes(const d3q19_batch* b, const int_param_struct* ip, size_t* bytes) {
+    if (!b || !ip || !bytes) return bfail(D3Q19_E_INVALID, "null argument");
+    d3q19_options o;
+    memset(&o, 0, sizeof(o));
+    o.storage = b->bo.storage;
+    flp_param_struct fp;
+    memset(&fp, 0, sizeof(fp));
+    *bytes = sim_state_bytes(ip, &fp, &o);
+    return 0;
+}
+
+int d3q19_batch_memory(const d3q19_batch* b, size_t* arena_bytes, size_t* used_bytes, int* cells) {
+    if (!b) return bfail(D3Q19_E_INVALID, "null batch");
+    if (arena_bytes) *arena_bytes = b->arena.size;
+    if (used_bytes) *used_bytes = b->arena.used;
+    if (cells) {
+        int n = 0;
+        for (const Member& m : b->cells)
+            if (m.sim) n++;
+        *cells = n;
+    }
+    return 0;
+}
+
+int d3q19_batch_run(d3q19_batch* b, int n, int flags) {
+    CHECK_BATCH(b);
+    return batch_run(b, n, flags);
+}
+
+int d3q19_batch_sync(d3q19_batch* b) {
+    CHECK_BATCH(b);
+    HIP_OK(hipStreamSynchronize(b->stream));
+    return 0;
+}
+
+int d3q19_batch_observables(d3q19_batch* b, const int* ids, int n, d3q19_observables* out) {
+    CHECK_BATCH(b);
+    if (n < 0 || (n > 0 && (!ids || !out))) return bfail(D3Q19_E_INVALID, "bad observables request");
+    return batch_observables(b, ids, n, out);
+}
+
+static d3q19_sim* member(d3q19_batch* b, int id) {
+    if (!b || id < 0 || id >= b->max_cells || !b->cells[id].sim) return nullptr;
+    Member& m = b->cells[id];
+    m.sim->parity = m.par ^ b->phase;
+    return m.sim;
+}
+
+int d3q19_batch_read_u(d3q19_batch* b, int id, float* u) {
+    CHECK_BATCH(b);
+    d3q19_sim* s = member(b, id);
+    if (!s) return bfail(D3Q19_E_INVALID, "no such cell");
+    return d3q19_read_u(s, u);
+}
+
+int d3q19_batch_read_f(d3q19_batch* b, int id, float* f) {
+    CHECK_BATCH(b);
+    d3q19_sim* s = member(b, id);
+    if (!s) return bfail(D3Q19_E_INVALID, "no such cell");
+    return d3q19_read_f(s, f);
+}
+
+int d3q19_batch_read_particles_f64(d3q19_batch* b, int id, double* pos, double* vel, double* av, double* ff, double* tf) {
+    CHECK_BATCH(b);
+    d3q19_sim* s = member(b, id);
+    if (!s) return bfail(D3Q19_E_INVALID, "no such cell");
+    return d3q19_read_particles_f64(s, pos, vel, av, ff, tf);
+}
+
+int d3q19_batch_autotune(d3q19_batch* b, d3q19_tuning* out) {
+    CHECK_BATCH(b);
+    int rc = autotune(b);
+    if (rc) return rc;
+    if (out) *out = b->tuning;
+    return 0;
+}
+
+int d3q19_batch_get_tuning(const d3q19_batch* b, d3q19_tuning* out) {
+    if (!b || !out) return bfail(D3Q19_E_INVALID, "null argument");
+    *out = b->tuning;
+    return 0;
+}
+
+int d3q19_batch_set_tuning(d3q19_batch* b, const d3q19_tuning* t) {
+    CHECK_BATCH(b);
+    if (!t || !valid_tuning(t)) return bfail(D3Q19_E_INVALID, "bad tuning");
+    HIP_OK(hipStreamSynchronize(b->stream));
+    b->tuning = *t;
+    b->dirty = true;
+    b->graph_valid = false;
+    return 0;
+}
+
+int d3q19_batch_time_steps(d3q19_batch* b, int n, double* total_ms, double* collide_ms) {
+    CHECK_BATCH(b);
+    if (n <= 0 || !total_ms) return bfail(D3Q19_E_INVALID, "bad timing request");
+    int rc;
+    if (b->dirty && (rc = upload_tables(b))) return rc;
+    if (n >= 2 && !b->graph_valid && (rc = capture_graph(b))) return rc;
+    HIP_OK(hipStreamSynchronize(b->stream));
+    HIP_OK(hipEventRecord(b->ev0, b->stream));
+    if ((rc = batch_run(b, n, 0))) return rc;
+    HIP_OK(hipEventRecord(b->ev1, b->stream));
+    HIP_OK(hipEventSynchronize(b->ev1));
+    float t = 0;
+    HIP_OK(hipEventElapsedTime(&t, b->ev0, b->ev1));
+    *total_ms = (double)t;
+    if (collide_ms && (rc = time_probe(b, 16, collide_ms))) return rc;
+    return 0;
+}
+
+int d3q19_server_create(int device, const d3q19_batch_options* bo, d3q19_server** out) {
+    if (!out) return bfail(D3Q19_E_INVALID, "null output");
+    *out = nullptr;
+    Batch* b = nullptr;
+    int rc = batch_create(device, bo, &b);
+    if (rc) return rc;
+    d3q19_server* v = new d3q19_server();
+    v->b = b;
+    *out = v;
+    return 0;
+}
+
+int d3q19_server_destroy(d3q19_server* v) {
+    if (!v) return 0;
+    for (Request* r : v->queue) delete r;
+    for (Request* r : v->resident) delete r;
+    if (v->b) d3q19_batch_destroy(static_cast<d3q19_batch*>(v->b));
+    delete v;
+    return 0;
+}
+
+int d3q19_server_submit(d3q19_server* v, const int_param_struct* ip, const flp_param_struct* fp, const d3q19_options* opt,
+                        const d3q19_float4* pts, const d3q19_float4* kin, const d3q19_solve_params* sp, int64_t* id) {
+    if (!v || !ip || !fp || !id) return bfail(D3Q19_E_INVALID, "null argument");
+    if (!valid_solve_params(sp)) return bfail(D3Q19_E_INVALID, "bad solve parameters (check_interval must be even and > 0)");
+    if (v->interval == 0) v->interval = sp->check_interval;
+    if (sp->check_interval != v->interval) return bfail(D3Q19_E_INVALID, "all requests of a server share one check_interval");
+    if (ip->NumParticles > 0 && !pts) return bfail(D3Q19_E_INVALID, "sphere_points required when NumParticles > 0");
+    Request* r = new Request();
+    r->id = v->next_id++;
+    r->ip = *ip;
+    r->fp = *fp;
+    if (opt) r->opt = *opt;
+    else memset(&r->opt, 0, sizeof(r->opt));
+    r->sp = *sp;
+    if (ip->NumParticles > 0) {
+        r->pts.assign(pts, pts + ip->PointsPerParticle);
+        if (kin) r->kin.assign(kin, kin + 4 * (size_t)ip->NumParticles);
+    }
+    r->t_submit = now_s();
+    v->queue.push_back(r);
+    *id = r->id;
+    return 0;
+}
+
+int d3q19_server_progress(d3q19_server* v, d3q19_result* done, int cap, int* n_done) {
+    if (!v || !n_done || (cap > 0 && !done)) return bfail(D3Q19_E_INVALID, "null argument");
+    HIP_OK(hipSetDevice(v->b->device));
+    return server_progress(v, done, cap, n_done);
+}
+
+int d3q19_server_status(const d3q19_server* v, int* queued, int* resident, size_t* used) {
+    if (!v) return bfail(D3Q19_E_INVALID, "null server");
+    if (queued) *queued = (int)v->queue.size();
+    if (resident) *resident = (int)(v->resident.size() + v->outbox.size());
+    if (used) *used = v->b->arena.used;
+    return 0;
+}
+
+int d3q19_server_get_batch(d3q19_server* v, d3q19_batch** b) {
+    if (!v || !b) return bfail(D3Q19_E_INVALID, "null argument");
+    *b = static_cast<d3q19_batch*>(v->b);
+    return 0;
+}
+
+int d3q19_solve(int device, const int_param_struct* ip, const flp_param_struct* fp, const d3q19_options* opt,
+                const d3q19_float4* pts, const d3q19_float4* kin, const d3q19_solve_params* sp, d3q19_result* out) {
+    if (!ip || !fp || !out) return bfail(D3Q19_E_INVALID, "null argument");
+    if (!valid_solve_params(sp)) return bfail(D3Q19_E_INVALID, "bad solve parameters (check_interval must be even and > 0)");
+    Request r;
+    r.id = 0;
+    r.sp = *sp;
+    r.t_submit = r.t_admit = now_s();
+    d3q19_sim* s = nullptr;
+    int rc = d3q19_create(device, ip, fp, opt, pts, &s);
+    if (rc) return rc;
+    if (kin && ip->NumParticles > 0) rc = d3q19_write_particles(s, kin, nullptr, nullptr);
+    int t = 1;
+    while (!rc) {
+        if ((rc = d3q19_run(s, t, sp->check_interval, 0))) break;
+        t += sp->check_interval;
+        r.steps += sp->check_interval;
+        d3q19_observables o;
+        if ((rc = d3q19_observables_get(s, &o))) break;
+        if (check_request(&r, o, out)) break;
+    }
+    d3q19_destroy(s);
+    return rc;
+}
+
+double d3q19_clock(void) { return now_s(); }
+
+}  // extern "C"

@@ -13,20 +13,6 @@
 
 namespace d3q19 {
 
-struct IbmArgs {
-    Geom g;
-    int np, ppp, mode;
-    double dA;
-    const double* pos;
-    const double* vel;
-    const double* omega;
-    const float* pts;
-    const void* u;
-    long long* facc;
-    double* mforce;
-    uint32_t* epochs;
-};
-
 __device__ __forceinline__ double peskin4(double r) {
     r = fabs(r);
     if (r < 1.0) return (3.0 - 2.0 * r + sqrt(1.0 + 4.0 * r - 4.0 * r * r)) * 0.125;
@@ -51,20 +37,11 @@ __device__ __forceinline__ double wave_sum(double v) {
 // One 64-lane wavefront per marker; lane k = (kx, ky, kz) in [0,4)^3 owns one node of the
 // marker's 4x4x4 support.  Interpolation is a wave reduction of the lanes' weighted velocities;
 // spreading is three integer atomics per lane.
+// Marker m of sphere p (mforce slot p*ppp + m), executed by one whole wavefront.
 template <class R, int MODE>
-__global__ void __launch_bounds__(256) ibm_marker_kernel(IbmArgs a) {
-    const int lane = threadIdx.x & 63;
-    const int gid = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    if (gid >= a.np * a.ppp) return;  // uniform per wavefront
-    if (gid == 0 && lane == 0) {
-        // the step's epoch (read as epochs[0] + 1 by the kernels before this one) is committed,
-        // and its spread force is now the pending one
-        const uint32_t e = a.epochs[0] + 1u;
-        a.epochs[0] = e;
-        a.epochs[1] = e;
-    }
+__device__ __forceinline__ void ibm_marker(const IbmArgs& a, int p, int m, int lane) {
+    const int gid = p * a.ppp + m;
     const Geom& g = a.g;
-    const int p = gid / a.ppp, m = gid - p * a.ppp;
     const double r0[3] = {(double)a.pts[4 * m], (double)a.pts[4 * m + 1], (double)a.pts[4 * m + 2]};
     const int nd[3] = {g.nx, g.ny, g.nz};
     double rpp[3];
@@ -159,8 +136,42 @@ __global__ void __launch_bounds__(256) ibm_marker_kernel(IbmArgs a) {
     }
 }
 
-int launch_ibm(Sim* s, hipStream_t st) {
-    if (s->np == 0) return 0;
+template <class R, int MODE>
+__global__ void __launch_bounds__(256) ibm_marker_kernel(IbmArgs a) {
+    const int lane = threadIdx.x & 63;
+    const int gid = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (gid >= a.np * a.ppp) return;  // uniform per wavefront
+    if (gid == 0 && lane == 0) {
+        // the step's epoch (read as epochs[0] + 1 by the kernels before this one) is committed,
+        // and its spread force is now the pending one
+        const uint32_t e = a.epochs[0] + 1u;
+        a.epochs[0] = e;
+        a.epochs[1] = e;
+    }
+    const int p = gid / a.ppp;
+    ibm_marker<R, MODE>(a, p, gid - p * a.ppp, lane);
+}
+
+// Batched: a persistent grid of wavefronts walks the marker list of all cells; the batch's one
+// epoch counter is committed by the first lane of the grid.
+template <class R, int MODE>
+__global__ void __launch_bounds__(256) ibm_batch_kernel(BatchTables t) {
+    const int lane = threadIdx.x & 63;
+    const int wpb = blockDim.x >> 6;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        const uint32_t e = t.epochs[0] + 1u;
+        t.epochs[0] = e;
+        t.epochs[1] = e;
+    }
+    const int nmark = t.cnt->nmark;
+    for (int gid = blockIdx.x * wpb + (int)(threadIdx.x >> 6); gid < nmark; gid += gridDim.x * wpb) {
+        const int sph = t.msph[gid];
+        const int4 d = t.sph[sph];  // cell, local sphere, first global marker, markers per sphere
+        ibm_marker<R, MODE>(t.cells[d.x].i, d.y, gid - d.z, lane);
+    }
+}
+
+IbmArgs make_ibm_args(Sim* s) {
     IbmArgs a;
     a.g = s->g;
     a.np = s->np;
@@ -175,6 +186,12 @@ int launch_ibm(Sim* s, hipStream_t st) {
     a.facc = s->facc;
     a.mforce = s->mforce;
     a.epochs = s->epochs;
+    return a;
+}
+
+int launch_ibm(Sim* s, hipStream_t st) {
+    if (s->np == 0) return 0;
+    const IbmArgs a = make_ibm_args(s);
     const int total = s->np * s->ppp;
     const int th = 256, blocks = (total + 3) / 4;  // four markers (wavefronts) per block
     const bool dbl = s->storage == D3Q19_STORE_FP64;
@@ -184,6 +201,20 @@ int launch_ibm(Sim* s, hipStream_t st) {
     } else {
         if (dbl) hipLaunchKernelGGL((ibm_marker_kernel<double, D3Q19_IBM_LINEAR_SMOOTHED>), dim3(blocks), dim3(th), 0, st, a);
         else hipLaunchKernelGGL((ibm_marker_kernel<float, D3Q19_IBM_LINEAR_SMOOTHED>), dim3(blocks), dim3(th), 0, st, a);
+    }
+    HIP_OK(hipGetLastError());
+    return 0;
+}
+
+int launch_ibm_batch(const BatchLaunch& L, hipStream_t st) {
+    const bool dbl = L.storage == D3Q19_STORE_FP64;
+    const dim3 grid(L.grid_ibm), block(256);
+    if (L.ibm_mode == D3Q19_IBM_PESKIN4) {
+        if (dbl) hipLaunchKernelGGL((ibm_batch_kernel<double, D3Q19_IBM_PESKIN4>), grid, block, 0, st, L.t);
+        else hipLaunchKernelGGL((ibm_batch_kernel<float, D3Q19_IBM_PESKIN4>), grid, block, 0, st, L.t);
+    } else {
+        if (dbl) hipLaunchKernelGGL((ibm_batch_kernel<double, D3Q19_IBM_LINEAR_SMOOTHED>), grid, block, 0, st, L.t);
+        else hipLaunchKernelGGL((ibm_batch_kernel<float, D3Q19_IBM_LINEAR_SMOOTHED>), grid, block, 0, st, L.t);
     }
     HIP_OK(hipGetLastError());
     return 0;

@@ -18,6 +18,7 @@
 #include <hip/hip_runtime.h>
 #include <type_traits>
 #include <stdint.h>
+#include <map>
 #include <string>
 #include "d3q19_abi.h"
 
@@ -104,6 +105,123 @@ struct FluidParams {
     int tan_mask;         // bit a: tangential velocity BC on axis a
 };
 
+// ---- per-cell kernel arguments ------------------------------------------------------------------
+// The same structs drive the single-simulation kernels (passed by value) and the batched kernels
+// (read from a device table of CellDesc, one per cell).
+struct CollideArgs {
+    const void* fin;
+    void* fout;
+    void* u;
+    void* tau;
+    long long* facc;
+    const uint32_t* mask0;
+    const uint32_t* mask1;
+    const uint32_t* epochs;
+    const void* bcraw;
+    Geom g;
+    FluidParams fl;
+    int write_u_all;
+    int use_umask;
+    int use_bcraw;
+    int has_force;
+    int vt;                // per-node tau (runtime form of the kernels' VT template argument)
+    int bx, by;            // block tile
+    int nbx, nby;          // tiles per plane
+    int nblk;
+};
+
+struct IbmArgs {
+    Geom g;
+    int np, ppp, mode;
+    double dA;
+    const double* pos;
+    const double* vel;
+    const double* omega;
+    const float* pts;
+    const void* u;
+    long long* facc;
+    double* mforce;
+    uint32_t* epochs;
+};
+
+struct PartArgs {
+    int np, ppp;
+    Geom g;
+    double sys[3];
+    double gconst[3];
+    double mass, inertia;
+    double diam;
+    double kcol;
+    double radius;  // brick-marking radius
+    int force_model;
+    int visc_model;
+    double tau0;
+    double zbuf;
+    double* pos;
+    double* vel;
+    double* quat;
+    double* omega;
+    double* ffluid;
+    double* tfluid;
+    double* fcont;
+    double* tcont;
+    const double* mforce;
+    uint32_t* mask0;
+    uint32_t* mask1;
+    const uint32_t* epochs;
+};
+
+// Observable definition of a cell: wall-normal axis A, flow axis F (largest wall-velocity
+// difference among the other axes), MRT rate of the F-A stress moment.
+struct ObsArgs {
+    int axisA, axisF;
+    int coll;            // D3Q19_COLL_*
+    float rate_fixed;    // > 0: the F-A stress moment relaxes at this fixed rate (MRT x-y moment)
+};
+
+// One cell of a batch (or the single simulation): everything its kernels need.  f[par ^ sp] holds
+// the post-collision state before a step of parity sp.
+struct CellDesc {
+    CollideArgs c;
+    IbmArgs i;
+    PartArgs p;
+    ObsArgs o;
+    const void* f[2];
+    int par;
+    int nint;
+    int nx, nxy;
+    float inv_nx, inv_nxy;  // flat node index -> (x, y, z) by float reciprocal + exact correction
+};
+
+// Work-list sizes of a batch, read by the persistent kernels from device memory so that a captured
+// graph stays valid while cells join and leave.
+struct BatchCounts {
+    int ntiles;   // collide tiles (tile = blockDim consecutive nodes of one cell)
+    int nsph;     // spheres over all cells
+    int nmark;    // markers over all cells
+    int ncells;
+};
+
+struct BatchTables {
+    const CellDesc* cells;
+    const uint2* tiles;      // (cell, first node)
+    const int4* sph;         // (cell, local sphere, first global marker, markers per sphere)
+    const int* msph;         // global marker -> global sphere
+    const BatchCounts* cnt;
+    uint32_t* epochs;        // the batch's epoch counter (shared by all its cells)
+};
+
+// ---- device memory arena: one allocation, first-fit sub-allocation (256-byte aligned) ------------
+struct Arena {
+    char* base = nullptr;
+    size_t size = 0;
+    size_t used = 0;
+    std::map<size_t, size_t> free_;  // offset -> length
+    std::map<size_t, size_t> live_;  // offset -> length
+    void* alloc(size_t bytes);
+    bool release(void* p);
+};
+
 // ---- simulation object ------------------------------------------------------------------------
 struct Sim {
     int device = 0;
@@ -148,9 +266,31 @@ struct Sim {
     int parity = 0;            // 0: current post-collision field in fA
     int block[3] = {64, 4, 1};
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // batch membership: device memory comes from the batch's arena, the batch's stream and epoch
+    // counter are used, and the batch owns neither
+    Arena* arena = nullptr;
+    bool shares_stream = false;
+    bool shares_epochs = false;
+    // single-cell descriptor for the observables kernel (host copy kept for the async upload)
+    CellDesc* d_desc = nullptr;
+    double* d_obs = nullptr;
+    CellDesc h_desc;
 };
 
-// ---- launchers (defined in the .hip files) --------------------------------------------------------
+// allocation + initial state (shared by d3q19_create and batch members) and release
+int sim_init(Sim* s, int device, const int_param_struct* ip, const flp_param_struct* fp, const d3q19_options* opt,
+             const d3q19_float4* sphere_points, Arena* arena, hipStream_t shared_stream, uint32_t* shared_epochs);
+void sim_release(Sim* s);
+int sim_validate(const int_param_struct* ip, const flp_param_struct* fp, const d3q19_options* opt);
+size_t sim_state_bytes(const int_param_struct* ip, const flp_param_struct* fp, const d3q19_options* opt);
+// observables of one cell from the 8 raw sums of the observables kernel
+void derive_observables(const Sim* s, const double* raw, d3q19_observables* out);
+
+// ---- argument builders and launchers (defined in the .hip files) ---------------------------------
+CollideArgs make_collide_args(Sim* s, const void* fin, void* fout, int write_u_all, int use_maskU);
+IbmArgs make_ibm_args(Sim* s);
+PartArgs make_part_args(Sim* s);
+CellDesc make_cell_desc(Sim* s);
 int launch_collide(Sim* s, const void* fin, void* fout, int write_u_all, int use_maskU, hipStream_t st);
 int launch_import_f(Sim* s, const void* ref_f, int is_double, void* fdst);
 int launch_export_f(Sim* s, const void* fsrc, void* ref_out, int as_double);
@@ -164,11 +304,34 @@ int launch_ibm(Sim* s, hipStream_t st);
 int launch_sphere_pre(Sim* s, int do_dyn, int do_mark, hipStream_t st);       // dynamics, brick marks
 int launch_sphere_post(Sim* s, int do_reduce, int do_contacts, hipStream_t st); // force sums, contacts
 int launch_fill_equilibrium(Sim* s, void* f);
+int launch_fill_tau(Sim* s);
+
+// observables: per cell 8 doubles (FP64): [sum of the F-A shear stress over the lower wall-adjacent
+// plane, node count, same for the upper plane, sum over spheres of |fluid force|, spheres, 0, 0]
+constexpr int OBS_STRIDE = 8;
+int launch_observables(const CellDesc* cells, int ncells, int storage, int sp, double* out, hipStream_t st);
+
+// batched (persistent) kernels over the work lists of BatchTables; grid sizes are fixed per batch
+struct BatchLaunch {
+    BatchTables t;
+    int storage, collision, ibm_mode;
+    int tile;              // threads (= nodes) per collide tile
+    int grid_collide;      // persistent blocks of the collide kernel
+    int grid_sphere;       // persistent blocks of the sphere kernels
+    int grid_ibm;          // persistent blocks (4 markers each per iteration) of the IBM kernel
+};
+int launch_collide_batch(const BatchLaunch& L, int sp, int write_u_all, int probe, hipStream_t st);
+int launch_ibm_batch(const BatchLaunch& L, hipStream_t st);
+int launch_sphere_pre_batch(const BatchLaunch& L, hipStream_t st);
+int launch_sphere_post_batch(const BatchLaunch& L, hipStream_t st);
 
 // error reporting
 void set_error(const std::string& msg);
 
 }  // namespace d3q19
+
+// the opaque handle of the C-ABI is the simulation object itself
+struct d3q19_sim : public d3q19::Sim {};
 
 #define HIP_OK(expr)                                                                       \
     do {                                                                                   \

@@ -29,12 +29,14 @@ template <> struct Pop<D3Q19_STORE_FP32> {
     using T = float;
     using R = float;
     template <int QI> __device__ static R ld(const T* p, unsigned off) { return p[off]; }
+    template <int QI> __device__ static double ldd(const T* p, unsigned off) { return (double)p[off]; }
     template <int QI> __device__ static void st(T* p, long long off, R v) { p[off] = v; }
 };
 template <> struct Pop<D3Q19_STORE_FP64> {
     using T = double;
     using R = double;
     template <int QI> __device__ static R ld(const T* p, unsigned off) { return p[off]; }
+    template <int QI> __device__ static double ldd(const T* p, unsigned off) { return p[off]; }
     template <int QI> __device__ static void st(T* p, long long off, R v) { p[off] = v; }
 };
 // binary16 of the relative deviation (f_q - w_q)/w_q scaled by 2^14: deviations down to ~1e-8
@@ -46,6 +48,9 @@ template <> struct Pop<D3Q19_STORE_FP16_SHIFT> {
         constexpr float w = (float)wq(QI);
         constexpr float sc = (float)(wq(QI) / 16384.0);
         return fmaf((float)p[off], sc, w);
+    }
+    template <int QI> __device__ static double ldd(const T* p, unsigned off) {
+        return wq(QI) + (double)p[off] * (wq(QI) / 16384.0);
     }
     template <int QI> __device__ static void st(T* p, long long off, R v) {
         constexpr float w = (float)wq(QI);
@@ -60,6 +65,9 @@ template <> struct Pop<D3Q19_STORE_BF16_SHIFT> {
     template <int QI> __device__ static R ld(const T* p, unsigned off) {
         constexpr float w = (float)wq(QI);
         return w + bf16_to_f32(p[off]);
+    }
+    template <int QI> __device__ static double ldd(const T* p, unsigned off) {
+        return wq(QI) + (double)bf16_to_f32(p[off]);
     }
     template <int QI> __device__ static void st(T* p, long long off, R v) {
         constexpr float w = (float)wq(QI);
@@ -91,27 +99,6 @@ template <> __device__ __forceinline__ float pow_<float>(float x, float y) {
 template <class R> __device__ __forceinline__ R exp_(R x) { return exp(x); }
 template <> __device__ __forceinline__ float exp_<float>(float x) { return __builtin_amdgcn_exp2f(x * 1.4426950408889634f); }
 
-struct CollideArgs {
-    const void* fin;
-    void* fout;
-    void* u;
-    void* tau;
-    long long* facc;
-    const uint32_t* mask0;
-    const uint32_t* mask1;
-    const uint32_t* epochs;
-    const void* bcraw;
-    Geom g;
-    FluidParams fl;
-    int write_u_all;
-    int use_umask;
-    int use_bcraw;
-    int has_force;
-    int bx, by;            // block tile
-    int nbx, nby;          // tiles per plane
-    int nblk;
-};
-
 // ---- moving-wall unknowns (GPU_program.cl:737-865) -----------------------------------------------
 // Axis A, side S (0: lower wall, inward +A; 1: upper wall, inward -A).  Knowns: populations with
 // zero or outward A-component.  Unknown q = its opposite + rho*(c_q . U)/3 (axis) or /6
@@ -120,8 +107,8 @@ struct CollideArgs {
 // rho from the knowns and the prescribed u_n.  Once after an upload the raw uploaded unknowns
 // are used instead (layout [side][k][plane], k = rank of q among the unknowns).
 // Unknown values are selected, not branched on, so every store to f[] has a fixed index.
-template <int A, int S, class R>
-__device__ __forceinline__ void wall_layer(R* f, const float* uw, int calc_rho, int use_raw, const R* raw, int pidx, int psz) {
+template <int A, int S, class R, class RW>
+__device__ __forceinline__ void wall_layer(R* f, const float* uw, int calc_rho, int use_raw, const RW* raw, int pidx, int psz) {
     constexpr int inward = S == 0 ? 1 : -1;
     constexpr int a1 = A == 0 ? 1 : 0;
     constexpr int a2 = A == 2 ? 1 : 2;
@@ -148,7 +135,7 @@ __device__ __forceinline__ void wall_layer(R* f, const float* uw, int calc_rho, 
             constexpr bool axial = (cc(q, a1) == 0 && cc(q, a2) == 0);
             const R cu = un + (R)cc(q, a1) * ua1 + (R)cc(q, a2) * ua2;
             R v = f[o] + rho * cu * (axial ? (R)(1.0 / 3.0) : (R)(1.0 / 6.0));
-            if (use_raw) v = raw[((size_t)(S * 5 + k)) * psz + pidx];
+            if (use_raw) v = (R)raw[((size_t)(S * 5 + k)) * psz + pidx];
             f[q] = v;
             k++;
         }
@@ -165,12 +152,13 @@ __host__ __device__ inline int plane_size(const Geom& g, int A) {
 // WALLZ = true: specialisation for the configuration of every shipped input and benchmark
 // (periodic x and y, moving walls normal to z, no tangential passes), which keeps f[] in
 // registers; WALLZ = false handles any wall axis, none, and the tangential passes.
-template <int ST, bool WALLZ>
+// RO = double decodes the stored populations in double precision (observables).
+template <int ST, bool WALLZ, class RO = typename Pop<ST>::R>
 __device__ __forceinline__ void gather_pre(const typename Pop<ST>::T* __restrict__ fin, const Geom& g, const FluidParams& fl,
-                                           const void* bcraw, int use_raw, int x, int y, int z,
-                                           typename Pop<ST>::R* f) {
+                                           const void* bcraw, int use_raw, int x, int y, int z, RO* f) {
     using P = Pop<ST>;
-    using R = typename P::R;
+    using R = RO;
+    using RW = typename P::R;  // type of the raw wall-unknown buffer
     // Per-axis neighbour offsets with periodic wrap.  A source beyond a wall is replaced by the
     // node itself: those populations are wall unknowns, rebuilt below, so every load is in bounds
     // and no load is predicated.
@@ -190,31 +178,32 @@ __device__ __forceinline__ void gather_pre(const typename Pop<ST>::T* __restrict
         const int oy = cy(q) == 0 ? 0 : (cy(q) > 0 ? oym : oyp);
         const int oz = cz(q) == 0 ? 0 : (cz(q) > 0 ? ozm : ozp);
         const typename P::T* base = fin + (long long)q * qs;
-        f[q] = P::template ld<q>(base, (unsigned)((int)n + ox + oy + oz));
+        if constexpr (std::is_same<RO, typename P::R>::value) f[q] = P::template ld<q>(base, (unsigned)((int)n + ox + oy + oz));
+        else f[q] = P::template ldd<q>(base, (unsigned)((int)n + ox + oy + oz));
     });
-    const R* rw = (const R*)bcraw;
+    const RW* rw = (const RW*)bcraw;
     const int cr = fl.calc_rho;
     if (g.wall == 0) {
-        if (x == 0) wall_layer<0, 0, R>(f, fl.uw[0], cr, use_raw, rw, y + g.ny * z, g.ny * g.nz);
-        if (x == g.nx - 1) wall_layer<0, 1, R>(f, fl.uw[1], cr, use_raw, rw, y + g.ny * z, g.ny * g.nz);
+        if (x == 0) wall_layer<0, 0, R, RW>(f, fl.uw[0], cr, use_raw, rw, y + g.ny * z, g.ny * g.nz);
+        if (x == g.nx - 1) wall_layer<0, 1, R, RW>(f, fl.uw[1], cr, use_raw, rw, y + g.ny * z, g.ny * g.nz);
     }
     if (g.wall == 1) {
-        if (y == 0) wall_layer<1, 0, R>(f, fl.uw[0], cr, use_raw, rw, x + g.nx * z, g.nx * g.nz);
-        if (y == g.ny - 1) wall_layer<1, 1, R>(f, fl.uw[1], cr, use_raw, rw, x + g.nx * z, g.nx * g.nz);
+        if (y == 0) wall_layer<1, 0, R, RW>(f, fl.uw[0], cr, use_raw, rw, x + g.nx * z, g.nx * g.nz);
+        if (y == g.ny - 1) wall_layer<1, 1, R, RW>(f, fl.uw[1], cr, use_raw, rw, x + g.nx * z, g.nx * g.nz);
     }
     if (g.wall == 2) {
-        if (z == 0) wall_layer<2, 0, R>(f, fl.uw[0], cr, use_raw, rw, x + g.nx * y, g.nx * g.ny);
-        if (z == g.nz - 1) wall_layer<2, 1, R>(f, fl.uw[1], cr, use_raw, rw, x + g.nx * y, g.nx * g.ny);
+        if (z == 0) wall_layer<2, 0, R, RW>(f, fl.uw[0], cr, use_raw, rw, x + g.nx * y, g.nx * g.ny);
+        if (z == g.nz - 1) wall_layer<2, 1, R, RW>(f, fl.uw[1], cr, use_raw, rw, x + g.nx * y, g.nx * g.ny);
     }
     // tangential passes in axis order after the wall pass (sim_main.c:388-404); skipped while the
     // raw uploaded unknowns are in use, since the periodic gather then returns the uploaded values
     if (fl.tan_mask && !use_raw) {
-        if ((fl.tan_mask & 1) && x == 0) wall_layer<0, 0, R>(f, fl.uw[0], 0, 0, rw, 0, 1);
-        if ((fl.tan_mask & 1) && x == g.nx - 1) wall_layer<0, 1, R>(f, fl.uw[1], 0, 0, rw, 0, 1);
-        if ((fl.tan_mask & 2) && y == 0) wall_layer<1, 0, R>(f, fl.uw[0], 0, 0, rw, 0, 1);
-        if ((fl.tan_mask & 2) && y == g.ny - 1) wall_layer<1, 1, R>(f, fl.uw[1], 0, 0, rw, 0, 1);
-        if ((fl.tan_mask & 4) && z == 0) wall_layer<2, 0, R>(f, fl.uw[0], 0, 0, rw, 0, 1);
-        if ((fl.tan_mask & 4) && z == g.nz - 1) wall_layer<2, 1, R>(f, fl.uw[1], 0, 0, rw, 0, 1);
+        if ((fl.tan_mask & 1) && x == 0) wall_layer<0, 0, R, RW>(f, fl.uw[0], 0, 0, rw, 0, 1);
+        if ((fl.tan_mask & 1) && x == g.nx - 1) wall_layer<0, 1, R, RW>(f, fl.uw[1], 0, 0, rw, 0, 1);
+        if ((fl.tan_mask & 2) && y == 0) wall_layer<1, 0, R, RW>(f, fl.uw[0], 0, 0, rw, 0, 1);
+        if ((fl.tan_mask & 2) && y == g.ny - 1) wall_layer<1, 1, R, RW>(f, fl.uw[1], 0, 0, rw, 0, 1);
+        if ((fl.tan_mask & 4) && z == 0) wall_layer<2, 0, R, RW>(f, fl.uw[0], 0, 0, rw, 0, 1);
+        if ((fl.tan_mask & 4) && z == g.nz - 1) wall_layer<2, 1, R, RW>(f, fl.uw[1], 0, 0, rw, 0, 1);
     }
 }
 
@@ -317,25 +306,16 @@ __device__ __forceinline__ R strain_rate(const R* ss, R ux, R uy, R uz, R gx, R 
     return sqrt_(sum) * (R)2.1213203435596424 * inv_rho;
 }
 
+// One lattice update at interior node (x, y, z).  VT: 0 constant tau, 1 per-node tau, 2 per-node tau
+// when a.vt is set (batched cells of mixed rheology).  probe = 1 (autotuning trials) leaves the
+// pending spread force and the tau field untouched so a trial launch does not change the state
+// a later step reads.
 template <int ST, int COLL, int VT, bool WZ>
-__global__ void __launch_bounds__(256) collide_kernel(CollideArgs a) {
+__device__ __forceinline__ void collide_node(const CollideArgs& a, int x, int y, int z, int probe) {
     using P = Pop<ST>;
     using R = typename P::R;
     using T = typename P::T;
     const Geom& g = a.g;
-    // Tiles are taken in dispatch order (x fastest, then y, then z).  An XCD-contiguous remap of
-    // the tiles (each XCD owning a z-slab) measured 2-7% slower on MI355X for 8-34 M node lattices:
-    // with plain order all XCDs stream neighbouring addresses and the shared Infinity Cache serves
-    // the y/z halo rows.
-    const int L = blockIdx.x;
-    const int per_plane = a.nbx * a.nby;
-    const int z = L / per_plane;
-    const int rem = L - z * per_plane;
-    const int tyb = rem / a.nbx;
-    const int txb = rem - tyb * a.nbx;
-    const int x = txb * a.bx + (int)(threadIdx.x % a.bx);
-    const int y = tyb * a.by + (int)(threadIdx.x / a.bx);
-    if (x >= g.nx || y >= g.ny || z >= g.nz) return;
     const int n = x + g.nx * (y + g.ny * z);
 
     R f[Q];
@@ -345,7 +325,7 @@ __global__ void __launch_bounds__(256) collide_kernel(CollideArgs a) {
     const int brick = (x >> 2) + g.bnx * ((y >> 2) + g.bny * (z >> 2));
     uint32_t ep = 0;
     if (a.has_force || a.use_umask) ep = a.epochs[0] + 1u;  // this step's epoch
-    if (a.has_force) {
+    if (a.has_force && !probe) {
         const uint32_t fe = a.epochs[1];
         const uint32_t* fm = (fe & 1u) ? a.mask1 : a.mask0;
         if (fe != 0u && fm[brick] == fe) {
@@ -396,7 +376,7 @@ __global__ void __launch_bounds__(256) collide_kernel(CollideArgs a) {
 
     R tau = (R)a.fl.tau0;
     R* taup = (R*)a.tau;
-    if constexpr (VT) {
+    if (VT == 1 || (VT == 2 && a.vt)) {
         const R told = taup[n];
         const R itold = rcp_(told);
         R srt;
@@ -434,7 +414,7 @@ __global__ void __launch_bounds__(256) collide_kernel(CollideArgs a) {
             srt = strain_rate<R>(ss, ux, uy, uz, gx, gy, gz, inv_rho);
         }
         tau = tau_law<R>(a.fl, srt);
-        taup[n] = tau;
+        if (!probe) taup[n] = tau;
     }
     const R itau = rcp_(tau);
     R fo[Q];
@@ -463,6 +443,57 @@ __global__ void __launch_bounds__(256) collide_kernel(CollideArgs a) {
     });
 }
 
+template <int ST, int COLL, int VT, bool WZ>
+__global__ void __launch_bounds__(256) collide_kernel(CollideArgs a) {
+    // Tiles are taken in dispatch order (x fastest, then y, then z).  An XCD-contiguous remap of
+    // the tiles (each XCD owning a z-slab) measured 2-7% slower on MI355X for 8-34 M node lattices:
+    // with plain order all XCDs stream neighbouring addresses and the shared Infinity Cache serves
+    // the y/z halo rows.
+    const int L = blockIdx.x;
+    const int per_plane = a.nbx * a.nby;
+    const int z = L / per_plane;
+    const int rem = L - z * per_plane;
+    const int tyb = rem / a.nbx;
+    const int txb = rem - tyb * a.nbx;
+    const int x = txb * a.bx + (int)(threadIdx.x % a.bx);
+    const int y = tyb * a.by + (int)(threadIdx.x / a.bx);
+    if (x >= a.g.nx || y >= a.g.ny || z >= a.g.nz) return;
+    collide_node<ST, COLL, VT, WZ>(a, x, y, z, 0);
+}
+
+// flat node index of a cell -> (x, y, z); exact for n < 2^24 (float estimate within +-1, corrected)
+__device__ __forceinline__ void split_node(const CellDesc& cd, int n, int& x, int& y, int& z) {
+    int zz = (int)((float)n * cd.inv_nxy);
+    int r = n - zz * cd.nxy;
+    if (r < 0) { zz--; r += cd.nxy; } else if (r >= cd.nxy) { zz++; r -= cd.nxy; }
+    int yy = (int)((float)r * cd.inv_nx);
+    int xx = r - yy * cd.nx;
+    if (xx < 0) { yy--; xx += cd.nx; } else if (xx >= cd.nx) { yy++; xx -= cd.nx; }
+    x = xx; y = yy; z = zz;
+}
+
+// Batched collide+stream: a persistent grid walks the tile list of all cells (a tile is blockDim
+// consecutive nodes of one cell).  Cells are shear cells (periodic x and y, walls normal to z);
+// each cell's tau law, wall speeds, spheres and lattice pointers come from its descriptor.  sp is
+// the step parity: the cell's state is read from f[par ^ sp] and written to the other lattice.
+template <int ST, int COLL>
+__global__ void __launch_bounds__(256) collide_batch_kernel(BatchTables t, int sp, int write_u_all, int probe) {
+    const int ntiles = t.cnt->ntiles;
+    for (int k = blockIdx.x; k < ntiles; k += gridDim.x) {
+        const uint2 tl = t.tiles[k];
+        const CellDesc& cd = t.cells[tl.x];
+        const int n = (int)tl.y + (int)threadIdx.x;
+        if (n >= cd.nint) continue;
+        int x, y, z;
+        split_node(cd, n, x, y, z);
+        CollideArgs a = cd.c;
+        a.fin = cd.f[cd.par ^ sp];
+        a.fout = (void*)cd.f[cd.par ^ sp ^ 1];
+        a.write_u_all = write_u_all;
+        collide_node<ST, COLL, 2, true>(a, x, y, z, probe);
+    }
+}
+
 template <int ST, int COLL, int VT>
 static void launch_one(const CollideArgs& a, int threads, hipStream_t st) {
     const bool wz = a.g.wall == 2 && a.g.per[0] && a.g.per[1] && a.fl.tan_mask == 0;
@@ -481,7 +512,7 @@ static void dispatch_coll(const CollideArgs& a, int coll, int vt, int threads, h
     }
 }
 
-int launch_collide(Sim* s, const void* fin, void* fout, int write_u_all, int use_maskU, hipStream_t st) {
+CollideArgs make_collide_args(Sim* s, const void* fin, void* fout, int write_u_all, int use_maskU) {
     CollideArgs a;
     a.fin = fin;
     a.fout = fout;
@@ -499,11 +530,17 @@ int launch_collide(Sim* s, const void* fin, void* fout, int write_u_all, int use
     a.use_umask = (use_maskU && s->mask[0] != nullptr) ? 1 : 0;
     a.use_bcraw = s->bcraw_valid;
     a.has_force = (s->np > 0 && s->mask[0] != nullptr && s->facc != nullptr) ? 1 : 0;
+    a.vt = s->opt.variable_viscosity ? 1 : 0;
     a.bx = s->block[0];
     a.by = s->block[1];
     a.nbx = (s->g.nx + a.bx - 1) / a.bx;
     a.nby = (s->g.ny + a.by - 1) / a.by;
     a.nblk = a.nbx * a.nby * s->g.nz;
+    return a;
+}
+
+int launch_collide(Sim* s, const void* fin, void* fout, int write_u_all, int use_maskU, hipStream_t st) {
+    const CollideArgs a = make_collide_args(s, fin, fout, write_u_all, use_maskU);
     const int threads = a.bx * a.by;
     switch (s->storage) {
     case D3Q19_STORE_FP16_SHIFT: dispatch_coll<D3Q19_STORE_FP16_SHIFT>(a, s->opt.collision, s->opt.variable_viscosity, threads, st); break;
@@ -513,6 +550,25 @@ int launch_collide(Sim* s, const void* fin, void* fout, int write_u_all, int use
     }
     HIP_OK(hipGetLastError());
     s->bcraw_valid = 0;
+    return 0;
+}
+
+template <int ST>
+static void collide_batch_st(const BatchLaunch& L, int sp, int write_u_all, int probe, hipStream_t st) {
+    if (L.collision == D3Q19_COLL_BGK)
+        hipLaunchKernelGGL((collide_batch_kernel<ST, D3Q19_COLL_BGK>), dim3(L.grid_collide), dim3(L.tile), 0, st, L.t, sp, write_u_all, probe);
+    else
+        hipLaunchKernelGGL((collide_batch_kernel<ST, D3Q19_COLL_MRT>), dim3(L.grid_collide), dim3(L.tile), 0, st, L.t, sp, write_u_all, probe);
+}
+
+int launch_collide_batch(const BatchLaunch& L, int sp, int write_u_all, int probe, hipStream_t st) {
+    switch (L.storage) {
+    case D3Q19_STORE_FP16_SHIFT: collide_batch_st<D3Q19_STORE_FP16_SHIFT>(L, sp, write_u_all, probe, st); break;
+    case D3Q19_STORE_BF16_SHIFT: collide_batch_st<D3Q19_STORE_BF16_SHIFT>(L, sp, write_u_all, probe, st); break;
+    case D3Q19_STORE_FP64: collide_batch_st<D3Q19_STORE_FP64>(L, sp, write_u_all, probe, st); break;
+    default: collide_batch_st<D3Q19_STORE_FP32>(L, sp, write_u_all, probe, st); break;
+    }
+    HIP_OK(hipGetLastError());
     return 0;
 }
 
@@ -704,6 +760,23 @@ int launch_fill_equilibrium(Sim* s, void* f) {
     return 0;
 }
 
+template <class R>
+__global__ void fill_value_kernel(R* p, long long n, R v) {
+    const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+int launch_fill_tau(Sim* s) {
+    const int th = 256, blocks = (int)((s->g.nint + th - 1) / th);
+    if (s->storage == D3Q19_STORE_FP64)
+        hipLaunchKernelGGL(fill_value_kernel<double>, dim3(blocks), dim3(th), 0, s->stream, (double*)s->tau, (long long)s->g.nint,
+                           (double)s->fp.NewtonianTau);
+    else
+        hipLaunchKernelGGL(fill_value_kernel<float>, dim3(blocks), dim3(th), 0, s->stream, (float*)s->tau, (long long)s->g.nint,
+                           s->fp.NewtonianTau);
+    HIP_OK(hipGetLastError());
+    return 0;
+}
+
 // ---- observables ------------------------------------------------------------------------------------
 // plane means of u over each interior z plane, FP64, one block per plane, fixed reduction order.
 template <class R>
@@ -798,5 +871,109 @@ static int moments_any(Sim* s, const void* fsrc, double* dout, int* bad) {
 }
 int launch_moments(Sim* s, const void* fsrc, double* dout) { return moments_any(s, fsrc, dout, nullptr); }
 int launch_check_finite(Sim* s, const void* fsrc, int* flag) { return moments_any(s, fsrc, nullptr, flag); }
+
+// ---- shear-cell observables (FP64) ---------------------------------------------------------------
+// Viscous shear stress sigma_FA at node (x, y, z) of the pre-collision field, evaluated in double
+// precision from the stored populations:
+//   sigma_FA = -(1 - 1/(2 tau)) (Pi_FA - rho u_F u_A + (u_F g_A + u_A g_F)/2),
+//   Pi_FA = sum_q c_qF c_qA f_q,  u = (sum_q c_q f_q + g/2)/rho,  g = the constant body force,
+// with tau the relaxation time of the F-A stress moment at the node (the per-node tau, NewtonianTau,
+// or 1/1.4 for the MRT x-y moment, whose rate the reference fixes, GPU_program.cl:453,567-568).
+template <int ST>
+__device__ double node_stress(const CellDesc& cd, const typename Pop<ST>::T* fin, int x, int y, int z) {
+    using R = typename Pop<ST>::R;
+    const CollideArgs& c = cd.c;
+    double f[Q];
+    gather_pre<ST, false, double>(fin, c.g, c.fl, c.bcraw, c.use_bcraw, x, y, z, f);
+    const int A = cd.o.axisA, F = cd.o.axisF;
+    double rho = 0, jx = 0, jy = 0, jz = 0, pi = 0;
+    Unroll<0, Q>::run([&](auto qc) __attribute__((always_inline)) {
+        constexpr int q = decltype(qc)::value;
+        rho += f[q];
+        if constexpr (cx(q) != 0) jx += (double)cx(q) * f[q];
+        if constexpr (cy(q) != 0) jy += (double)cy(q) * f[q];
+        if constexpr (cz(q) != 0) jz += (double)cz(q) * f[q];
+        const int w = cc(q, F) * cc(q, A);
+        if (w != 0) pi += (double)w * f[q];
+    });
+    const double g3[3] = {(double)c.fl.gconst[0], (double)c.fl.gconst[1], (double)c.fl.gconst[2]};
+    const double u3[3] = {(jx + 0.5 * g3[0]) / rho, (jy + 0.5 * g3[1]) / rho, (jz + 0.5 * g3[2]) / rho};
+    const double uF = F == 0 ? u3[0] : (F == 1 ? u3[1] : u3[2]);
+    const double uA = A == 0 ? u3[0] : (A == 1 ? u3[1] : u3[2]);
+    const double gF = F == 0 ? g3[0] : (F == 1 ? g3[1] : g3[2]);
+    const double gA = A == 0 ? g3[0] : (A == 1 ? g3[1] : g3[2]);
+    double tau = (double)c.fl.tau0;
+    if (cd.o.rate_fixed > 0.f) tau = 1.0 / (double)cd.o.rate_fixed;
+    else if (c.vt) tau = (double)((const R*)c.tau)[x + c.g.nx * (y + c.g.ny * z)];
+    return -(1.0 - 0.5 / tau) * (pi - rho * uF * uA + 0.5 * (uF * gA + uA * gF));
+}
+
+// Block b: cell b/3; role 0/1 sums sigma_FA over the lower/upper plane next to the walls (plane
+// index 1 and n_A - 2 along the wall axis), role 2 sums |fluid force| over the cell's spheres.
+// Fixed-order block reductions: deterministic.
+template <int ST>
+__global__ void __launch_bounds__(256) observables_kernel(const CellDesc* __restrict__ cells, int sp, double* __restrict__ out) {
+    const int cell = blockIdx.x / 3, role = blockIdx.x % 3;
+    const CellDesc& cd = cells[cell];
+    double v0 = 0.0, v1 = 0.0;
+    if (role < 2) {
+        const int A = cd.o.axisA;
+        if (A >= 0) {
+            const Geom& g = cd.c.g;
+            const int nA = A == 0 ? g.nx : (A == 1 ? g.ny : g.nz);
+            const int n1 = A == 0 ? g.ny : g.nx;
+            const int n2 = A == 2 ? g.ny : g.nz;
+            const int pidx = role == 0 ? 1 : nA - 2;
+            const typename Pop<ST>::T* fin = (const typename Pop<ST>::T*)cd.f[cd.par ^ sp];
+            for (int i = threadIdx.x; i < n1 * n2; i += blockDim.x) {
+                const int u1 = i % n1, u2 = i / n1;
+                const int x = A == 0 ? pidx : u1;
+                const int y = A == 1 ? pidx : (A == 0 ? u1 : u2);
+                const int z = A == 2 ? pidx : u2;
+                v0 += node_stress<ST>(cd, fin, x, y, z);
+                v1 += 1.0;
+            }
+        }
+    } else {
+        const PartArgs& p = cd.p;
+        for (int k = threadIdx.x; k < p.np; k += blockDim.x) {
+            const double* F = p.ffluid + 3 * k;
+            v0 += sqrt(F[0] * F[0] + F[1] * F[1] + F[2] * F[2]);
+            v1 += 1.0;
+        }
+    }
+    __shared__ double sh[2][256];
+    sh[0][threadIdx.x] = v0;
+    sh[1][threadIdx.x] = v1;
+    __syncthreads();
+    for (int st = blockDim.x / 2; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) {
+            sh[0][threadIdx.x] += sh[0][threadIdx.x + st];
+            sh[1][threadIdx.x] += sh[1][threadIdx.x + st];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[OBS_STRIDE * cell + 2 * role] = sh[0][0];
+        out[OBS_STRIDE * cell + 2 * role + 1] = sh[1][0];
+        if (role == 2) {
+            out[OBS_STRIDE * cell + 6] = 0.0;
+            out[OBS_STRIDE * cell + 7] = 0.0;
+        }
+    }
+}
+
+int launch_observables(const CellDesc* cells, int ncells, int storage, int sp, double* out, hipStream_t st) {
+    if (ncells <= 0) return 0;
+    const dim3 grid(3 * ncells), block(256);
+    switch (storage) {
+    case D3Q19_STORE_FP16_SHIFT: hipLaunchKernelGGL(observables_kernel<D3Q19_STORE_FP16_SHIFT>, grid, block, 0, st, cells, sp, out); break;
+    case D3Q19_STORE_BF16_SHIFT: hipLaunchKernelGGL(observables_kernel<D3Q19_STORE_BF16_SHIFT>, grid, block, 0, st, cells, sp, out); break;
+    case D3Q19_STORE_FP64: hipLaunchKernelGGL(observables_kernel<D3Q19_STORE_FP64>, grid, block, 0, st, cells, sp, out); break;
+    default: hipLaunchKernelGGL(observables_kernel<D3Q19_STORE_FP32>, grid, block, 0, st, cells, sp, out); break;
+    }
+    HIP_OK(hipGetLastError());
+    return 0;
+}
 
 }  // namespace d3q19

@@ -20,33 +20,6 @@
 
 namespace d3q19 {
 
-struct PartArgs {
-    int np, ppp;
-    Geom g;
-    double sys[3];
-    double gconst[3];
-    double mass, inertia;
-    double diam;
-    double kcol;
-    double radius;  // brick-marking radius
-    int force_model;
-    int visc_model;
-    double tau0;
-    double zbuf;
-    double* pos;
-    double* vel;
-    double* quat;
-    double* omega;
-    double* ffluid;
-    double* tfluid;
-    double* fcont;
-    double* tcont;
-    const double* mforce;
-    uint32_t* mask0;
-    uint32_t* mask1;
-    const uint32_t* epochs;
-};
-
 // ---- brick marking ------------------------------------------------------------------------------------
 // Stamps epoch e on every brick that a marker stencil of a sphere centred at c3 can touch.  Marker
 // floor coordinates along an axis span [floor(c-R), floor(c+R)]; each is wrapped like the markers
@@ -121,9 +94,8 @@ __device__ __forceinline__ void dynamics_one(const PartArgs& a, int p) {
     }
 }
 
-// block p: dynamics of sphere p (thread 0), then marking of its bricks for this step's epoch
-__global__ void __launch_bounds__(256) sphere_pre_kernel(PartArgs a, int do_dyn, int do_mark) {
-    const int p = blockIdx.x;
+// dynamics of sphere p (thread 0), then marking of its bricks for this step's epoch; whole block
+__device__ __forceinline__ void sphere_pre(const PartArgs& a, int p, int do_dyn, int do_mark) {
     __shared__ double c3[3];
     if (threadIdx.x == 0) {
         if (do_dyn) dynamics_one(a, p);
@@ -134,6 +106,10 @@ __global__ void __launch_bounds__(256) sphere_pre_kernel(PartArgs a, int do_dyn,
         const uint32_t e = a.epochs[0] + 1u;
         mark_sphere(c3, a.g, a.radius, (e & 1u) ? a.mask1 : a.mask0, e);
     }
+}
+
+__global__ void __launch_bounds__(256) sphere_pre_kernel(PartArgs a, int do_dyn, int do_mark) {
+    sphere_pre(a, blockIdx.x, do_dyn, do_mark);
 }
 
 // fixed-order sums over the block of K values per thread (wave butterfly, then the waves in
@@ -154,9 +130,8 @@ __device__ __forceinline__ void block_sums(double* v, double (*sh)[K]) {
     __syncthreads();
 }
 
-// block p: FP64 sums of sphere p's marker forces/torques, then its contact forces
-__global__ void __launch_bounds__(256) sphere_post_kernel(PartArgs a, int do_reduce, int do_contacts) {
-    const int p = blockIdx.x;
+// FP64 sums of sphere p's marker forces/torques, then its contact forces; whole block
+__device__ __forceinline__ void sphere_post(const PartArgs& a, int p, int do_reduce, int do_contacts) {
     __shared__ double sh6[4][6];
     __shared__ double sh3[4][3];
     if (do_reduce) {
@@ -218,7 +193,32 @@ __global__ void __launch_bounds__(256) sphere_post_kernel(PartArgs a, int do_red
     }
 }
 
-static PartArgs part_args(Sim* s) {
+__global__ void __launch_bounds__(256) sphere_post_kernel(PartArgs a, int do_reduce, int do_contacts) {
+    sphere_post(a, blockIdx.x, do_reduce, do_contacts);
+}
+
+// Batched: a persistent grid of blocks walks the sphere list of all cells.
+__global__ void __launch_bounds__(256) sphere_pre_batch_kernel(BatchTables t) {
+    const int nsph = t.cnt->nsph;
+    for (int k = blockIdx.x; k < nsph; k += gridDim.x) {
+        const int4 d = t.sph[k];
+        const CellDesc& cd = t.cells[d.x];
+        sphere_pre(cd.p, d.y, 1, 1);
+        __syncthreads();
+    }
+}
+
+__global__ void __launch_bounds__(256) sphere_post_batch_kernel(BatchTables t) {
+    const int nsph = t.cnt->nsph;
+    for (int k = blockIdx.x; k < nsph; k += gridDim.x) {
+        const int4 d = t.sph[k];
+        const CellDesc& cd = t.cells[d.x];
+        sphere_post(cd.p, d.y, 1, 1);
+        __syncthreads();
+    }
+}
+
+PartArgs make_part_args(Sim* s) {
     PartArgs a;
     a.np = s->np;
     a.ppp = s->ppp;
@@ -254,14 +254,26 @@ static PartArgs part_args(Sim* s) {
 
 int launch_sphere_pre(Sim* s, int do_dyn, int do_mark, hipStream_t st) {
     if (s->np == 0) return 0;
-    hipLaunchKernelGGL(sphere_pre_kernel, dim3(s->np), dim3(256), 0, st, part_args(s), do_dyn, do_mark);
+    hipLaunchKernelGGL(sphere_pre_kernel, dim3(s->np), dim3(256), 0, st, make_part_args(s), do_dyn, do_mark);
     HIP_OK(hipGetLastError());
     return 0;
 }
 
 int launch_sphere_post(Sim* s, int do_reduce, int do_contacts, hipStream_t st) {
     if (s->np == 0) return 0;
-    hipLaunchKernelGGL(sphere_post_kernel, dim3(s->np), dim3(256), 0, st, part_args(s), do_reduce, do_contacts);
+    hipLaunchKernelGGL(sphere_post_kernel, dim3(s->np), dim3(256), 0, st, make_part_args(s), do_reduce, do_contacts);
+    HIP_OK(hipGetLastError());
+    return 0;
+}
+
+int launch_sphere_pre_batch(const BatchLaunch& L, hipStream_t st) {
+    hipLaunchKernelGGL(sphere_pre_batch_kernel, dim3(L.grid_sphere), dim3(256), 0, st, L.t);
+    HIP_OK(hipGetLastError());
+    return 0;
+}
+
+int launch_sphere_post_batch(const BatchLaunch& L, hipStream_t st) {
+    hipLaunchKernelGGL(sphere_post_batch_kernel, dim3(L.grid_sphere), dim3(256), 0, st, L.t);
     HIP_OK(hipGetLastError());
     return 0;
 }

@@ -106,6 +106,39 @@ class OutputData(C.Structure):
     _fields_ = [("ShearStressCount", C.c_int), ("ShearStressAvg", C.c_float), ("ActualShearRate", C.c_float)]
 
 
+class Observables(C.Structure):
+    _fields_ = [("wall_stress", C.c_double), ("wall_stress_lower", C.c_double), ("wall_stress_upper", C.c_double),
+                ("shear_rate", C.c_double), ("apparent_viscosity", C.c_double), ("fluid_viscosity", C.c_double),
+                ("relative_viscosity", C.c_double), ("drag_coefficient", C.c_double), ("mean_fluid_force", C.c_double),
+                ("wall_axis", C.c_int32), ("flow_axis", C.c_int32), ("reserved", C.c_double * 4)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+
+
+class BatchOptions(C.Structure):
+    _fields_ = [("storage", C.c_int32), ("collision", C.c_int32), ("ibm_kernel", C.c_int32), ("max_cells", C.c_int32),
+                ("arena_bytes", C.c_uint64), ("reserved", C.c_int32 * 8)]
+
+
+class Tuning(C.Structure):
+    _fields_ = [("tile", C.c_int32), ("grid_collide", C.c_int32), ("grid_sphere", C.c_int32), ("grid_ibm", C.c_int32),
+                ("collide_ms", C.c_double), ("reserved", C.c_int32 * 4)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+
+
+class SolveParams(C.Structure):
+    _fields_ = [("tolerance", C.c_double), ("check_interval", C.c_int32), ("min_steps", C.c_int32),
+                ("max_steps", C.c_int32), ("reserved", C.c_int32 * 5)]
+
+
+class Result(C.Structure):
+    _fields_ = [("id", C.c_int64), ("status", C.c_int32), ("steps", C.c_int32), ("obs", Observables),
+                ("t_submit", C.c_double), ("t_admit", C.c_double), ("t_finish", C.c_double)]
+
+
 COLL_MRT, COLL_BGK = 0, 1
 IBM_LINEAR, IBM_PESKIN4 = 0, 1
 STORE_FP32, STORE_FP16, STORE_BF16, STORE_FP64 = 0, 1, 2, 3
@@ -173,6 +206,31 @@ def lib():
         L.d3q19_time_collide.argtypes = [V, C.c_int, C.c_int, P(C.c_double)]
         L.d3q19_time_steps.argtypes = [V, C.c_int, C.c_int, P(C.c_double), P(C.c_double)]
         L.d3q19_footprint.argtypes = [P(IntParams), P(Options), P(C.c_size_t)]
+        L.d3q19_observables_get.argtypes = [V, P(Observables)]
+        L.d3q19_batch_create.argtypes = [C.c_int, P(BatchOptions), P(V)]
+        L.d3q19_batch_destroy.argtypes = [V]
+        L.d3q19_batch_add.argtypes = [V, P(IntParams), P(FlpParams), P(Options), V, V, P(C.c_int)]
+        L.d3q19_batch_remove.argtypes = [V, C.c_int]
+        L.d3q19_batch_cell_bytes.argtypes = [V, P(IntParams), P(C.c_size_t)]
+        L.d3q19_batch_memory.argtypes = [V, P(C.c_size_t), P(C.c_size_t), P(C.c_int)]
+        L.d3q19_batch_run.argtypes = [V, C.c_int, C.c_int]
+        L.d3q19_batch_sync.argtypes = [V]
+        L.d3q19_batch_observables.argtypes = [V, P(C.c_int), C.c_int, P(Observables)]
+        L.d3q19_batch_read_u.argtypes = [V, C.c_int, V]
+        L.d3q19_batch_read_f.argtypes = [V, C.c_int, V]
+        L.d3q19_batch_read_particles_f64.argtypes = [V, C.c_int, V, V, V, V, V]
+        L.d3q19_batch_autotune.argtypes = [V, P(Tuning)]
+        L.d3q19_batch_get_tuning.argtypes = [V, P(Tuning)]
+        L.d3q19_batch_set_tuning.argtypes = [V, P(Tuning)]
+        L.d3q19_batch_time_steps.argtypes = [V, C.c_int, P(C.c_double), P(C.c_double)]
+        L.d3q19_server_create.argtypes = [C.c_int, P(BatchOptions), P(V)]
+        L.d3q19_server_destroy.argtypes = [V]
+        L.d3q19_server_submit.argtypes = [V, P(IntParams), P(FlpParams), P(Options), V, V, P(SolveParams), P(C.c_int64)]
+        L.d3q19_server_progress.argtypes = [V, P(Result), C.c_int, P(C.c_int)]
+        L.d3q19_server_status.argtypes = [V, P(C.c_int), P(C.c_int), P(C.c_size_t)]
+        L.d3q19_server_get_batch.argtypes = [V, P(V)]
+        L.d3q19_solve.argtypes = [C.c_int, P(IntParams), P(FlpParams), P(Options), V, V, P(SolveParams), P(Result)]
+        L.d3q19_clock.restype = C.c_double
         L.d3q19_strerror.argtypes = [C.c_int]
         L.d3q19_strerror.restype = C.c_char_p
         L.d3q19_last_error.restype = C.c_char_p
@@ -389,6 +447,11 @@ class Sim:
         check(self.L.d3q19_time_steps(self.h, t_first, n, C.byref(tot), C.byref(col)), "time_steps")
         return tot.value, col.value
 
+    def observables(self):
+        o = Observables()
+        check(self.L.d3q19_observables_get(self.h, C.byref(o)), "observables_get")
+        return o
+
     def upload_case(self):
         c = self.case
         self.write_f(c.f)
@@ -396,3 +459,169 @@ class Sim:
         self.write_tau(c.tau)
         if c.np > 0:
             self.write_particles(c.parKin, c.parForce, c.parFluid)
+
+
+def _case_arrays(case):
+    pts = case.points if case.np > 0 else None
+    kin = case.parKin if case.np > 0 else None
+    return pts, kin
+
+
+class Batch:
+    """A batched ensemble of shear cells (one arena, persistent kernels, graph-replayed steps)."""
+
+    def __init__(self, storage=STORE_FP32, collision=COLL_BGK, ibm=IBM_PESKIN4, arena_bytes=0, max_cells=0, device=0):
+        self.L = lib()
+        self.h = C.c_void_p()
+        bo = BatchOptions()
+        bo.storage, bo.collision, bo.ibm_kernel, bo.max_cells, bo.arena_bytes = storage, collision, ibm, max_cells, arena_bytes
+        check(self.L.d3q19_batch_create(device, C.byref(bo), C.byref(self.h)), "batch_create")
+        self.cases = {}
+
+    def close(self):
+        if self.h:
+            self.L.d3q19_batch_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add(self, case, opt=None):
+        cid = C.c_int(-1)
+        pts, kin = _case_arrays(case)
+        o = opt if opt is not None else case.opt
+        check(self.L.d3q19_batch_add(self.h, C.byref(case.ip), C.byref(case.fp), C.byref(o), _ptr(pts), _ptr(kin), C.byref(cid)),
+              "batch_add")
+        self.cases[cid.value] = case
+        return cid.value
+
+    def remove(self, cid):
+        check(self.L.d3q19_batch_remove(self.h, cid), "batch_remove")
+        self.cases.pop(cid, None)
+
+    def cell_bytes(self, case):
+        b = C.c_size_t()
+        check(self.L.d3q19_batch_cell_bytes(self.h, C.byref(case.ip), C.byref(b)), "batch_cell_bytes")
+        return b.value
+
+    def memory(self):
+        a, u, n = C.c_size_t(), C.c_size_t(), C.c_int()
+        check(self.L.d3q19_batch_memory(self.h, C.byref(a), C.byref(u), C.byref(n)), "batch_memory")
+        return a.value, u.value, n.value
+
+    def run(self, n, flags=0):
+        check(self.L.d3q19_batch_run(self.h, n, flags), "batch_run")
+
+    def sync(self):
+        check(self.L.d3q19_batch_sync(self.h), "batch_sync")
+
+    def observables(self, ids):
+        ids = list(ids)
+        arr = (C.c_int * len(ids))(*ids)
+        out = (Observables * len(ids))()
+        check(self.L.d3q19_batch_observables(self.h, arr, len(ids), out), "batch_observables")
+        return list(out)
+
+    def read_u(self, cid):
+        u = np.empty(3 * self.cases[cid].nn, dtype=np.float32)
+        check(self.L.d3q19_batch_read_u(self.h, cid, _ptr(u)), "batch_read_u")
+        return u
+
+    def read_f(self, cid):
+        f = np.empty(19 * self.cases[cid].nn, dtype=np.float32)
+        check(self.L.d3q19_batch_read_f(self.h, cid, _ptr(f)), "batch_read_f")
+        return f
+
+    def read_particles_f64(self, cid):
+        n = max(self.cases[cid].np, 1)
+        out = [np.zeros(3 * n) for _ in range(5)]
+        check(self.L.d3q19_batch_read_particles_f64(self.h, cid, *[_ptr(x) for x in out]), "batch_read_particles_f64")
+        return out
+
+    def autotune(self):
+        t = Tuning()
+        check(self.L.d3q19_batch_autotune(self.h, C.byref(t)), "batch_autotune")
+        return t
+
+    def tuning(self):
+        t = Tuning()
+        check(self.L.d3q19_batch_get_tuning(self.h, C.byref(t)), "batch_get_tuning")
+        return t
+
+    def set_tuning(self, t):
+        check(self.L.d3q19_batch_set_tuning(self.h, C.byref(t)), "batch_set_tuning")
+
+    def time_steps(self, n):
+        tot, col = C.c_double(), C.c_double()
+        check(self.L.d3q19_batch_time_steps(self.h, n, C.byref(tot), C.byref(col)), "batch_time_steps")
+        return tot.value, col.value
+
+
+def solve_params(tolerance=1e-4, check_interval=100, min_steps=0, max_steps=100000):
+    sp = SolveParams()
+    sp.tolerance, sp.check_interval, sp.min_steps, sp.max_steps = tolerance, check_interval, min_steps, max_steps
+    return sp
+
+
+class Server:
+    """Continuous-batching solver of shear-cell requests (admission by device-memory footprint)."""
+
+    def __init__(self, storage=STORE_FP32, collision=COLL_BGK, ibm=IBM_PESKIN4, arena_bytes=0, max_cells=0, device=0):
+        self.L = lib()
+        self.h = C.c_void_p()
+        bo = BatchOptions()
+        bo.storage, bo.collision, bo.ibm_kernel, bo.max_cells, bo.arena_bytes = storage, collision, ibm, max_cells, arena_bytes
+        check(self.L.d3q19_server_create(device, C.byref(bo), C.byref(self.h)), "server_create")
+
+    def close(self):
+        if self.h:
+            self.L.d3q19_server_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit(self, case, sp, opt=None):
+        rid = C.c_int64()
+        pts, kin = _case_arrays(case)
+        o = opt if opt is not None else case.opt
+        check(self.L.d3q19_server_submit(self.h, C.byref(case.ip), C.byref(case.fp), C.byref(o), _ptr(pts), _ptr(kin),
+                                         C.byref(sp), C.byref(rid)), "server_submit")
+        return rid.value
+
+    def progress(self, cap=256):
+        out = (Result * cap)()
+        n = C.c_int()
+        check(self.L.d3q19_server_progress(self.h, out, cap, C.byref(n)), "server_progress")
+        return list(out[: n.value])
+
+    def status(self):
+        q, r, u = C.c_int(), C.c_int(), C.c_size_t()
+        check(self.L.d3q19_server_status(self.h, C.byref(q), C.byref(r), C.byref(u)), "server_status")
+        return q.value, r.value, u.value
+
+    def batch_handle(self):
+        b = C.c_void_p()
+        check(self.L.d3q19_server_get_batch(self.h, C.byref(b)), "server_get_batch")
+        return b
+
+
+def solve(case, sp, opt=None, device=0):
+    """Plain sequential solve of one request (single simulation, same convergence rule)."""
+    L = lib()
+    res = Result()
+    pts, kin = _case_arrays(case)
+    o = opt if opt is not None else case.opt
+    check(L.d3q19_solve(device, C.byref(case.ip), C.byref(case.fp), C.byref(o), _ptr(pts), _ptr(kin), C.byref(sp), C.byref(res)),
+          "solve")
+    return res
+
+
+def clock():
+    return lib().d3q19_clock()

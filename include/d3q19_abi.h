@@ -206,6 +206,134 @@ int d3q19_moments(d3q19_sim* sim, double* out4);
  * density. */
 int d3q19_check_finite(d3q19_sim* sim);
 
+/* Shear-cell observables, reduced on the device in FP64 from the stored populations.
+ * Wall axis A (BoundaryConds), flow axis F = the other axis with the largest wall-velocity
+ * difference.  The viscous shear stress of a node is
+ *   sigma_FA = -(1 - 1/(2 tau)) (sum_q c_qF c_qA f_q - rho u_F u_A + (u_F g_A + u_A g_F)/2)
+ * (g = ConstBodyForce, tau = the node's relaxation time of the F-A stress moment).  It is averaged
+ * over the two planes next to the wet wall layers.  The reference's apparent-viscosity observable,
+ * a finite-difference shear rate of the plane-mean velocity (compute_shear_stress,
+ * D3Q19-OpenCL.c:670-750), stays available through d3q19_plane_mean_u.
+ * drag_coefficient = mean over spheres of |F_fluid| / (rho/2 * U^2 * pi a^2), with
+ * U = |VelUpper_F - VelLower_F| / 2, a = ParticleDiam / 2 and rho = 1. */
+typedef struct {
+    double wall_stress;         /* mean of the two wall-adjacent planes */
+    double wall_stress_lower;
+    double wall_stress_upper;
+    double shear_rate;          /* nominal: (VelUpper_F - VelLower_F) / (n_A - 1), n_A interior extent */
+    double apparent_viscosity;  /* wall_stress / shear_rate */
+    double fluid_viscosity;     /* the fluid's viscosity (tau law and clamps) at the nominal shear rate */
+    double relative_viscosity;  /* apparent_viscosity / fluid_viscosity */
+    double drag_coefficient;
+    double mean_fluid_force;    /* mean over spheres of |F_fluid| */
+    int32_t wall_axis, flow_axis;
+    double reserved[4];
+} d3q19_observables;
+int d3q19_observables_get(d3q19_sim* sim, d3q19_observables* out);
+
+/* ------------------------------------------------------------------------------------------
+ * Batched ensembles.  No counterpart in the reference: independent shear cells (periodic x and
+ * y, moving walls normal to z, no tangential passes) of different sizes, rheologies and sphere
+ * sets step together.  Each step is four persistent launches over work lists held in device
+ * memory, and two steps are replayed from one HIP graph.  The lists are rewritten when cells
+ * join or leave, so the graph is never re-captured for that.  All cells share the batch's
+ * population storage, collision operator and IBM kernel; per-cell options (variable_viscosity,
+ * tau clamps, calc_rho) come from the d3q19_options given at d3q19_batch_add.  Device memory
+ * comes from one arena allocated at creation.  A cell's results do not depend on the other cells
+ * of the batch: they are bitwise those of the same cell alone in a batch.
+ * ------------------------------------------------------------------------------------------ */
+typedef struct d3q19_batch d3q19_batch;
+typedef struct {
+    int32_t storage;            /* D3Q19_STORE_* */
+    int32_t collision;          /* D3Q19_COLL_* */
+    int32_t ibm_kernel;         /* D3Q19_IBM_* */
+    int32_t max_cells;          /* 0: 1024 */
+    uint64_t arena_bytes;       /* device memory for the cells' state; 0: half of the free memory */
+    int32_t reserved[8];
+} d3q19_batch_options;
+/* Launch shape of the batched kernels, chosen by d3q19_batch_autotune or set by the caller. */
+typedef struct {
+    int32_t tile;               /* threads = nodes per collide tile: 64, 128 or 256 */
+    int32_t grid_collide;       /* persistent blocks of the collide kernel */
+    int32_t grid_sphere;        /* persistent blocks of the sphere kernels */
+    int32_t grid_ibm;           /* persistent blocks of the IBM kernel (4 markers each) */
+    double collide_ms;          /* measured by the autotuner: mean collide launch */
+    int32_t reserved[4];
+} d3q19_tuning;
+int d3q19_batch_create(int device, const d3q19_batch_options* bo, d3q19_batch** out);
+int d3q19_batch_destroy(d3q19_batch* b);
+/* Adds a cell at rest (f = f_eq(1, 0), u = 0, tau = NewtonianTau) with its spheres at the
+ * positions and velocities of parKin (float4[4*np] as d3q19_write_particles; NULL = all zero).
+ * Returns the cell's id (>= 0) in *cell_id; D3Q19_E_CAPACITY when the arena cannot hold it. */
+int d3q19_batch_add(d3q19_batch* b, const int_param_struct* ip, const flp_param_struct* fp, const d3q19_options* opt,
+                    const d3q19_float4* sphere_points, const d3q19_float4* parKin, int* cell_id);
+int d3q19_batch_remove(d3q19_batch* b, int cell_id);
+/* Bytes one cell of these parameters takes from the arena; and the arena's size and use. */
+int d3q19_batch_cell_bytes(const d3q19_batch* b, const int_param_struct* ip, size_t* bytes);
+int d3q19_batch_memory(const d3q19_batch* b, size_t* arena_bytes, size_t* used_bytes, int* cells);
+/* n_steps steps of every cell (graph replay of step pairs). */
+int d3q19_batch_run(d3q19_batch* b, int n_steps, int flags);
+int d3q19_batch_sync(d3q19_batch* b);
+/* Observables of every cell: out[i] for the i-th id of ids[0..n-1]. */
+int d3q19_batch_observables(d3q19_batch* b, const int* ids, int n, d3q19_observables* out);
+/* Per-cell read-backs in the reference layouts (see the single-simulation calls). */
+int d3q19_batch_read_u(d3q19_batch* b, int cell_id, float* u);
+int d3q19_batch_read_f(d3q19_batch* b, int cell_id, float* f);
+int d3q19_batch_read_particles_f64(d3q19_batch* b, int cell_id, double* pos, double* vel, double* angvel,
+                                   double* fluid_force, double* fluid_torque);
+/* Times candidate launch shapes on the batch's current cells (trial launches leave the state
+ * unchanged), keeps the fastest and returns it. */
+int d3q19_batch_autotune(d3q19_batch* b, d3q19_tuning* out);
+int d3q19_batch_get_tuning(const d3q19_batch* b, d3q19_tuning* out);
+int d3q19_batch_set_tuning(d3q19_batch* b, const d3q19_tuning* t);
+/* n steps timed with HIP events on the batch's stream: total milliseconds, and the mean duration
+ * of the collide launch alone (16 trial launches that leave the state unchanged). */
+int d3q19_batch_time_steps(d3q19_batch* b, int n, double* total_ms, double* collide_ms);
+
+/* ------------------------------------------------------------------------------------------
+ * Serving.  No counterpart in the reference.  Shear-cell solve requests are queued as they
+ * arrive and admitted in arrival order when their footprint fits the batch arena.  Every admitted
+ * cell stays resident and steps in the shared batch (continuous batching): requests that arrive
+ * while others run join at the next check point, and converged cells leave, freeing their
+ * memory.  A request converges when the relative change of its wall stress between two checks
+ * (every check_interval of its own steps) falls below tolerance, after at least min_steps.
+ * Results are independent of the other requests in the batch.
+ * ------------------------------------------------------------------------------------------ */
+typedef struct {
+    double tolerance;           /* relative change of wall_stress between checks */
+    int32_t check_interval;     /* steps between checks (even) */
+    int32_t min_steps;
+    int32_t max_steps;          /* give up (status 1) after this many steps */
+    int32_t reserved[5];
+} d3q19_solve_params;
+typedef struct {
+    int64_t id;
+    int32_t status;             /* 0 converged, 1 max_steps reached, <0 error code */
+    int32_t steps;
+    d3q19_observables obs;
+    double t_submit, t_admit, t_finish;   /* d3q19_clock() seconds */
+} d3q19_result;
+typedef struct d3q19_server d3q19_server;
+int d3q19_server_create(int device, const d3q19_batch_options* bo, d3q19_server** out);
+int d3q19_server_destroy(d3q19_server* srv);
+/* Queues a request (inputs are copied); *id receives its id. */
+int d3q19_server_submit(d3q19_server* srv, const int_param_struct* ip, const flp_param_struct* fp, const d3q19_options* opt,
+                        const d3q19_float4* sphere_points, const d3q19_float4* parKin, const d3q19_solve_params* sp,
+                        int64_t* id);
+/* One scheduling round: admit what fits, run one check interval of steps for all resident cells,
+ * check them, retire the finished ones.  Completed results are written to done[0..*n_done-1]
+ * (at most cap).  Returns with *n_done = 0 and nothing run when no cell is resident. */
+int d3q19_server_progress(d3q19_server* srv, d3q19_result* done, int cap, int* n_done);
+int d3q19_server_status(const d3q19_server* srv, int* queued, int* resident, size_t* arena_used);
+int d3q19_server_get_batch(d3q19_server* srv, d3q19_batch** b);
+/* The plain sequential solve of one request: a single simulation (d3q19_create) stepped with
+ * d3q19_run in check_interval chunks with the same convergence rule. */
+int d3q19_solve(int device, const int_param_struct* ip, const flp_param_struct* fp, const d3q19_options* opt,
+                const d3q19_float4* sphere_points, const d3q19_float4* parKin, const d3q19_solve_params* sp,
+                d3q19_result* out);
+/* Monotonic clock (seconds) used for the result time stamps. */
+double d3q19_clock(void);
+
 /* Timing on the simulation's own stream (HIP events). Runs n collide_stream launches and
  * returns the mean duration per launch in milliseconds. */
 int d3q19_time_collide(d3q19_sim* sim, int n, int flags, double* ms_per_launch);

@@ -1,0 +1,36 @@
+"""GPU: population storage precision against the fp64 build on a Casson suspension.
+
+Scaled-down form of the 256x256x128 precision workload: a 64x64x32 box, 20 spheres, 300 steps.
+fp32, fp16-shifted and bf16-shifted storage (all fp32 arithmetic) are compared with the fp64 build
+of the same kernels on the same seeded input.
+
+Bars:
+- fp32: u within 1e-3 relative L-inf (float32 rounding alone moves u by ~1e-4 relative within a
+  hundred steps); drag coefficient within 1e-3 absolute.
+- fp16-shifted: u within 5e-2 relative L-inf; drag coefficient within 1e-2 absolute.
+- bf16-shifted: a 7-bit mantissa on f - w gives ~4e-3 relative rounding per population and step,
+  so it carries no bar beyond finiteness; its errors are reported.
+"""
+import json
+import math
+
+import pytest
+
+from workloads import c3_deck, precision_study
+
+pytestmark = pytest.mark.gpu
+
+
+def small_c3(storage=0):
+    return c3_deck(storage=storage, size=(64, 64, 32), spheres=20)
+
+
+def test_storage_precision_against_fp64_build():
+    r = precision_study(small_c3, 300)
+    print(json.dumps(r, indent=1), flush=True)
+    assert r["fp32"]["u_linf_rel"] < 1e-3
+    assert r["fp32"]["drag_abs_err"] < 1e-3
+    assert r["fp16_shift"]["u_linf_rel"] < 5e-2
+    assert r["fp16_shift"]["drag_abs_err"] < 1e-2
+    for k in ("u_l2_rel", "u_linf_rel", "drag_abs_err"):
+        assert math.isfinite(r["bf16_shift"][k])
