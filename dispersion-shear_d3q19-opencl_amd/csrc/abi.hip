@@ -8,6 +8,7 @@
 #include <stdio.h>
 #include <string.h>
 #include <cmath>
+#include <utility>
 #include <string>
 #include <vector>
 #include "internal.h"
@@ -118,8 +119,9 @@ static void* cur_f(Sim* s) { return s->parity == 0 ? s->fA : s->fB; }
 static void* nxt_f(Sim* s) { return s->parity == 0 ? s->fB : s->fA; }
 
 void sim_release(Sim* s) {
-    void* ptrs[] = {s->fA, s->fB, s->u, s->tau, s->facc, s->mask[0], s->mask[1], s->bcraw, s->pos, s->vel, s->quat,
-                    s->omega, s->ffluid, s->tfluid, s->fcont, s->tcont, s->pts, s->mforce, s->red, s->d_desc, s->d_obs};
+    void* ptrs[] = {s->fA, s->fB, s->u, s->tau, s->facc, s->mask[0], s->mask[1], s->mask[2], s->bcraw, s->pos, s->vel,
+                    s->pos_alt, s->vel_alt, s->quat, s->omega, s->ffluid, s->tfluid, s->fcont, s->tcont, s->pts, s->mforce,
+                    s->red, s->d_desc, s->d_obs, s->d_flag};
     for (void* p : ptrs)
         if (p) {
             if (s->arena) s->arena->release(p);
@@ -202,6 +204,10 @@ CellDesc make_cell_desc(Sim* s) {
     d.p = make_part_args(s);
     d.f[0] = s->fA;
     d.f[1] = s->fB;
+    d.posb[0] = s->pos;
+    d.posb[1] = s->pos_alt;
+    d.velb[0] = s->vel;
+    d.velb[1] = s->vel_alt;
     d.par = s->parity;
     d.nint = s->g.nint;
     d.nx = s->g.nx;
@@ -238,18 +244,20 @@ static int do_collide_standalone(Sim* s, int flags) {
     return 0;
 }
 
-// One step in sim_main.c:315-471 order.  particle_dynamics (CPU queue in the reference) runs
-// first: it reads only forces of the previous step, so the order relative to collide is free.
-static int enqueue_step(Sim* s, int flags, hipStream_t st) {
+// One step in sim_main.c:315-471 order, fused: the spheres' turn -- the previous step's force sums
+// and contacts (do_post) and this step's particle_dynamics, which the reference runs on the CPU
+// queue beside the collide -- runs in extra blocks of the collide launch, then the IBM pass.  The
+// collide writes u in the bricks marked for this epoch, made by the previous fused step (or by the
+// prime kernel) from the positions before this step's dynamics, dilated by one lattice unit.
+static int enqueue_fused(Sim* s, int flags, int do_post, hipStream_t st) {
     int rc;
-    // dynamics + marks of the bricks this step's IBM pass touches
-    if ((rc = launch_sphere_pre(s, 1, 1, st))) return rc;
-    if ((rc = launch_collide(s, cur_f(s), nxt_f(s), (flags & D3Q19_STEP_WRITE_U) ? 1 : 0, 1, st))) return rc;
+    if ((rc = launch_step_fused(s, cur_f(s), nxt_f(s), (flags & D3Q19_STEP_WRITE_U) ? 1 : 0, do_post, st))) return rc;
     s->parity ^= 1;
-    if ((rc = launch_ibm(s, st))) return rc;
-    // per-sphere force/torque sums + contacts
-    if ((rc = launch_sphere_post(s, 1, 1, st))) return rc;
-    return 0;
+    if (s->np > 0) {
+        std::swap(s->pos, s->pos_alt);
+        std::swap(s->vel, s->vel_alt);
+    }
+    return launch_ibm(s, st);
 }
 
 static int build_graph(Sim* s) {
@@ -259,17 +267,46 @@ static int build_graph(Sim* s) {
     }
     hipGraph_t graph;
     HIP_OK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
-    int rc = enqueue_step(s, 0, s->stream);
-    if (!rc) rc = enqueue_step(s, 0, s->stream);
+    int rc = enqueue_fused(s, 0, 1, s->stream);
+    if (!rc) rc = enqueue_fused(s, 0, 1, s->stream);
     hipError_t e = hipStreamEndCapture(s->stream, &graph);
     if (rc) return rc;
     if (e != hipSuccess) return fail(D3Q19_E_DEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
     e = hipGraphInstantiate(&s->graph2, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
     if (e != hipSuccess) return fail(D3Q19_E_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
-    // capture enqueued nothing; the two swaps of parity and masks cancel
+    // capture enqueued nothing; the two flips of the lattice and of the position buffers cancel
     s->graph_f = cur_f(s);
+    s->graph_pos = s->pos;
     return 0;
+}
+
+// n steps: the prime marks if needed, a first fused step without a pending post, graph-replayed
+// pairs, the remaining steps (the last one carries flags), then the last step's force sums and
+// contacts, so that the state after the call is the reference's state after n steps.
+static int run_steps(Sim* s, int n, int flags) {
+    if (n <= 0) return 0;
+    int rc;
+    if (s->np > 0 && !s->marks_next) {
+        if ((rc = launch_sphere_prime(s, s->stream))) return rc;
+        s->marks_next = true;
+    }
+    if ((rc = enqueue_fused(s, n == 1 ? flags : 0, 0, s->stream))) return rc;
+    int done = 1;
+    const int tail = (flags && n > 1) ? 1 : 0;
+    const int pairs = (n - 1 - tail) / 2;
+    if (pairs > 0) {
+        if (!s->graph2 || s->graph_f != cur_f(s) || s->graph_pos != (const void*)s->pos) {
+            if ((rc = build_graph(s))) return rc;
+        }
+        for (int i = 0; i < pairs; i++) HIP_OK(hipGraphLaunch(s->graph2, s->stream));
+        done += 2 * pairs;
+    }
+    while (done < n) {
+        if ((rc = enqueue_fused(s, done == n - 1 ? flags : 0, 1, s->stream))) return rc;
+        done++;
+    }
+    return launch_sphere_post(s, 1, 1, s->stream);
 }
 
 }  // namespace d3q19
@@ -320,13 +357,15 @@ static void alloc_list(Sim* s, std::vector<AllocItem>& v) {
     v.push_back(item(&s->red, (size_t)(4 * 512 + 8) * sizeof(double)));
     v.push_back(item(&s->d_desc, sizeof(CellDesc)));
     v.push_back(item(&s->d_obs, (size_t)OBS_STRIDE * sizeof(double)));
+    v.push_back(item(&s->d_flag, sizeof(int)));
     if (s->np > 0) {
         const size_t npd = (size_t)s->np;
         v.push_back(item(&s->facc, 3 * nint * sizeof(long long)));
-        v.push_back(item(&s->mask[0], (size_t)s->nbricks * sizeof(uint32_t)));
-        v.push_back(item(&s->mask[1], (size_t)s->nbricks * sizeof(uint32_t)));
+        for (int k = 0; k < 3; k++) v.push_back(item(&s->mask[k], (size_t)s->nbricks * sizeof(uint32_t)));
         v.push_back(item(&s->pos, 3 * npd * 8));
         v.push_back(item(&s->vel, 3 * npd * 8));
+        v.push_back(item(&s->pos_alt, 3 * npd * 8));
+        v.push_back(item(&s->vel_alt, 3 * npd * 8));
         v.push_back(item(&s->quat, 4 * npd * 8));
         v.push_back(item(&s->omega, 3 * npd * 8));
         v.push_back(item(&s->ffluid, 3 * npd * 8));
@@ -402,13 +441,14 @@ int sim_init(Sim* s, int device, const int_param_struct* ip, const flp_param_str
         return rc;
     }
     e = hipMemsetAsync(s->u, 0, 3 * nint * rs, s->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(s->d_flag, 0, sizeof(int), s->stream);
     if (e == hipSuccess && s->np > 0) {
         e = hipMemsetAsync(s->facc, 0, 3 * nint * sizeof(long long), s->stream);
-        if (e == hipSuccess) e = hipMemsetAsync(s->mask[0], 0, s->nbricks * sizeof(uint32_t), s->stream);
-        if (e == hipSuccess) e = hipMemsetAsync(s->mask[1], 0, s->nbricks * sizeof(uint32_t), s->stream);
+        for (int k = 0; k < 3; k++)
+            if (e == hipSuccess) e = hipMemsetAsync(s->mask[k], 0, s->nbricks * sizeof(uint32_t), s->stream);
         static const uint32_t ep0[2] = {0u, 0u};  // next step has epoch 1; no spread force pending
         if (e == hipSuccess && !s->shares_epochs) e = hipMemcpyAsync(s->epochs, ep0, sizeof(ep0), hipMemcpyHostToDevice, s->stream);
-        double* zs[] = {s->pos, s->vel, s->omega, s->ffluid, s->tfluid, s->fcont, s->tcont};
+        double* zs[] = {s->pos, s->vel, s->pos_alt, s->vel_alt, s->omega, s->ffluid, s->tfluid, s->fcont, s->tcont};
         for (double* z : zs)
             if (e == hipSuccess) e = hipMemsetAsync(z, 0, 3 * (size_t)s->np * 8, s->stream);
         if (e == hipSuccess) {
@@ -646,6 +686,7 @@ int d3q19_write_particles(d3q19_sim* s, const d3q19_float4* pk, const d3q19_floa
     CHECK_SIM(s);
     const int np = s->np;
     if (np == 0) return 0;
+    s->marks_next = false;
     HIP_OK(hipStreamSynchronize(s->stream));
     std::vector<double> a(3 * (size_t)np), b(3 * (size_t)np), c(4 * (size_t)np), d(3 * (size_t)np);
     if (pk) {
@@ -746,12 +787,14 @@ int d3q19_collide_stream(d3q19_sim* s, int flags) {
 
 int d3q19_particle_dynamics(d3q19_sim* s) {
     CHECK_SIM(s);
+    s->marks_next = false;  // the positions move
     return launch_sphere_pre(s, 1, 0, s->stream);
 }
 
 int d3q19_particle_fluid_forces(d3q19_sim* s) {
     CHECK_SIM(s);
     if (s->np == 0) return 0;
+    s->marks_next = false;  // this pass commits an epoch
     int rc = launch_sphere_pre(s, 0, 1, s->stream);
     if (!rc) rc = launch_ibm(s, s->stream);
     if (!rc) rc = launch_sphere_post(s, 1, 0, s->stream);
@@ -766,34 +809,13 @@ int d3q19_particle_particle_forces(d3q19_sim* s) {
 int d3q19_step(d3q19_sim* s, int t, int flags) {
     CHECK_SIM(s);
     (void)t;
-    return enqueue_step(s, flags, s->stream);
+    return run_steps(s, 1, flags);
 }
 
 int d3q19_run(d3q19_sim* s, int t_first, int n, int flags) {
     CHECK_SIM(s);
-    if (n <= 0) return 0;
-    int rc;
-    int done = 0;
-    // the first collide after an upload uses the raw wall unknowns: run it directly
-    if (s->bcraw_valid) {
-        if ((rc = enqueue_step(s, n == 1 ? flags : 0, s->stream))) return rc;
-        done = 1;
-    }
-    const int body = n - done - 1;  // the last step carries the flags
-    int pairs = body / 2;
-    if (pairs > 0) {
-        if (!s->graph2 || s->graph_f != cur_f(s)) {
-            if ((rc = build_graph(s))) return rc;
-        }
-        for (int i = 0; i < pairs; i++) HIP_OK(hipGraphLaunch(s->graph2, s->stream));
-        done += 2 * pairs;
-    }
-    while (done < n) {
-        if ((rc = enqueue_step(s, done == n - 1 ? flags : 0, s->stream))) return rc;
-        done++;
-    }
     (void)t_first;
-    return 0;
+    return run_steps(s, n, flags);
 }
 
 int d3q19_sync(d3q19_sim* s) {
@@ -858,6 +880,9 @@ int d3q19_check_finite(d3q19_sim* s) {
     if (rc) return rc;
     if (e != hipSuccess || e2 != hipSuccess) return fail(D3Q19_E_DEVICE, "check_finite failed");
     if (h) return fail(D3Q19_E_NONFINITE, "non-finite or non-positive density in the population field");
+    int moved = 0;
+    HIP_OK(hipMemcpy(&moved, s->d_flag, sizeof(int), hipMemcpyDeviceToHost));
+    if (moved) return fail(D3Q19_E_STATE, "a sphere moved one lattice unit or more in one step");
     return 0;
 }
 

@@ -35,7 +35,22 @@ struct Member {
     Arena sub;            // the member's own block of the batch arena
     void* block = nullptr;
     int par = 0;
+    double* pb[2] = {nullptr, nullptr};  // position / velocity buffers as they were at join
+    double* vb[2] = {nullptr, nullptr};
 };
+
+// Point a member's single-simulation view (current lattice and position buffers) at its state.
+static void sync_view(Member& m, int phase) {
+    Sim* s = m.sim;
+    const int i = m.par ^ phase;
+    s->parity = i;
+    if (s->np > 0) {
+        s->pos = m.pb[i];
+        s->pos_alt = m.pb[i ^ 1];
+        s->vel = m.vb[i];
+        s->vel_alt = m.vb[i ^ 1];
+    }
+}
 
 struct Batch {
     int device = 0;
@@ -65,6 +80,9 @@ struct Batch {
     BatchCounts h_cnt{};
     bool dirty = true;
     d3q19_tuning tuning{};
+    // launch grids in effect: a tuning field of 0 means "one work item per block / wavefront",
+    // rounded up to a power of two so that the captured graph survives most membership changes
+    int eff_collide = 1, eff_sphere = 1, eff_ibm = 1;
     hipGraphExec_t graph2 = nullptr;
     bool graph_valid = false;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -82,9 +100,9 @@ static BatchLaunch launch_of(const Batch* b) {
     L.collision = b->bo.collision;
     L.ibm_mode = b->bo.ibm_kernel;
     L.tile = b->tuning.tile;
-    L.grid_collide = b->tuning.grid_collide;
-    L.grid_sphere = b->tuning.grid_sphere;
-    L.grid_ibm = b->tuning.grid_ibm;
+    L.grid_collide = b->eff_collide;
+    L.grid_sphere = b->eff_sphere;
+    L.grid_ibm = b->eff_ibm;
     return L;
 }
 
@@ -111,6 +129,12 @@ static int grow(T** p, size_t* cap, size_t need, bool* moved) {
     return 0;
 }
 
+static int pow2_at_least(long long n, int floor_) {
+    long long p = floor_;
+    while (p < n && p < (1LL << 24)) p <<= 1;
+    return (int)p;
+}
+
 // Rebuild the work lists from the member set and upload them (stream ordered).
 static int upload_tables(Batch* b) {
     HIP_OK(hipStreamSynchronize(b->stream));  // host mirrors may still be read by earlier copies
@@ -126,9 +150,15 @@ static int upload_tables(Batch* b) {
             continue;
         }
         Sim* s = m.sim;
-        s->parity = m.par ^ b->phase;  // keep the single-simulation view consistent
+        sync_view(m, b->phase);
         CellDesc d = make_cell_desc(s);
         d.par = m.par;
+        d.f[0] = s->fA;
+        d.f[1] = s->fB;
+        for (int k = 0; k < 2; k++) {
+            d.posb[k] = m.pb[k];
+            d.velb[k] = m.vb[k];
+        }
         b->h_cells[c] = d;
         for (int base = 0; base < s->g.nint; base += tile) b->h_tiles.push_back(make_uint2((unsigned)c, (unsigned)base));
         for (int p = 0; p < s->np; p++) {
@@ -142,6 +172,15 @@ static int upload_tables(Batch* b) {
     b->h_cnt.nsph = (int)b->h_sph.size();
     b->h_cnt.nmark = (int)b->h_msph.size();
     b->h_cnt.ncells = active_slots(b);
+    const int ec = b->tuning.grid_collide > 0 ? b->tuning.grid_collide : pow2_at_least(b->h_cnt.ntiles, 256);
+    const int es = b->tuning.grid_sphere > 0 ? b->tuning.grid_sphere : pow2_at_least(b->h_cnt.nsph, 64);
+    const int ei = b->tuning.grid_ibm > 0 ? b->tuning.grid_ibm : pow2_at_least((b->h_cnt.nmark + 3) / 4, 256);
+    if (ec != b->eff_collide || es != b->eff_sphere || ei != b->eff_ibm) {
+        b->eff_collide = ec;
+        b->eff_sphere = es;
+        b->eff_ibm = ei;
+        b->graph_valid = false;
+    }
     bool moved = false;
     int rc;
     if ((rc = grow(&b->d_tiles, &b->cap_tiles, b->h_tiles.size(), &moved))) return rc;
@@ -160,14 +199,13 @@ static int upload_tables(Batch* b) {
     return 0;
 }
 
-// One step of every cell with step parity sp.
-static int enqueue_batch_step(Batch* b, int sp, int write_u_all, hipStream_t st) {
+// One fused step of every cell with step parity sp: the spheres' turn beside the collide tiles,
+// then the IBM pass.
+static int enqueue_batch_step(Batch* b, int sp, int write_u_all, int do_post, hipStream_t st) {
     const BatchLaunch L = launch_of(b);
     int rc;
-    if ((rc = launch_sphere_pre_batch(L, st))) return rc;
-    if ((rc = launch_collide_batch(L, sp, write_u_all, 0, st))) return rc;
-    if ((rc = launch_ibm_batch(L, st))) return rc;
-    return launch_sphere_post_batch(L, st);
+    if ((rc = launch_step_batch(L, sp, write_u_all, do_post, st))) return rc;
+    return launch_ibm_batch(L, sp, st);
 }
 
 static int capture_graph(Batch* b) {
@@ -177,8 +215,8 @@ static int capture_graph(Batch* b) {
     }
     hipGraph_t g;
     HIP_OK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-    int rc = enqueue_batch_step(b, 0, 0, b->stream);
-    if (!rc) rc = enqueue_batch_step(b, 1, 0, b->stream);
+    int rc = enqueue_batch_step(b, 0, 0, 1, b->stream);
+    if (!rc) rc = enqueue_batch_step(b, 1, 0, 1, b->stream);
     hipError_t e = hipStreamEndCapture(b->stream, &g);
     if (rc) return rc;
     if (e != hipSuccess) return bfail(D3Q19_E_DEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
@@ -189,51 +227,57 @@ static int capture_graph(Batch* b) {
     return 0;
 }
 
-static int direct_step(Batch* b, int write_u_all) {
-    int rc = enqueue_batch_step(b, b->phase, write_u_all, b->stream);
+static int direct_step(Batch* b, int write_u_all, int do_post) {
+    int rc = enqueue_batch_step(b, b->phase, write_u_all, do_post, b->stream);
     if (rc) return rc;
     b->phase ^= 1;
     for (Member& m : b->cells)
-        if (m.sim) m.sim->parity = m.par ^ b->phase;
+        if (m.sim) sync_view(m, b->phase);
     return 0;
 }
 
+// n steps of every cell: dilated marks of the next epoch from the current positions, a first step
+// with no pending force sums, graph-replayed pairs at phase 0, the remaining steps (the last one
+// carries flags), then the last step's force sums and contacts.
 static int batch_run(Batch* b, int n, int flags) {
     if (n <= 0) return 0;
     int rc;
     if (b->dirty && (rc = upload_tables(b))) return rc;
     if (b->h_cnt.ncells == 0) return 0;
     const int wu = (flags & D3Q19_STEP_WRITE_U) ? 1 : 0;
-    int left = n;
-    if (b->phase == 1) {  // realign with the graph's parity
-        if ((rc = direct_step(b, (left == 1) ? wu : 0))) return rc;
+    const BatchLaunch L0 = launch_of(b);
+    if ((rc = launch_sphere_prime_batch(L0, b->phase, b->stream))) return rc;
+    if ((rc = direct_step(b, n == 1 ? wu : 0, 0))) return rc;
+    int left = n - 1;
+    if (left > 0 && b->phase == 1) {  // realign with the graph's parity
+        if ((rc = direct_step(b, left == 1 ? wu : 0, 1))) return rc;
         left--;
     }
-    const int body = wu ? left - 1 : left;
-    const int pairs = body > 0 ? body / 2 : 0;
+    const int tail = (wu && left > 0) ? 1 : 0;
+    const int pairs = (left - tail) / 2;
     if (pairs > 0) {
         if (!b->graph_valid && (rc = capture_graph(b))) return rc;
         for (int i = 0; i < pairs; i++) HIP_OK(hipGraphLaunch(b->graph2, b->stream));
         left -= 2 * pairs;
     }
     while (left > 0) {
-        if ((rc = direct_step(b, (left == 1) ? wu : 0))) return rc;
+        if ((rc = direct_step(b, left == 1 ? wu : 0, 1))) return rc;
         left--;
     }
-    return 0;
+    return launch_sphere_post_batch(launch_of(b), b->phase, b->stream);
 }
 
 static void default_tuning(Batch* b) {
     b->tuning.tile = 256;
-    b->tuning.grid_collide = 4 * b->ncu;
-    b->tuning.grid_sphere = 2 * b->ncu;
-    b->tuning.grid_ibm = 4 * b->ncu;
+    b->tuning.grid_collide = 0;  // one tile per block
+    b->tuning.grid_sphere = 0;   // one sphere per block
+    b->tuning.grid_ibm = 0;      // one marker per wavefront
     b->tuning.collide_ms = 0.0;
 }
 
 static int valid_tuning(const d3q19_tuning* t) {
     if (t->tile != 64 && t->tile != 128 && t->tile != 256) return 0;
-    if (t->grid_collide < 1 || t->grid_sphere < 1 || t->grid_ibm < 1) return 0;
+    if (t->grid_collide < 0 || t->grid_sphere < 0 || t->grid_ibm < 0) return 0;
     if (t->grid_collide > (1 << 20) || t->grid_sphere > (1 << 20) || t->grid_ibm > (1 << 20)) return 0;
     return 1;
 }
@@ -262,18 +306,19 @@ static int autotune(Batch* b) {
     const d3q19_tuning keep = b->tuning;
     d3q19_tuning best = keep;
     double best_ms = 1e30;
-    // candidates: tile of 64/128/256 nodes x k*256 threads per CU in flight (k blocks of 256
-    // threads per CU, or the same thread count in smaller tiles); k = 64 is close to one tile
-    // per block for batches of up to ~16 M nodes
+    // candidates: tile of 64/128/256 nodes x a persistent grid of k*256 threads per CU (k blocks
+    // of 256 threads per CU, or the same thread count in smaller tiles), or one tile per block (0)
     const int tiles[3] = {64, 128, 256};
-    const int ks[6] = {1, 2, 4, 8, 16, 64};
+    const int ks[6] = {0, 2, 4, 8, 16, 32};
     for (int ti = 0; ti < 3; ti++) {
         b->tuning.tile = tiles[ti];
+        b->tuning.grid_collide = 0;
         b->dirty = true;
         if ((rc = upload_tables(b))) return rc;
         for (int pi = 0; pi < 6; pi++) {
             const int grid = ks[pi] * b->ncu * (256 / tiles[ti]);
             b->tuning.grid_collide = grid;
+            b->eff_collide = grid > 0 ? grid : pow2_at_least(b->h_cnt.ntiles, 256);
             double ms = 0;
             if ((rc = time_probe(b, 6, &ms))) return rc;
             if (ms < best_ms) {
@@ -354,7 +399,11 @@ static int batch_add(Batch* b, const int_param_struct* ip, const flp_param_struc
         return rc;
     }
     m.sim = s;
-    m.par = b->phase;  // f[par ^ phase] = fA, the initial state
+    m.par = b->phase;  // f[par ^ phase] = fA and posb[par ^ phase] = the initial positions
+    m.pb[0] = s->pos;
+    m.pb[1] = s->pos_alt;
+    m.vb[0] = s->vel;
+    m.vb[1] = s->vel_alt;
     s->parity = 0;
     b->dirty = true;
     *id = slot;
@@ -639,7 +688,7 @@ int d3q19_batch_observables(d3q19_batch* b, const int* ids, int n, d3q19_observa
 static d3q19_sim* member(d3q19_batch* b, int id) {
     if (!b || id < 0 || id >= b->max_cells || !b->cells[id].sim) return nullptr;
     Member& m = b->cells[id];
-    m.sim->parity = m.par ^ b->phase;
+    sync_view(m, b->phase);
     return m.sim;
 }
 

@@ -155,7 +155,7 @@ __global__ void __launch_bounds__(256) ibm_marker_kernel(IbmArgs a) {
 // Batched: a persistent grid of wavefronts walks the marker list of all cells; the batch's one
 // epoch counter is committed by the first lane of the grid.
 template <class R, int MODE>
-__global__ void __launch_bounds__(256) ibm_batch_kernel(BatchTables t) {
+__global__ void __launch_bounds__(256) ibm_batch_kernel(BatchTables t, int sp) {
     const int lane = threadIdx.x & 63;
     const int wpb = blockDim.x >> 6;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -167,7 +167,11 @@ __global__ void __launch_bounds__(256) ibm_batch_kernel(BatchTables t) {
     for (int gid = blockIdx.x * wpb + (int)(threadIdx.x >> 6); gid < nmark; gid += gridDim.x * wpb) {
         const int sph = t.msph[gid];
         const int4 d = t.sph[sph];  // cell, local sphere, first global marker, markers per sphere
-        ibm_marker<R, MODE>(t.cells[d.x].i, d.y, gid - d.z, lane);
+        const CellDesc& cd = t.cells[d.x];
+        IbmArgs ia = cd.i;
+        ia.pos = cd.posb[cd.par ^ sp ^ 1];  // x(t+1): written by this step's turn
+        ia.vel = cd.velb[cd.par ^ sp ^ 1];
+        ibm_marker<R, MODE>(ia, d.y, gid - d.z, lane);
     }
 }
 
@@ -206,15 +210,15 @@ int launch_ibm(Sim* s, hipStream_t st) {
     return 0;
 }
 
-int launch_ibm_batch(const BatchLaunch& L, hipStream_t st) {
+int launch_ibm_batch(const BatchLaunch& L, int sp, hipStream_t st) {
     const bool dbl = L.storage == D3Q19_STORE_FP64;
     const dim3 grid(L.grid_ibm), block(256);
     if (L.ibm_mode == D3Q19_IBM_PESKIN4) {
-        if (dbl) hipLaunchKernelGGL((ibm_batch_kernel<double, D3Q19_IBM_PESKIN4>), grid, block, 0, st, L.t);
-        else hipLaunchKernelGGL((ibm_batch_kernel<float, D3Q19_IBM_PESKIN4>), grid, block, 0, st, L.t);
+        if (dbl) hipLaunchKernelGGL((ibm_batch_kernel<double, D3Q19_IBM_PESKIN4>), grid, block, 0, st, L.t, sp);
+        else hipLaunchKernelGGL((ibm_batch_kernel<float, D3Q19_IBM_PESKIN4>), grid, block, 0, st, L.t, sp);
     } else {
-        if (dbl) hipLaunchKernelGGL((ibm_batch_kernel<double, D3Q19_IBM_LINEAR_SMOOTHED>), grid, block, 0, st, L.t);
-        else hipLaunchKernelGGL((ibm_batch_kernel<float, D3Q19_IBM_LINEAR_SMOOTHED>), grid, block, 0, st, L.t);
+        if (dbl) hipLaunchKernelGGL((ibm_batch_kernel<double, D3Q19_IBM_LINEAR_SMOOTHED>), grid, block, 0, st, L.t, sp);
+        else hipLaunchKernelGGL((ibm_batch_kernel<float, D3Q19_IBM_LINEAR_SMOOTHED>), grid, block, 0, st, L.t, sp);
     }
     HIP_OK(hipGetLastError());
     return 0;

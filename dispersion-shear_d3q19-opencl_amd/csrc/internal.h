@@ -9,11 +9,15 @@
 //   u, tau       R[3*nint], R[nint]           (R = float, or double for the fp64 build)
 //   force        int64[3*nint] fixed point (2^-40 units), spread with integer atomics so the
 //                sum is independent of order; consumed and zeroed by the next collide.
-//   masks        uint32 epoch stamp per 4x4x4 brick, two arrays by epoch parity: a brick is
-//                marked for the step whose epoch it holds (no per-step clearing).  The kernels of a
-//                step use e = epochs[0] + 1; the spreading kernel commits epochs[0] = e and records
-//                epochs[1] = e as the epoch whose spread force is pending.
-//   particles    double SoA.
+//   masks        uint32 epoch stamp per 4x4x4 brick, three arrays by epoch mod 3: a brick is
+//                marked for the step whose epoch it holds (no per-step clearing).  The step with
+//                epoch e = epochs[0] + 1 writes u in the bricks marked e, consumes the pending
+//                spread force (epoch epochs[1]) in the bricks marked epochs[1], and its spreading
+//                kernel commits epochs[0] = epochs[1] = e.  In the fused step the spheres' turn,
+//                running beside the collide, marks epoch e + 1 (array (e+1) mod 3, which no other
+//                block of the launch reads) for its new positions dilated by one lattice unit.
+//   particles    double SoA; positions and velocities double-buffered (the fused turn reads
+//                x(t), v(t) of every partner while writing x(t+1), v(t+1)).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <type_traits>
@@ -116,6 +120,7 @@ struct CollideArgs {
     long long* facc;
     const uint32_t* mask0;
     const uint32_t* mask1;
+    const uint32_t* mask2;
     const uint32_t* epochs;
     const void* bcraw;
     Geom g;
@@ -157,8 +162,10 @@ struct PartArgs {
     int visc_model;
     double tau0;
     double zbuf;
-    double* pos;
+    double* pos;       // x(t), v(t): read
     double* vel;
+    double* pos_out;   // dynamics writes here (== pos/vel except in the fused turn)
+    double* vel_out;
     double* quat;
     double* omega;
     double* ffluid;
@@ -168,7 +175,9 @@ struct PartArgs {
     const double* mforce;
     uint32_t* mask0;
     uint32_t* mask1;
+    uint32_t* mask2;
     const uint32_t* epochs;
+    int* flag;         // set when a sphere moves a lattice unit or more in one step
 };
 
 // Observable definition of a cell: wall-normal axis A, flow axis F (largest wall-velocity
@@ -187,6 +196,8 @@ struct CellDesc {
     PartArgs p;
     ObsArgs o;
     const void* f[2];
+    double* posb[2];        // sphere positions / velocities: [par ^ sp] is x(t) before a step of parity sp
+    double* velb[2];
     int par;
     int nint;
     int nx, nxy;
@@ -240,7 +251,7 @@ struct Sim {
     void* u = nullptr;    // R[3*nint]
     void* tau = nullptr;  // R[nint]
     long long* facc = nullptr;
-    uint32_t* mask[2] = {nullptr, nullptr};  // brick epoch stamps, by epoch parity
+    uint32_t* mask[3] = {nullptr, nullptr, nullptr};  // brick epoch stamps, by epoch mod 3
     uint32_t* epochs = nullptr;              // [current epoch, epoch of the pending spread force]
     int nbricks = 0;
     // raw wall unknowns of an uploaded pre-collision field: [2 sides][5][plane]
@@ -249,8 +260,13 @@ struct Sim {
     int bc_plane = 0;
     // particles
     int np = 0, ppp = 0, nfa = 1;
-    double* pos = nullptr;     // 3*np
-    double* vel = nullptr;     // 3*np
+    double* pos = nullptr;     // 3*np, current
+    double* vel = nullptr;     // 3*np, current
+    double* pos_alt = nullptr; // the other buffer of the fused turn
+    double* vel_alt = nullptr;
+    int* d_flag = nullptr;     // displacement watchdog of the dilated marks
+    bool pending_post = false; // the last step's force sums and contacts are not done yet
+    bool marks_next = false;   // marks of the next epoch exist (made from the current positions)
     double* quat = nullptr;    // 4*np
     double* omega = nullptr;   // 3*np
     double* ffluid = nullptr;  // 3*np
@@ -263,6 +279,7 @@ struct Sim {
     // step graph
     hipGraphExec_t graph2 = nullptr;   // two steps (ping-pong closes)
     const void* graph_f = nullptr;     // state the graph was captured from
+    const void* graph_pos = nullptr;
     int parity = 0;            // 0: current post-collision field in fA
     int block[3] = {64, 4, 1};
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -303,6 +320,10 @@ int launch_check_finite(Sim* s, const void* fsrc, int* dev_flag);
 int launch_ibm(Sim* s, hipStream_t st);
 int launch_sphere_pre(Sim* s, int do_dyn, int do_mark, hipStream_t st);       // dynamics, brick marks
 int launch_sphere_post(Sim* s, int do_reduce, int do_contacts, hipStream_t st); // force sums, contacts
+int launch_sphere_prime(Sim* s, hipStream_t st);                                // dilated marks, next epoch
+// fused step: the spheres' turn (post of the last step if pending, dynamics into the other
+// position buffer, dilated marks of the epoch after this one) in np extra blocks of the collide
+int launch_step_fused(Sim* s, const void* fin, void* fout, int write_u_all, int do_post, hipStream_t st);
 int launch_fill_equilibrium(Sim* s, void* f);
 int launch_fill_tau(Sim* s);
 
@@ -321,9 +342,11 @@ struct BatchLaunch {
     int grid_ibm;          // persistent blocks (4 markers each per iteration) of the IBM kernel
 };
 int launch_collide_batch(const BatchLaunch& L, int sp, int write_u_all, int probe, hipStream_t st);
-int launch_ibm_batch(const BatchLaunch& L, hipStream_t st);
-int launch_sphere_pre_batch(const BatchLaunch& L, hipStream_t st);
-int launch_sphere_post_batch(const BatchLaunch& L, hipStream_t st);
+// a batched step's sphere turn (its own launch) and collide
+int launch_step_batch(const BatchLaunch& L, int sp, int write_u_all, int do_post, hipStream_t st);
+int launch_ibm_batch(const BatchLaunch& L, int sp, hipStream_t st);
+int launch_sphere_prime_batch(const BatchLaunch& L, int sp, hipStream_t st);
+int launch_sphere_post_batch(const BatchLaunch& L, int sp, hipStream_t st);
 
 // error reporting
 void set_error(const std::string& msg);

@@ -12,7 +12,7 @@
 // (152 B fp32, 76 B fp16/bf16, 304 B fp64); +8 B when tau is per node; +24 B read (+24 B
 // written) only at nodes of bricks holding pending spread force; +12 B u stores only at nodes
 // the next immersed-boundary pass reads (or everywhere when requested).
-#include "internal.h"
+#include "sphere_dev.h"
 
 namespace d3q19 {
 
@@ -327,7 +327,7 @@ __device__ __forceinline__ void collide_node(const CollideArgs& a, int x, int y,
     if (a.has_force || a.use_umask) ep = a.epochs[0] + 1u;  // this step's epoch
     if (a.has_force && !probe) {
         const uint32_t fe = a.epochs[1];
-        const uint32_t* fm = (fe & 1u) ? a.mask1 : a.mask0;
+        const uint32_t* fm = mask_of(a.mask0, a.mask1, a.mask2, fe);
         if (fe != 0u && fm[brick] == fe) {
             long long* fa = a.facc + n;
             const long long vx = fa[0], vy = fa[g.nint], vz = fa[2 * (long long)g.nint];
@@ -348,7 +348,7 @@ __device__ __forceinline__ void collide_node(const CollideArgs& a, int x, int y,
     const R ux = (f[1] - f[2] + f[7] + f[8] + f[9] + f[10] - f[11] - f[12] - f[13] - f[14] + (R)0.5 * gx) * inv_rho;
     const R uy = (f[3] - f[4] + f[7] - f[8] + f[11] - f[12] + f[15] + f[16] - f[17] - f[18] + (R)0.5 * gy) * inv_rho;
     const R uz = (f[5] - f[6] + f[9] - f[10] + f[13] - f[14] + f[15] - f[16] + f[17] - f[18] + (R)0.5 * gz) * inv_rho;
-    if (a.write_u_all || (a.use_umask && ((ep & 1u) ? a.mask1 : a.mask0)[brick] == ep)) {
+    if (a.write_u_all || (a.use_umask && mask_of(a.mask0, a.mask1, a.mask2, ep)[brick] == ep)) {
         R* u = (R*)a.u;
         u[n] = ux;
         u[n + g.nint] = uy;
@@ -461,6 +461,29 @@ __global__ void __launch_bounds__(256) collide_kernel(CollideArgs a) {
     collide_node<ST, COLL, VT, WZ>(a, x, y, z, 0);
 }
 
+// Fused step: blocks [0, nsph) run the spheres' turn (sphere_turn: the force sums and contacts of
+// the last step when do_post, the dynamics into the other position buffer, the dilated marks of
+// the epoch after this one); the other blocks are the collide tiles of collide_kernel.  Nothing a
+// sphere block writes is read by a collide block of the same launch: the collide reads the marks of
+// this epoch and of the pending force, the turn writes the marks of the next epoch (a third array).
+template <int ST, int COLL, int VT, bool WZ>
+__global__ void __launch_bounds__(256) step_kernel(CollideArgs a, PartArgs p, int nsph, int do_post) {
+    if ((int)blockIdx.x < nsph) {
+        sphere_turn(p, blockIdx.x, do_post, p.epochs[0] + 2u);
+        return;
+    }
+    const int L = (int)blockIdx.x - nsph;
+    const int per_plane = a.nbx * a.nby;
+    const int z = L / per_plane;
+    const int rem = L - z * per_plane;
+    const int tyb = rem / a.nbx;
+    const int txb = rem - tyb * a.nbx;
+    const int x = txb * a.bx + (int)(threadIdx.x % a.bx);
+    const int y = tyb * a.by + (int)(threadIdx.x / a.bx);
+    if (x >= a.g.nx || y >= a.g.ny || z >= a.g.nz) return;
+    collide_node<ST, COLL, VT, WZ>(a, x, y, z, 0);
+}
+
 // flat node index of a cell -> (x, y, z); exact for n < 2^24 (float estimate within +-1, corrected)
 __device__ __forceinline__ void split_node(const CellDesc& cd, int n, int& x, int& y, int& z) {
     int zz = (int)((float)n * cd.inv_nxy);
@@ -477,9 +500,9 @@ __device__ __forceinline__ void split_node(const CellDesc& cd, int n, int& x, in
 // each cell's tau law, wall speeds, spheres and lattice pointers come from its descriptor.  sp is
 // the step parity: the cell's state is read from f[par ^ sp] and written to the other lattice.
 template <int ST, int COLL>
-__global__ void __launch_bounds__(256) collide_batch_kernel(BatchTables t, int sp, int write_u_all, int probe) {
+__device__ __forceinline__ void collide_batch_tiles(const BatchTables& t, int sp, int write_u_all, int probe, int first, int stride) {
     const int ntiles = t.cnt->ntiles;
-    for (int k = blockIdx.x; k < ntiles; k += gridDim.x) {
+    for (int k = first; k < ntiles; k += stride) {
         const uint2 tl = t.tiles[k];
         const CellDesc& cd = t.cells[tl.x];
         const int n = (int)tl.y + (int)threadIdx.x;
@@ -492,6 +515,11 @@ __global__ void __launch_bounds__(256) collide_batch_kernel(BatchTables t, int s
         a.write_u_all = write_u_all;
         collide_node<ST, COLL, 2, true>(a, x, y, z, probe);
     }
+}
+
+template <int ST, int COLL>
+__global__ void __launch_bounds__(256) collide_batch_kernel(BatchTables t, int sp, int write_u_all, int probe) {
+    collide_batch_tiles<ST, COLL>(t, sp, write_u_all, probe, blockIdx.x, gridDim.x);
 }
 
 template <int ST, int COLL, int VT>
@@ -521,6 +549,7 @@ CollideArgs make_collide_args(Sim* s, const void* fin, void* fout, int write_u_a
     a.facc = s->facc;
     a.mask0 = s->mask[0];
     a.mask1 = s->mask[1];
+    a.mask2 = s->mask[2];
     a.epochs = s->epochs;
     a.bcraw = s->bcraw;
     a.g = s->g;
@@ -547,6 +576,44 @@ int launch_collide(Sim* s, const void* fin, void* fout, int write_u_all, int use
     case D3Q19_STORE_BF16_SHIFT: dispatch_coll<D3Q19_STORE_BF16_SHIFT>(a, s->opt.collision, s->opt.variable_viscosity, threads, st); break;
     case D3Q19_STORE_FP64: dispatch_coll<D3Q19_STORE_FP64>(a, s->opt.collision, s->opt.variable_viscosity, threads, st); break;
     default: dispatch_coll<D3Q19_STORE_FP32>(a, s->opt.collision, s->opt.variable_viscosity, threads, st); break;
+    }
+    HIP_OK(hipGetLastError());
+    s->bcraw_valid = 0;
+    return 0;
+}
+
+template <int ST, int COLL, int VT>
+static void launch_fused_one(const CollideArgs& a, const PartArgs& p, int nsph, int do_post, int threads, hipStream_t st) {
+    const bool wz = a.g.wall == 2 && a.g.per[0] && a.g.per[1] && a.fl.tan_mask == 0;
+    const dim3 grid(a.nblk + nsph);
+    if (wz) hipLaunchKernelGGL((step_kernel<ST, COLL, VT, true>), grid, dim3(threads), 0, st, a, p, nsph, do_post);
+    else hipLaunchKernelGGL((step_kernel<ST, COLL, VT, false>), grid, dim3(threads), 0, st, a, p, nsph, do_post);
+}
+
+template <int ST>
+static void fused_st(const CollideArgs& a, const PartArgs& p, int nsph, int do_post, int coll, int vt, int threads, hipStream_t st) {
+    if (coll == D3Q19_COLL_BGK) {
+        if (vt) launch_fused_one<ST, D3Q19_COLL_BGK, 1>(a, p, nsph, do_post, threads, st);
+        else launch_fused_one<ST, D3Q19_COLL_BGK, 0>(a, p, nsph, do_post, threads, st);
+    } else {
+        if (vt) launch_fused_one<ST, D3Q19_COLL_MRT, 1>(a, p, nsph, do_post, threads, st);
+        else launch_fused_one<ST, D3Q19_COLL_MRT, 0>(a, p, nsph, do_post, threads, st);
+    }
+}
+
+int launch_step_fused(Sim* s, const void* fin, void* fout, int write_u_all, int do_post, hipStream_t st) {
+    const CollideArgs a = make_collide_args(s, fin, fout, write_u_all, 1);
+    PartArgs p = make_part_args(s);
+    p.pos_out = s->pos_alt;
+    p.vel_out = s->vel_alt;
+    const int nsph = s->np;
+    const int threads = a.bx * a.by;
+    const int coll = s->opt.collision, vt = s->opt.variable_viscosity;
+    switch (s->storage) {
+    case D3Q19_STORE_FP16_SHIFT: fused_st<D3Q19_STORE_FP16_SHIFT>(a, p, nsph, do_post, coll, vt, threads, st); break;
+    case D3Q19_STORE_BF16_SHIFT: fused_st<D3Q19_STORE_BF16_SHIFT>(a, p, nsph, do_post, coll, vt, threads, st); break;
+    case D3Q19_STORE_FP64: fused_st<D3Q19_STORE_FP64>(a, p, nsph, do_post, coll, vt, threads, st); break;
+    default: fused_st<D3Q19_STORE_FP32>(a, p, nsph, do_post, coll, vt, threads, st); break;
     }
     HIP_OK(hipGetLastError());
     s->bcraw_valid = 0;

@@ -525,18 +525,23 @@ class Batch:
         check(self.L.d3q19_batch_observables(self.h, arr, len(ids), out), "batch_observables")
         return list(out)
 
+    def _nn(self, cid):
+        c = self.cases.get(cid)
+        return c.nn if c is not None else 1  # unknown ids are rejected by the library
+
     def read_u(self, cid):
-        u = np.empty(3 * self.cases[cid].nn, dtype=np.float32)
+        u = np.empty(3 * self._nn(cid), dtype=np.float32)
         check(self.L.d3q19_batch_read_u(self.h, cid, _ptr(u)), "batch_read_u")
         return u
 
     def read_f(self, cid):
-        f = np.empty(19 * self.cases[cid].nn, dtype=np.float32)
+        f = np.empty(19 * self._nn(cid), dtype=np.float32)
         check(self.L.d3q19_batch_read_f(self.h, cid, _ptr(f)), "batch_read_f")
         return f
 
     def read_particles_f64(self, cid):
-        n = max(self.cases[cid].np, 1)
+        c = self.cases.get(cid)
+        n = max(c.np if c is not None else 1, 1)
         out = [np.zeros(3 * n) for _ in range(5)]
         check(self.L.d3q19_batch_read_particles_f64(self.h, cid, *[_ptr(x) for x in out]), "batch_read_particles_f64")
         return out

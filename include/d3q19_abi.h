@@ -251,12 +251,14 @@ typedef struct {
     uint64_t arena_bytes;       /* device memory for the cells' state; 0: half of the free memory */
     int32_t reserved[8];
 } d3q19_batch_options;
-/* Launch shape of the batched kernels, chosen by d3q19_batch_autotune or set by the caller. */
+/* Launch shape of the batched kernels, chosen by d3q19_batch_autotune or set by the caller.
+ * A grid of 0 means one work item per block (collide tile, sphere) or per wavefront (marker),
+ * rounded up to a power of two; a positive grid is a persistent grid that strides over the work. */
 typedef struct {
     int32_t tile;               /* threads = nodes per collide tile: 64, 128 or 256 */
-    int32_t grid_collide;       /* persistent blocks of the collide kernel */
-    int32_t grid_sphere;        /* persistent blocks of the sphere kernels */
-    int32_t grid_ibm;           /* persistent blocks of the IBM kernel (4 markers each) */
+    int32_t grid_collide;       /* blocks of the collide kernel */
+    int32_t grid_sphere;        /* blocks of the sphere kernels */
+    int32_t grid_ibm;           /* blocks of the IBM kernel (4 markers at a time each) */
     double collide_ms;          /* measured by the autotuner: mean collide launch */
     int32_t reserved[4];
 } d3q19_tuning;

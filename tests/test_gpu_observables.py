@@ -3,7 +3,8 @@
 Pins:
 - Closed form.  In steady plane Couette flow between wet-node walls at interior z = 0 and
   z = n_z - 1 moving at -U and +U, the shear rate is uniform, 2U/(n_z - 1).  So the wall stress
-  of a Newtonian fluid is nu * 2U/(n_z - 1) and its apparent viscosity is nu.  For a power-law
+  of a Newtonian fluid is nu * 2U/(n_z - 1) and its apparent viscosity is nu (to 1e-6 with fp64
+  populations).  For a power-law
   fluid the local viscosity is K * gdot^(n - 1) everywhere, so the relative viscosity is 1.
 - Restatement.  The kernel's plane sums equal a numpy evaluation of the stress formula on the
   same population field (read back from the GPU in float64), and the drag coefficient equals
@@ -23,22 +24,35 @@ def couette(coll, visc=1, vparams=(0, 0, 0, 0), local=0, tau=0.8, uw=0.01, nz=20
                 clamp=(0.505, 0.0) if local else (0.0, 0.0), storage=storage)
 
 
+@pytest.mark.parametrize("storage", [3, 0])
 @pytest.mark.parametrize("coll", [d3q19.COLL_MRT, d3q19.COLL_BGK])
-def test_newtonian_couette_apparent_viscosity_is_nu(coll):
-    c = couette(coll)
+def test_newtonian_couette_apparent_viscosity_is_nu(coll, storage):
+    # fp64 storage: the steady profile is linear and the apparent viscosity is nu to ~1e-7 (nu
+    # itself comes from the float32 NewtonianTau).  fp32 storage: float32 rounding of the
+    # populations shifts the steady state; the stress moves by ~1.2e-4 relative and the profile
+    # bends by ~3e-5 of U.
+    tol = 1e-6 if storage == 3 else 2e-4
+    c = couette(coll, storage=storage)
     s = d3q19.Sim(c)
-    s.run(1, 40000)
+    s.run(1, 40000, d3q19.STEP_WRITE_U)
     o = s.observables()
+    um = s.plane_mean_u()[0]  # reference z index; interior planes 1 .. Nz-2
     s.close()
-    nu = (np.float32(0.8) - 0.5) / 3.0
-    gdot = 0.02 / 20.0
-    print("coll %d: wall stress %.9e (lo %.9e hi %.9e) nu_app %.9e nu %.9e" % (
-        coll, o.wall_stress, o.wall_stress_lower, o.wall_stress_upper, o.apparent_viscosity, nu), flush=True)
+    nu = (float(np.float32(0.8)) - 0.5) / 3.0
+    U = float(np.float32(0.01))
+    prof = um[1:-1]
+    z = np.arange(len(prof), dtype=np.float64)
+    slope, icpt = np.polyfit(z, prof, 1)
+    lin = np.abs(prof - (slope * z + icpt)).max()
+    print("coll %d: wall stress %.9e (lo %.9e hi %.9e) nu_app %.9e nu %.9e  slope %.9e nominal %.9e  lin %.2e" % (
+        coll, o.wall_stress, o.wall_stress_lower, o.wall_stress_upper, o.apparent_viscosity, nu, slope, 2 * U / 20, lin),
+        flush=True)
     assert o.wall_axis == 2 and o.flow_axis == 0
-    assert abs(o.shear_rate - gdot) < 1e-15
-    assert abs(o.wall_stress - nu * gdot) < 1e-5 * nu * gdot
-    assert abs(o.apparent_viscosity / nu - 1.0) < 1e-5
-    assert abs(o.relative_viscosity - 1.0) < 1e-5
+    assert abs(o.shear_rate - 2 * U / 20.0) < 1e-15
+    assert lin < (1e-6 if storage == 3 else 1e-4) * U
+    assert abs(o.wall_stress - nu * slope) < tol * nu * slope
+    assert abs(o.wall_stress_lower - o.wall_stress_upper) < tol * nu * slope
+    assert abs(o.apparent_viscosity / nu - 1.0) < tol
     assert abs(o.fluid_viscosity - nu) < 1e-12
 
 
@@ -107,7 +121,7 @@ def test_drag_coefficient_matches_numpy():
     s.close()
     F = ff.reshape(-1, 3)[: c.np]
     mean = np.linalg.norm(F, axis=1).mean()
-    U, a = 0.02, 3.0
+    U, a = float(np.float32(0.02)), 3.0  # wall speeds are float32 parameters
     cd = mean / (0.5 * U * U * np.pi * a * a)
     print("drag gpu %.15e numpy %.15e (mean |F| %.6e)" % (o.drag_coefficient, cd, mean), flush=True)
     assert mean > 0
