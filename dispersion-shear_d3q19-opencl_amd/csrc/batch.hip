@@ -174,7 +174,7 @@ static int upload_tables(Batch* b) {
     b->h_cnt.ncells = active_slots(b);
     const int ec = b->tuning.grid_collide > 0 ? b->tuning.grid_collide : pow2_at_least(b->h_cnt.ntiles, 256);
     const int es = b->tuning.grid_sphere > 0 ? b->tuning.grid_sphere : pow2_at_least(b->h_cnt.nsph, 64);
-    const int ei = b->tuning.grid_ibm > 0 ? b->tuning.grid_ibm : pow2_at_least((b->h_cnt.nmark + 3) / 4, 256);
+    const int ei = b->tuning.grid_ibm > 0 ? b->tuning.grid_ibm : pow2_at_least((b->h_cnt.nmark + IBM_WAVES - 1) / IBM_WAVES, 256);
     if (ec != b->eff_collide || es != b->eff_sphere || ei != b->eff_ibm) {
         b->eff_collide = ec;
         b->eff_sphere = es;

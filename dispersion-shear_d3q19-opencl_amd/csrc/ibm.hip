@@ -3,9 +3,12 @@
 // sum_particle_fluid_forces and reset_particle_fluid_forces, GPU_program.cl:81-313).
 //
 // Spreading is order independent: every contribution is rounded to a 2^-40 fixed-point integer
-// and added with 64-bit integer atomics, whose sum does not depend on the order of arrival, so the
-// force field is bitwise reproducible (the reference's slot counter + float read-modify-write,
-// GPU_program.cl:217-224, is not).  The reference's 27-point smoothing of the spread force at
+// and summed with 64-bit integer additions, whose result does not depend on the order of arrival,
+// so the force field is bitwise reproducible (the reference's slot counter + float
+// read-modify-write, GPU_program.cl:217-224, is not).  The L2 atomics bound this pass (about 1.2 M
+// of them per step for 32 spheres of 192 markers); summing a block's 8 consecutive markers in an
+// LDS table first halves their number but measured slower on MI355X, so every lane adds its own
+// contribution.  The reference's 27-point smoothing of the spread force at
 // the next collide (GPU_program.cl:356-377, buffer shell contributing zero and no periodic wrap)
 // is folded into the spread: each marker spreads directly with the convolution of its trilinear
 // weights and the (1/4,1/2,1/4)^3 kernel, restricted to interior target nodes.
@@ -20,6 +23,8 @@ __device__ __forceinline__ double peskin4(double r) {
     return 0.0;
 }
 
+// contribution -w F of one stencil node, rounded to fixed point as a whole and added with 64-bit
+// integer atomics (order independent)
 __device__ __forceinline__ void spread_add(long long* facc, long long nint, int node, double w, const double* F) {
     for (int c = 0; c < 3; c++) {
         const double v = -w * F[c];
@@ -37,9 +42,10 @@ __device__ __forceinline__ double wave_sum(double v) {
 // One 64-lane wavefront per marker; lane k = (kx, ky, kz) in [0,4)^3 owns one node of the
 // marker's 4x4x4 support.  Interpolation is a wave reduction of the lanes' weighted velocities;
 // spreading is three integer atomics per lane.
-// Marker m of sphere p (mforce slot p*ppp + m), executed by one whole wavefront.
-template <class R, int MODE>
-__device__ __forceinline__ void ibm_marker(const IbmArgs& a, int p, int m, int lane) {
+// Marker m of sphere p (mforce slot p*ppp + m), executed by one whole wavefront; each lane whose
+// stencil node receives force calls sink(node, weight, F).
+template <class R, int MODE, class Sink>
+__device__ __forceinline__ void ibm_marker(const IbmArgs& a, int p, int m, int lane, Sink&& sink) {
     const int gid = p * a.ppp + m;
     const Geom& g = a.g;
     const double r0[3] = {(double)a.pts[4 * m], (double)a.pts[4 * m + 1], (double)a.pts[4 * m + 2]};
@@ -121,10 +127,7 @@ __device__ __forceinline__ void ibm_marker(const IbmArgs& a, int p, int m, int l
     const double wr[3] = {om[1] * r0[2] - om[2] * r0[1], om[2] * r0[0] - om[0] * r0[2], om[0] * r0[1] - om[1] * r0[0]};
     double F[3];
     for (int k = 0; k < 3; k++) F[k] = (up[k] - (a.vel[3 * p + k] + wr[k])) * a.dA;
-    if (sok) {
-        const int node = sn[0] + g.nx * (sn[1] + g.ny * sn[2]);
-        spread_add(a.facc, nint, node, sw[0] * sw[1] * sw[2], F);
-    }
+    if (sok) sink(sn[0] + g.nx * (sn[1] + g.ny * sn[2]), sw[0] * sw[1] * sw[2], F);
     if (lane == 0) {
         double* mf = a.mforce + 6 * (long long)gid;
         mf[0] = F[0];
@@ -137,9 +140,9 @@ __device__ __forceinline__ void ibm_marker(const IbmArgs& a, int p, int m, int l
 }
 
 template <class R, int MODE>
-__global__ void __launch_bounds__(256) ibm_marker_kernel(IbmArgs a) {
+__global__ void __launch_bounds__(64 * IBM_WAVES) ibm_marker_kernel(IbmArgs a) {
     const int lane = threadIdx.x & 63;
-    const int gid = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int gid = blockIdx.x * IBM_WAVES + (int)(threadIdx.x >> 6);
     if (gid >= a.np * a.ppp) return;  // uniform per wavefront
     if (gid == 0 && lane == 0) {
         // the step's epoch (read as epochs[0] + 1 by the kernels before this one) is committed,
@@ -149,29 +152,33 @@ __global__ void __launch_bounds__(256) ibm_marker_kernel(IbmArgs a) {
         a.epochs[1] = e;
     }
     const int p = gid / a.ppp;
-    ibm_marker<R, MODE>(a, p, gid - p * a.ppp, lane);
+    const long long nint = a.g.nint;
+    ibm_marker<R, MODE>(a, p, gid - p * a.ppp, lane,
+                        [&](int node, double w, const double* F) { spread_add(a.facc, nint, node, w, F); });
 }
 
 // Batched: a persistent grid of wavefronts walks the marker list of all cells; the batch's one
 // epoch counter is committed by the first lane of the grid.
 template <class R, int MODE>
-__global__ void __launch_bounds__(256) ibm_batch_kernel(BatchTables t, int sp) {
+__global__ void __launch_bounds__(64 * IBM_WAVES) ibm_batch_kernel(BatchTables t, int sp) {
     const int lane = threadIdx.x & 63;
-    const int wpb = blockDim.x >> 6;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         const uint32_t e = t.epochs[0] + 1u;
         t.epochs[0] = e;
         t.epochs[1] = e;
     }
     const int nmark = t.cnt->nmark;
-    for (int gid = blockIdx.x * wpb + (int)(threadIdx.x >> 6); gid < nmark; gid += gridDim.x * wpb) {
+    for (int gid = blockIdx.x * IBM_WAVES + (int)(threadIdx.x >> 6); gid < nmark; gid += gridDim.x * IBM_WAVES) {
         const int sph = t.msph[gid];
         const int4 d = t.sph[sph];  // cell, local sphere, first global marker, markers per sphere
         const CellDesc& cd = t.cells[d.x];
         IbmArgs ia = cd.i;
         ia.pos = cd.posb[cd.par ^ sp ^ 1];  // x(t+1): written by this step's turn
         ia.vel = cd.velb[cd.par ^ sp ^ 1];
-        ibm_marker<R, MODE>(ia, d.y, gid - d.z, lane);
+        const long long nint = ia.g.nint;
+        long long* facc = ia.facc;
+        ibm_marker<R, MODE>(ia, d.y, gid - d.z, lane,
+                            [&](int node, double w, const double* F) { spread_add(facc, nint, node, w, F); });
     }
 }
 
@@ -197,7 +204,7 @@ int launch_ibm(Sim* s, hipStream_t st) {
     if (s->np == 0) return 0;
     const IbmArgs a = make_ibm_args(s);
     const int total = s->np * s->ppp;
-    const int th = 256, blocks = (total + 3) / 4;  // four markers (wavefronts) per block
+    const int th = 64 * IBM_WAVES, blocks = (total + IBM_WAVES - 1) / IBM_WAVES;  // one wavefront per marker
     const bool dbl = s->storage == D3Q19_STORE_FP64;
     if (a.mode == D3Q19_IBM_PESKIN4) {
         if (dbl) hipLaunchKernelGGL((ibm_marker_kernel<double, D3Q19_IBM_PESKIN4>), dim3(blocks), dim3(th), 0, st, a);
@@ -212,7 +219,7 @@ int launch_ibm(Sim* s, hipStream_t st) {
 
 int launch_ibm_batch(const BatchLaunch& L, int sp, hipStream_t st) {
     const bool dbl = L.storage == D3Q19_STORE_FP64;
-    const dim3 grid(L.grid_ibm), block(256);
+    const dim3 grid(L.grid_ibm), block(64 * IBM_WAVES);
     if (L.ibm_mode == D3Q19_IBM_PESKIN4) {
         if (dbl) hipLaunchKernelGGL((ibm_batch_kernel<double, D3Q19_IBM_PESKIN4>), grid, block, 0, st, L.t, sp);
         else hipLaunchKernelGGL((ibm_batch_kernel<float, D3Q19_IBM_PESKIN4>), grid, block, 0, st, L.t, sp);

@@ -32,6 +32,7 @@ constexpr int Q = 19;
 constexpr double FIX_SCALE = 1099511627776.0;       // 2^40
 constexpr double FIX_INV = 1.0 / 1099511627776.0;
 constexpr int BRICK = 4;                              // brick edge (nodes)
+constexpr int IBM_WAVES = 4;                          // markers (wavefronts) per IBM block
 constexpr float FP16_SCALE = 16384.0f;                // 2^14, relative-deviation scale of fp16 storage
 
 // ---- velocity set (reference order, D3Q19-OpenCL.c:9-13) ---------------------------------

@@ -311,7 +311,8 @@ __device__ __forceinline__ R strain_rate(const R* ss, R ux, R uy, R uz, R gx, R 
 // pending spread force and the tau field untouched so a trial launch does not change the state
 // a later step reads.
 template <int ST, int COLL, int VT, bool WZ>
-__device__ __forceinline__ void collide_node(const CollideArgs& a, int x, int y, int z, int probe) {
+__device__ __forceinline__ void collide_node(const CollideArgs& a, const void* fin, void* fout, int write_u_all, int x, int y,
+                                             int z, int probe) {
     using P = Pop<ST>;
     using R = typename P::R;
     using T = typename P::T;
@@ -319,7 +320,7 @@ __device__ __forceinline__ void collide_node(const CollideArgs& a, int x, int y,
     const int n = x + g.nx * (y + g.ny * z);
 
     R f[Q];
-    gather_pre<ST, WZ>((const T*)a.fin, g, a.fl, a.bcraw, a.use_bcraw, x, y, z, f);
+    gather_pre<ST, WZ>((const T*)fin, g, a.fl, a.bcraw, a.use_bcraw, x, y, z, f);
 
     R gx = (R)a.fl.gconst[0], gy = (R)a.fl.gconst[1], gz = (R)a.fl.gconst[2];
     const int brick = (x >> 2) + g.bnx * ((y >> 2) + g.bny * (z >> 2));
@@ -348,7 +349,7 @@ __device__ __forceinline__ void collide_node(const CollideArgs& a, int x, int y,
     const R ux = (f[1] - f[2] + f[7] + f[8] + f[9] + f[10] - f[11] - f[12] - f[13] - f[14] + (R)0.5 * gx) * inv_rho;
     const R uy = (f[3] - f[4] + f[7] - f[8] + f[11] - f[12] + f[15] + f[16] - f[17] - f[18] + (R)0.5 * gy) * inv_rho;
     const R uz = (f[5] - f[6] + f[9] - f[10] + f[13] - f[14] + f[15] - f[16] + f[17] - f[18] + (R)0.5 * gz) * inv_rho;
-    if (a.write_u_all || (a.use_umask && mask_of(a.mask0, a.mask1, a.mask2, ep)[brick] == ep)) {
+    if (write_u_all || (a.use_umask && mask_of(a.mask0, a.mask1, a.mask2, ep)[brick] == ep)) {
         R* u = (R*)a.u;
         u[n] = ux;
         u[n + g.nint] = uy;
@@ -436,7 +437,7 @@ __device__ __forceinline__ void collide_node(const CollideArgs& a, int x, int y,
             fo[q] = f[q] + o[q] + fg[q];
         });
     }
-    T* out = (T*)a.fout;
+    T* out = (T*)fout;
     Unroll<0, Q>::run([&](auto qc) __attribute__((always_inline)) {
         constexpr int q = decltype(qc)::value;
         P::template st<q>(out, (long long)q * g.qstride + n, fo[q]);
@@ -458,7 +459,7 @@ __global__ void __launch_bounds__(256) collide_kernel(CollideArgs a) {
     const int x = txb * a.bx + (int)(threadIdx.x % a.bx);
     const int y = tyb * a.by + (int)(threadIdx.x / a.bx);
     if (x >= a.g.nx || y >= a.g.ny || z >= a.g.nz) return;
-    collide_node<ST, COLL, VT, WZ>(a, x, y, z, 0);
+    collide_node<ST, COLL, VT, WZ>(a, a.fin, a.fout, a.write_u_all, x, y, z, 0);
 }
 
 // Fused step: blocks [0, nsph) run the spheres' turn (sphere_turn: the force sums and contacts of
@@ -481,7 +482,7 @@ __global__ void __launch_bounds__(256) step_kernel(CollideArgs a, PartArgs p, in
     const int x = txb * a.bx + (int)(threadIdx.x % a.bx);
     const int y = tyb * a.by + (int)(threadIdx.x / a.bx);
     if (x >= a.g.nx || y >= a.g.ny || z >= a.g.nz) return;
-    collide_node<ST, COLL, VT, WZ>(a, x, y, z, 0);
+    collide_node<ST, COLL, VT, WZ>(a, a.fin, a.fout, a.write_u_all, x, y, z, 0);
 }
 
 // flat node index of a cell -> (x, y, z); exact for n < 2^24 (float estimate within +-1, corrected)
@@ -509,11 +510,7 @@ __device__ __forceinline__ void collide_batch_tiles(const BatchTables& t, int sp
         if (n >= cd.nint) continue;
         int x, y, z;
         split_node(cd, n, x, y, z);
-        CollideArgs a = cd.c;
-        a.fin = cd.f[cd.par ^ sp];
-        a.fout = (void*)cd.f[cd.par ^ sp ^ 1];
-        a.write_u_all = write_u_all;
-        collide_node<ST, COLL, 2, true>(a, x, y, z, probe);
+        collide_node<ST, COLL, 2, true>(cd.c, cd.f[cd.par ^ sp], (void*)cd.f[cd.par ^ sp ^ 1], write_u_all, x, y, z, probe);
     }
 }
 
