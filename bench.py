@@ -165,16 +165,28 @@ def ensemble(args):
 
 
 def serving(args):
-    """Poisson arrivals of shear-cell solve requests into the continuous-batching server (fp32
-    populations); then the same requests as sequential plain fp32 solves (single simulations, one
-    after another, same convergence rule).  Latency = completion - scheduled arrival."""
+    """Shear-cell solve requests as sequential plain fp32 solves (single simulations, one after another),
+    then the same requests arriving as a Poisson stream into the continuous-batching server.  The
+    arrival rate is `load` times the sequential capacity measured first (offered load above what the
+    sequential solver sustains), unless --rate is given.  Latency = completion - scheduled arrival;
+    the sequential latencies are those of a FIFO single server with the measured service times under
+    the same arrivals."""
     import d3q19
     decks = c5_requests(count=args.requests, edge_range=(32, args.max_edge), storage=0)
     cases = [d3q19.Case(lines=d) for d in decks]
-    arr = poisson_arrivals(args.rate, len(cases))
     sp = d3q19.solve_params(tolerance=1e-4, check_interval=args.check_interval, min_steps=10 * args.check_interval,
                             max_steps=args.max_steps)
-    srv = d3q19.Server(storage=0, collision=d3q19.COLL_BGK, ibm=d3q19.IBM_PESKIN4, arena_bytes=args.arena_gb << 30)
+    service, seq = [], {}
+    for i, c in enumerate(cases):
+        a = time.perf_counter()
+        seq[i] = d3q19.solve(c, sp)
+        service.append(time.perf_counter() - a)
+        print("serving: sequential %d/%d %s %d steps %.3f s" % (i + 1, len(cases), c.N, seq[i].steps, service[-1]), flush=True)
+    cap_seq = len(cases) / sum(service)
+    rate = args.rate if args.rate > 0 else args.load * cap_seq
+    arr = poisson_arrivals(rate, len(cases))
+    srv = d3q19.Server(storage=args.serve_storage, collision=d3q19.COLL_BGK, ibm=d3q19.IBM_PESKIN4,
+                       arena_bytes=args.arena_gb << 30)
     t0 = d3q19.clock()
     nxt = 0
     rid_of = {}
@@ -186,21 +198,12 @@ def serving(args):
             nxt += 1
         for r in srv.progress():
             done[rid_of[r.id]] = r
-            print("serving: request %d done (%s, status %d, %d steps)" % (rid_of[r.id], cases[rid_of[r.id]].N, r.status, r.steps),
-                  flush=True)
         q, res, _ = srv.status()
         if q == 0 and res == 0 and nxt < len(cases):
             time.sleep(max(0.0, arr[nxt] - (d3q19.clock() - t0)))
     srv.close()
     lat = [done[i].t_finish - (t0 + arr[i]) for i in range(len(cases))]
     span = max(done[i].t_finish for i in done) - (t0 + arr[0])
-    # sequential plain fp32 solves of the same requests
-    service = []
-    seq = {}
-    for i, c in enumerate(cases):
-        a = time.perf_counter()
-        seq[i] = d3q19.solve(c, sp)
-        service.append(time.perf_counter() - a)
     fin, lat_seq = 0.0, []
     for i in range(len(cases)):
         fin = max(fin, arr[i]) + service[i]
@@ -208,14 +211,16 @@ def serving(args):
     agree = max(abs(done[i].obs.wall_stress - seq[i].obs.wall_stress) / max(abs(seq[i].obs.wall_stress), 1e-300)
                 for i in range(len(cases)))
     steps_equal = sum(1 for i in range(len(cases)) if done[i].steps == seq[i].steps)
-    return {"requests": len(cases), "rate_per_s": args.rate, "edges": [c.N[0] - 2 for c in cases],
-            "spheres": [c.np for c in cases], "check_interval": args.check_interval, "tolerance": 1e-4,
-            "max_steps": args.max_steps, "converged": sum(1 for r in done.values() if r.status == 0),
+    names = {0: "fp32", 1: "fp16_shift", 2: "bf16_shift", 3: "fp64"}
+    return {"requests": len(cases), "arrival_rate_per_s": round(rate, 4), "offered_load_vs_sequential": round(rate / cap_seq, 3),
+            "edges": [c.N[0] - 2 for c in cases], "spheres": [c.np for c in cases], "check_interval": args.check_interval,
+            "tolerance": 1e-4, "max_steps": args.max_steps, "server_storage": names[args.serve_storage],
+            "converged": sum(1 for r in done.values() if r.status == 0),
             "batched": {"solves_per_s": round(len(cases) / span, 4), "p50_latency_s": round(pct(lat, 50), 4),
                         "p99_latency_s": round(pct(lat, 99), 4), "makespan_s": round(span, 3)},
-            "sequential": {"solves_per_s": round(len(cases) / sum(service), 4), "p50_latency_s": round(pct(lat_seq, 50), 4),
+            "sequential": {"solves_per_s": round(cap_seq, 4), "p50_latency_s": round(pct(lat_seq, 50), 4),
                            "p99_latency_s": round(pct(lat_seq, 99), 4), "busy_s": round(sum(service), 3)},
-            "speedup_solves_per_s": round((len(cases) / span) / (len(cases) / sum(service)), 3),
+            "speedup_solves_per_s": round((len(cases) / span) / cap_seq, 3),
             "max_rel_diff_wall_stress_vs_sequential": agree, "same_step_count": steps_equal}
 
 
@@ -269,7 +274,9 @@ def main():
     ap.add_argument("--ens-warmup", type=int, default=10)
     ap.add_argument("--ens-seq-steps", type=int, default=100)
     ap.add_argument("--requests", type=int, default=24)
-    ap.add_argument("--rate", type=float, default=4.0)
+    ap.add_argument("--rate", type=float, default=0.0, help="arrivals per second (0: load x sequential capacity)")
+    ap.add_argument("--load", type=float, default=4.0)
+    ap.add_argument("--serve-storage", type=int, default=0)
     ap.add_argument("--max-edge", type=int, default=160)
     ap.add_argument("--check-interval", type=int, default=100)
     ap.add_argument("--max-steps", type=int, default=40000)

@@ -204,10 +204,10 @@ CellDesc make_cell_desc(Sim* s) {
     d.p = make_part_args(s);
     d.f[0] = s->fA;
     d.f[1] = s->fB;
-    d.posb[0] = s->pos;
-    d.posb[1] = s->pos_alt;
-    d.velb[0] = s->vel;
-    d.velb[1] = s->vel_alt;
+    d.posb[0] = gp(s->pos);
+    d.posb[1] = gp(s->pos_alt);
+    d.velb[0] = gp(s->vel);
+    d.velb[1] = gp(s->vel_alt);
     d.par = s->parity;
     d.nint = s->g.nint;
     d.nx = s->g.nx;

@@ -82,7 +82,7 @@ struct Batch {
     d3q19_tuning tuning{};
     // launch grids in effect: a tuning field of 0 means "one work item per block / wavefront",
     // rounded up to a power of two so that the captured graph survives most membership changes
-    int eff_collide = 1, eff_sphere = 1, eff_ibm = 1;
+    int eff_collide = 1, eff_collide_vt = 0, eff_sphere = 1, eff_ibm = 1;
     hipGraphExec_t graph2 = nullptr;
     bool graph_valid = false;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -100,7 +100,9 @@ static BatchLaunch launch_of(const Batch* b) {
     L.collision = b->bo.collision;
     L.ibm_mode = b->bo.ibm_kernel;
     L.tile = b->tuning.tile;
+    L.span = b->tuning.span > 0 ? b->tuning.span : 1;
     L.grid_collide = b->eff_collide;
+    L.grid_collide_vt = b->eff_collide_vt;
     L.grid_sphere = b->eff_sphere;
     L.grid_ibm = b->eff_ibm;
     return L;
@@ -129,17 +131,17 @@ static int grow(T** p, size_t* cap, size_t need, bool* moved) {
     return 0;
 }
 
-static int pow2_at_least(long long n, int floor_) {
-    long long p = floor_;
-    while (p < n && p < (1LL << 24)) p <<= 1;
-    return (int)p;
+// one block per work item, rounded up to a multiple of 1024 blocks (at least 256)
+static int grid_for(long long n) {
+    if (n <= 256) return 256;
+    return (int)(((n + 1023) / 1024) * 1024);
 }
 
 // Rebuild the work lists from the member set and upload them (stream ordered).
 static int upload_tables(Batch* b) {
     HIP_OK(hipStreamSynchronize(b->stream));  // host mirrors may still be read by earlier copies
     const int nslots = active_slots(b);
-    const int tile = b->tuning.tile;
+    const int tile = b->tuning.tile * (b->tuning.span > 0 ? b->tuning.span : 1);  // nodes per work-list entry
     b->h_tiles.clear();
     b->h_sph.clear();
     b->h_msph.clear();
@@ -156,11 +158,23 @@ static int upload_tables(Batch* b) {
         d.f[0] = s->fA;
         d.f[1] = s->fB;
         for (int k = 0; k < 2; k++) {
-            d.posb[k] = m.pb[k];
-            d.velb[k] = m.vb[k];
+            d.posb[k] = gp(m.pb[k]);
+            d.velb[k] = gp(m.vb[k]);
         }
         b->h_cells[c] = d;
-        for (int base = 0; base < s->g.nint; base += tile) b->h_tiles.push_back(make_uint2((unsigned)c, (unsigned)base));
+    }
+    // tiles: the constant-tau cells first, then the per-node-tau cells
+    for (int pass = 0; pass < 2; pass++) {
+        if (pass == 1) b->h_cnt.ntiles_ct = (int)b->h_tiles.size();
+        for (int c = 0; c < b->max_cells; c++) {
+            const Sim* s = b->cells[c].sim;
+            if (!s || (s->opt.variable_viscosity ? 1 : 0) != pass) continue;
+            for (int base = 0; base < s->g.nint; base += tile) b->h_tiles.push_back(make_uint2((unsigned)c, (unsigned)base));
+        }
+    }
+    for (int c = 0; c < b->max_cells; c++) {
+        const Sim* s = b->cells[c].sim;
+        if (!s) continue;
         for (int p = 0; p < s->np; p++) {
             const int sidx = (int)b->h_sph.size();
             b->h_sph.push_back(make_int4(c, p, (int)b->h_msph.size(), s->ppp));
@@ -172,11 +186,15 @@ static int upload_tables(Batch* b) {
     b->h_cnt.nsph = (int)b->h_sph.size();
     b->h_cnt.nmark = (int)b->h_msph.size();
     b->h_cnt.ncells = active_slots(b);
-    const int ec = b->tuning.grid_collide > 0 ? b->tuning.grid_collide : pow2_at_least(b->h_cnt.ntiles, 256);
-    const int es = b->tuning.grid_sphere > 0 ? b->tuning.grid_sphere : pow2_at_least(b->h_cnt.nsph, 64);
-    const int ei = b->tuning.grid_ibm > 0 ? b->tuning.grid_ibm : pow2_at_least((b->h_cnt.nmark + IBM_WAVES - 1) / IBM_WAVES, 256);
-    if (ec != b->eff_collide || es != b->eff_sphere || ei != b->eff_ibm) {
+    b->h_cnt.span = b->tuning.span > 0 ? b->tuning.span : 1;
+    const int n_ct = b->h_cnt.ntiles_ct, n_vt = b->h_cnt.ntiles - b->h_cnt.ntiles_ct;
+    const int ec = n_ct == 0 ? 0 : (b->tuning.grid_collide > 0 ? b->tuning.grid_collide : grid_for(n_ct));
+    const int ev = n_vt == 0 ? 0 : (b->tuning.grid_collide > 0 ? b->tuning.grid_collide : grid_for(n_vt));
+    const int es = b->tuning.grid_sphere > 0 ? b->tuning.grid_sphere : grid_for(b->h_cnt.nsph);
+    const int ei = b->tuning.grid_ibm > 0 ? b->tuning.grid_ibm : grid_for((b->h_cnt.nmark + IBM_WAVES - 1) / IBM_WAVES);
+    if (ec != b->eff_collide || ev != b->eff_collide_vt || es != b->eff_sphere || ei != b->eff_ibm) {
         b->eff_collide = ec;
+        b->eff_collide_vt = ev;
         b->eff_sphere = es;
         b->eff_ibm = ei;
         b->graph_valid = false;
@@ -269,7 +287,8 @@ static int batch_run(Batch* b, int n, int flags) {
 
 static void default_tuning(Batch* b) {
     b->tuning.tile = 256;
-    b->tuning.grid_collide = 0;  // one tile per block
+    b->tuning.span = 1;
+    b->tuning.grid_collide = 0;  // one work-list entry per block
     b->tuning.grid_sphere = 0;   // one sphere per block
     b->tuning.grid_ibm = 0;      // one marker per wavefront
     b->tuning.collide_ms = 0.0;
@@ -277,6 +296,7 @@ static void default_tuning(Batch* b) {
 
 static int valid_tuning(const d3q19_tuning* t) {
     if (t->tile != 64 && t->tile != 128 && t->tile != 256) return 0;
+    if (t->span != 0 && t->span != 1 && t->span != 2 && t->span != 4 && t->span != 8) return 0;
     if (t->grid_collide < 0 || t->grid_sphere < 0 || t->grid_ibm < 0) return 0;
     if (t->grid_collide > (1 << 20) || t->grid_sphere > (1 << 20) || t->grid_ibm > (1 << 20)) return 0;
     return 1;
@@ -309,24 +329,29 @@ static int autotune(Batch* b) {
     // candidates: tile of 64/128/256 nodes x a persistent grid of k*256 threads per CU (k blocks
     // of 256 threads per CU, or the same thread count in smaller tiles), or one tile per block (0)
     const int tiles[3] = {64, 128, 256};
-    const int ks[6] = {0, 2, 4, 8, 16, 32};
-    for (int ti = 0; ti < 3; ti++) {
-        b->tuning.tile = tiles[ti];
-        b->tuning.grid_collide = 0;
-        b->dirty = true;
-        if ((rc = upload_tables(b))) return rc;
-        for (int pi = 0; pi < 6; pi++) {
-            const int grid = ks[pi] * b->ncu * (256 / tiles[ti]);
-            b->tuning.grid_collide = grid;
-            b->eff_collide = grid > 0 ? grid : pow2_at_least(b->h_cnt.ntiles, 256);
-            double ms = 0;
-            if ((rc = time_probe(b, 6, &ms))) return rc;
-            if (ms < best_ms) {
-                best_ms = ms;
-                best = b->tuning;
+    const int spans[4] = {1, 2, 4, 8};
+    const int ks[4] = {0, 4, 8, 16};
+    for (int ti = 0; ti < 3; ti++)
+        for (int si = 0; si < 4; si++) {
+            b->tuning.tile = tiles[ti];
+            b->tuning.span = spans[si];
+            b->tuning.grid_collide = 0;
+            b->dirty = true;
+            if ((rc = upload_tables(b))) return rc;
+            for (int pi = 0; pi < 4; pi++) {
+                const int grid = ks[pi] * b->ncu * (256 / tiles[ti]);
+                b->tuning.grid_collide = grid;
+                const int n_ct = b->h_cnt.ntiles_ct, n_vt = b->h_cnt.ntiles - n_ct;
+                b->eff_collide = n_ct == 0 ? 0 : (grid > 0 ? grid : grid_for(n_ct));
+                b->eff_collide_vt = n_vt == 0 ? 0 : (grid > 0 ? grid : grid_for(n_vt));
+                double ms = 0;
+                if ((rc = time_probe(b, 6, &ms))) return rc;
+                if (ms < best_ms) {
+                    best_ms = ms;
+                    best = b->tuning;
+                }
             }
         }
-    }
     best.collide_ms = best_ms;
     b->tuning = best;
     b->dirty = true;

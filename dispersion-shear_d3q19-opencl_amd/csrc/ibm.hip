@@ -25,11 +25,13 @@ __device__ __forceinline__ double peskin4(double r) {
 
 // contribution -w F of one stencil node, rounded to fixed point as a whole and added with 64-bit
 // integer atomics (order independent)
-__device__ __forceinline__ void spread_add(long long* facc, long long nint, int node, double w, const double* F) {
+__device__ __forceinline__ void spread_add(D3Q19_GLOBAL long long* facc, long long nint, int node, double w, const double* F) {
     for (int c = 0; c < 3; c++) {
         const double v = -w * F[c];
         const long long iv = __double2ll_rn(v * FIX_SCALE);
-        if (iv != 0) atomicAdd((unsigned long long*)(facc + c * nint + node), (unsigned long long)iv);
+        if (iv != 0)
+            __hip_atomic_fetch_add((D3Q19_GLOBAL unsigned long long*)(facc + c * nint + node), (unsigned long long)iv,
+                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 
@@ -114,7 +116,7 @@ __device__ __forceinline__ void ibm_marker(const IbmArgs& a, int p, int m, int l
         iw = sok ? w3 : 0.0;
         inode = sok ? sn[0] + g.nx * (sn[1] + g.ny * sn[2]) : 0;
     }
-    const R* u = (const R*)a.u;
+    const D3Q19_GLOBAL R* u = (const D3Q19_GLOBAL R*)a.u;
     const long long nint = g.nint;
     double c0 = 0, c1 = 0, c2 = 0;
     if (iok) {
@@ -123,13 +125,13 @@ __device__ __forceinline__ void ibm_marker(const IbmArgs& a, int p, int m, int l
         c2 = iw * (double)u[2 * nint + inode];
     }
     const double up[3] = {wave_sum(c0), wave_sum(c1), wave_sum(c2)};
-    const double* om = a.omega + 3 * p;
+    const D3Q19_GLOBAL double* om = a.omega + 3 * p;
     const double wr[3] = {om[1] * r0[2] - om[2] * r0[1], om[2] * r0[0] - om[0] * r0[2], om[0] * r0[1] - om[1] * r0[0]};
     double F[3];
     for (int k = 0; k < 3; k++) F[k] = (up[k] - (a.vel[3 * p + k] + wr[k])) * a.dA;
     if (sok) sink(sn[0] + g.nx * (sn[1] + g.ny * sn[2]), sw[0] * sw[1] * sw[2], F);
     if (lane == 0) {
-        double* mf = a.mforce + 6 * (long long)gid;
+        D3Q19_GLOBAL double* mf = a.mforce + 6 * (long long)gid;
         mf[0] = F[0];
         mf[1] = F[1];
         mf[2] = F[2];
@@ -176,7 +178,7 @@ __global__ void __launch_bounds__(64 * IBM_WAVES) ibm_batch_kernel(BatchTables t
         ia.pos = cd.posb[cd.par ^ sp ^ 1];  // x(t+1): written by this step's turn
         ia.vel = cd.velb[cd.par ^ sp ^ 1];
         const long long nint = ia.g.nint;
-        long long* facc = ia.facc;
+        D3Q19_GLOBAL long long* facc = ia.facc;
         ibm_marker<R, MODE>(ia, d.y, gid - d.z, lane,
                             [&](int node, double w, const double* F) { spread_add(facc, nint, node, w, F); });
     }
@@ -189,14 +191,14 @@ IbmArgs make_ibm_args(Sim* s) {
     a.ppp = s->ppp;
     a.mode = s->opt.ibm_kernel;
     a.dA = (double)s->fp.PointArea;
-    a.pos = s->pos;
-    a.vel = s->vel;
-    a.omega = s->omega;
-    a.pts = s->pts;
-    a.u = s->u;
-    a.facc = s->facc;
-    a.mforce = s->mforce;
-    a.epochs = s->epochs;
+    a.pos = gp(s->pos);
+    a.vel = gp(s->vel);
+    a.omega = gp(s->omega);
+    a.pts = gp(s->pts);
+    a.u = gp(s->u);
+    a.facc = gp(s->facc);
+    a.mforce = gp(s->mforce);
+    a.epochs = gp(s->epochs);
     return a;
 }
 

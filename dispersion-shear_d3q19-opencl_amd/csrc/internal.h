@@ -110,6 +110,25 @@ struct FluidParams {
     int tan_mask;         // bit a: tangential velocity BC on axis a
 };
 
+// Pointer fields that device code dereferences in place are declared in the global address space
+// for the device compilation (same size and layout as the host's plain pointers).
+#ifdef __HIP_DEVICE_COMPILE__
+#define D3Q19_GLOBAL __attribute__((address_space(1)))
+#else
+#define D3Q19_GLOBAL
+#endif
+// host-side stores into such fields (the cast is an identity in the host compilation)
+template <class T> inline D3Q19_GLOBAL T* gp(T* p) { return (D3Q19_GLOBAL T*)p; }
+template <class T> inline const D3Q19_GLOBAL T* gp(const T* p) { return (const D3Q19_GLOBAL T*)p; }
+// Device code reaching memory through a generic pointer that the compiler cannot trace to a kernel
+// argument (for example one read from the batched kernels' descriptor table) gets FLAT
+// instructions, whose waits also cover LDS traffic; the batched collide measured up to 30% slower
+// that way.  to_global states the address space so that loads, stores and atomics are global_*.
+template <class T> __host__ __device__ __forceinline__ const D3Q19_GLOBAL T* to_global(const void* p) {
+    return (const D3Q19_GLOBAL T*)p;
+}
+template <class T> __host__ __device__ __forceinline__ D3Q19_GLOBAL T* to_global(void* p) { return (D3Q19_GLOBAL T*)p; }
+
 // ---- per-cell kernel arguments ------------------------------------------------------------------
 // The same structs drive the single-simulation kernels (passed by value) and the batched kernels
 // (read from a device table of CellDesc, one per cell).
@@ -140,14 +159,14 @@ struct IbmArgs {
     Geom g;
     int np, ppp, mode;
     double dA;
-    const double* pos;
-    const double* vel;
-    const double* omega;
-    const float* pts;
-    const void* u;
-    long long* facc;
-    double* mforce;
-    uint32_t* epochs;
+    const D3Q19_GLOBAL double* pos;
+    const D3Q19_GLOBAL double* vel;
+    const D3Q19_GLOBAL double* omega;
+    const D3Q19_GLOBAL float* pts;
+    const D3Q19_GLOBAL void* u;
+    D3Q19_GLOBAL long long* facc;
+    D3Q19_GLOBAL double* mforce;
+    D3Q19_GLOBAL uint32_t* epochs;
 };
 
 struct PartArgs {
@@ -163,23 +182,25 @@ struct PartArgs {
     int visc_model;
     double tau0;
     double zbuf;
-    double* pos;       // x(t), v(t): read
-    double* vel;
-    double* pos_out;   // dynamics writes here (== pos/vel except in the fused turn)
-    double* vel_out;
-    double* quat;
-    double* omega;
-    double* ffluid;
-    double* tfluid;
-    double* fcont;
-    double* tcont;
-    const double* mforce;
-    uint32_t* mask0;
-    uint32_t* mask1;
-    uint32_t* mask2;
-    const uint32_t* epochs;
-    int* flag;         // set when a sphere moves a lattice unit or more in one step
+    D3Q19_GLOBAL double* pos;       // x(t), v(t): read
+    D3Q19_GLOBAL double* vel;
+    D3Q19_GLOBAL double* pos_out;   // dynamics writes here (== pos/vel except in the fused turn)
+    D3Q19_GLOBAL double* vel_out;
+    D3Q19_GLOBAL double* quat;
+    D3Q19_GLOBAL double* omega;
+    D3Q19_GLOBAL double* ffluid;
+    D3Q19_GLOBAL double* tfluid;
+    D3Q19_GLOBAL double* fcont;
+    D3Q19_GLOBAL double* tcont;
+    const D3Q19_GLOBAL double* mforce;
+    D3Q19_GLOBAL uint32_t* mask0;
+    D3Q19_GLOBAL uint32_t* mask1;
+    D3Q19_GLOBAL uint32_t* mask2;
+    const D3Q19_GLOBAL uint32_t* epochs;
+    D3Q19_GLOBAL int* flag;         // set when a sphere moves a lattice unit or more in one step
 };
+
+
 
 // Observable definition of a cell: wall-normal axis A, flow axis F (largest wall-velocity
 // difference among the other axes), MRT rate of the F-A stress moment.
@@ -197,8 +218,8 @@ struct CellDesc {
     PartArgs p;
     ObsArgs o;
     const void* f[2];
-    double* posb[2];        // sphere positions / velocities: [par ^ sp] is x(t) before a step of parity sp
-    double* velb[2];
+    D3Q19_GLOBAL double* posb[2];  // sphere positions / velocities: [par ^ sp] is x(t) before a step of parity sp
+    D3Q19_GLOBAL double* velb[2];
     int par;
     int nint;
     int nx, nxy;
@@ -208,7 +229,9 @@ struct CellDesc {
 // Work-list sizes of a batch, read by the persistent kernels from device memory so that a captured
 // graph stays valid while cells join and leave.
 struct BatchCounts {
-    int ntiles;   // collide tiles (tile = blockDim consecutive nodes of one cell)
+    int ntiles;     // collide work-list entries (span x blockDim consecutive nodes of one cell)
+    int ntiles_ct;  // the first ntiles_ct entries belong to cells of constant tau
+    int span;       // tiles per entry
     int nsph;     // spheres over all cells
     int nmark;    // markers over all cells
     int ncells;
@@ -216,7 +239,7 @@ struct BatchCounts {
 
 struct BatchTables {
     const CellDesc* cells;
-    const uint2* tiles;      // (cell, first node)
+    const uint2* tiles;      // collide work list: (cell, first node); an entry covers span tiles
     const int4* sph;         // (cell, local sphere, first global marker, markers per sphere)
     const int* msph;         // global marker -> global sphere
     const BatchCounts* cnt;
@@ -338,7 +361,9 @@ struct BatchLaunch {
     BatchTables t;
     int storage, collision, ibm_mode;
     int tile;              // threads (= nodes) per collide tile
-    int grid_collide;      // persistent blocks of the collide kernel
+    int span;              // tiles per collide work-list entry
+    int grid_collide;      // blocks of the collide kernel over the constant-tau cells' tiles
+    int grid_collide_vt;   // blocks of the collide kernel over the per-node-tau cells' tiles
     int grid_sphere;       // persistent blocks of the sphere kernels
     int grid_ibm;          // persistent blocks (4 markers each per iteration) of the IBM kernel
 };

@@ -28,31 +28,31 @@ template <int ST> struct Pop;
 template <> struct Pop<D3Q19_STORE_FP32> {
     using T = float;
     using R = float;
-    template <int QI> __device__ static R ld(const T* p, unsigned off) { return p[off]; }
-    template <int QI> __device__ static double ldd(const T* p, unsigned off) { return (double)p[off]; }
-    template <int QI> __device__ static void st(T* p, long long off, R v) { p[off] = v; }
+    template <int QI, class PT> __device__ static R ld(PT p, unsigned off) { return p[off]; }
+    template <int QI, class PT> __device__ static double ldd(PT p, unsigned off) { return (double)p[off]; }
+    template <int QI, class PT> __device__ static void st(PT p, long long off, R v) { p[off] = v; }
 };
 template <> struct Pop<D3Q19_STORE_FP64> {
     using T = double;
     using R = double;
-    template <int QI> __device__ static R ld(const T* p, unsigned off) { return p[off]; }
-    template <int QI> __device__ static double ldd(const T* p, unsigned off) { return p[off]; }
-    template <int QI> __device__ static void st(T* p, long long off, R v) { p[off] = v; }
+    template <int QI, class PT> __device__ static R ld(PT p, unsigned off) { return p[off]; }
+    template <int QI, class PT> __device__ static double ldd(PT p, unsigned off) { return p[off]; }
+    template <int QI, class PT> __device__ static void st(PT p, long long off, R v) { p[off] = v; }
 };
 // binary16 of the relative deviation (f_q - w_q)/w_q scaled by 2^14: deviations down to ~1e-8
 // stay normal numbers and the headroom reaches |dev| = 4.
 template <> struct Pop<D3Q19_STORE_FP16_SHIFT> {
     using T = _Float16;
     using R = float;
-    template <int QI> __device__ static R ld(const T* p, unsigned off) {
+    template <int QI, class PT> __device__ static R ld(PT p, unsigned off) {
         constexpr float w = (float)wq(QI);
         constexpr float sc = (float)(wq(QI) / 16384.0);
         return fmaf((float)p[off], sc, w);
     }
-    template <int QI> __device__ static double ldd(const T* p, unsigned off) {
+    template <int QI, class PT> __device__ static double ldd(PT p, unsigned off) {
         return wq(QI) + (double)p[off] * (wq(QI) / 16384.0);
     }
-    template <int QI> __device__ static void st(T* p, long long off, R v) {
+    template <int QI, class PT> __device__ static void st(PT p, long long off, R v) {
         constexpr float w = (float)wq(QI);
         constexpr float isc = (float)(16384.0 / wq(QI));
         p[off] = (_Float16)((v - w) * isc);
@@ -62,14 +62,14 @@ template <> struct Pop<D3Q19_STORE_FP16_SHIFT> {
 template <> struct Pop<D3Q19_STORE_BF16_SHIFT> {
     using T = uint16_t;
     using R = float;
-    template <int QI> __device__ static R ld(const T* p, unsigned off) {
+    template <int QI, class PT> __device__ static R ld(PT p, unsigned off) {
         constexpr float w = (float)wq(QI);
         return w + bf16_to_f32(p[off]);
     }
-    template <int QI> __device__ static double ldd(const T* p, unsigned off) {
+    template <int QI, class PT> __device__ static double ldd(PT p, unsigned off) {
         return wq(QI) + (double)bf16_to_f32(p[off]);
     }
-    template <int QI> __device__ static void st(T* p, long long off, R v) {
+    template <int QI, class PT> __device__ static void st(PT p, long long off, R v) {
         constexpr float w = (float)wq(QI);
         p[off] = f32_to_bf16(v - w);
     }
@@ -107,8 +107,8 @@ template <> __device__ __forceinline__ float exp_<float>(float x) { return __bui
 // rho from the knowns and the prescribed u_n.  Once after an upload the raw uploaded unknowns
 // are used instead (layout [side][k][plane], k = rank of q among the unknowns).
 // Unknown values are selected, not branched on, so every store to f[] has a fixed index.
-template <int A, int S, class R, class RW>
-__device__ __forceinline__ void wall_layer(R* f, const float* uw, int calc_rho, int use_raw, const RW* raw, int pidx, int psz) {
+template <int A, int S, class R, class RWP>
+__device__ __forceinline__ void wall_layer(R* f, const float* uw, int calc_rho, int use_raw, RWP raw, int pidx, int psz) {
     constexpr int inward = S == 0 ? 1 : -1;
     constexpr int a1 = A == 0 ? 1 : 0;
     constexpr int a2 = A == 2 ? 1 : 2;
@@ -153,8 +153,8 @@ __host__ __device__ inline int plane_size(const Geom& g, int A) {
 // (periodic x and y, moving walls normal to z, no tangential passes), which keeps f[] in
 // registers; WALLZ = false handles any wall axis, none, and the tangential passes.
 // RO = double decodes the stored populations in double precision (observables).
-template <int ST, bool WALLZ, class RO = typename Pop<ST>::R>
-__device__ __forceinline__ void gather_pre(const typename Pop<ST>::T* __restrict__ fin, const Geom& g, const FluidParams& fl,
+template <int ST, bool WALLZ, class RO = typename Pop<ST>::R, class PIn = const typename Pop<ST>::T*>
+__device__ __forceinline__ void gather_pre(PIn fin, const Geom& g, const FluidParams& fl,
                                            const void* bcraw, int use_raw, int x, int y, int z, RO* f) {
     using P = Pop<ST>;
     using R = RO;
@@ -177,33 +177,33 @@ __device__ __forceinline__ void gather_pre(const typename Pop<ST>::T* __restrict
         const int ox = cx(q) == 0 ? 0 : (cx(q) > 0 ? oxm : oxp);
         const int oy = cy(q) == 0 ? 0 : (cy(q) > 0 ? oym : oyp);
         const int oz = cz(q) == 0 ? 0 : (cz(q) > 0 ? ozm : ozp);
-        const typename P::T* base = fin + (long long)q * qs;
+        const auto base = fin + (long long)q * qs;
         if constexpr (std::is_same<RO, typename P::R>::value) f[q] = P::template ld<q>(base, (unsigned)((int)n + ox + oy + oz));
         else f[q] = P::template ldd<q>(base, (unsigned)((int)n + ox + oy + oz));
     });
-    const RW* rw = (const RW*)bcraw;
+    const auto rw = to_global<RW>(bcraw);
     const int cr = fl.calc_rho;
     if (g.wall == 0) {
-        if (x == 0) wall_layer<0, 0, R, RW>(f, fl.uw[0], cr, use_raw, rw, y + g.ny * z, g.ny * g.nz);
-        if (x == g.nx - 1) wall_layer<0, 1, R, RW>(f, fl.uw[1], cr, use_raw, rw, y + g.ny * z, g.ny * g.nz);
+        if (x == 0) wall_layer<0, 0, R>(f, fl.uw[0], cr, use_raw, rw, y + g.ny * z, g.ny * g.nz);
+        if (x == g.nx - 1) wall_layer<0, 1, R>(f, fl.uw[1], cr, use_raw, rw, y + g.ny * z, g.ny * g.nz);
     }
     if (g.wall == 1) {
-        if (y == 0) wall_layer<1, 0, R, RW>(f, fl.uw[0], cr, use_raw, rw, x + g.nx * z, g.nx * g.nz);
-        if (y == g.ny - 1) wall_layer<1, 1, R, RW>(f, fl.uw[1], cr, use_raw, rw, x + g.nx * z, g.nx * g.nz);
+        if (y == 0) wall_layer<1, 0, R>(f, fl.uw[0], cr, use_raw, rw, x + g.nx * z, g.nx * g.nz);
+        if (y == g.ny - 1) wall_layer<1, 1, R>(f, fl.uw[1], cr, use_raw, rw, x + g.nx * z, g.nx * g.nz);
     }
     if (g.wall == 2) {
-        if (z == 0) wall_layer<2, 0, R, RW>(f, fl.uw[0], cr, use_raw, rw, x + g.nx * y, g.nx * g.ny);
-        if (z == g.nz - 1) wall_layer<2, 1, R, RW>(f, fl.uw[1], cr, use_raw, rw, x + g.nx * y, g.nx * g.ny);
+        if (z == 0) wall_layer<2, 0, R>(f, fl.uw[0], cr, use_raw, rw, x + g.nx * y, g.nx * g.ny);
+        if (z == g.nz - 1) wall_layer<2, 1, R>(f, fl.uw[1], cr, use_raw, rw, x + g.nx * y, g.nx * g.ny);
     }
     // tangential passes in axis order after the wall pass (sim_main.c:388-404); skipped while the
     // raw uploaded unknowns are in use, since the periodic gather then returns the uploaded values
     if (fl.tan_mask && !use_raw) {
-        if ((fl.tan_mask & 1) && x == 0) wall_layer<0, 0, R, RW>(f, fl.uw[0], 0, 0, rw, 0, 1);
-        if ((fl.tan_mask & 1) && x == g.nx - 1) wall_layer<0, 1, R, RW>(f, fl.uw[1], 0, 0, rw, 0, 1);
-        if ((fl.tan_mask & 2) && y == 0) wall_layer<1, 0, R, RW>(f, fl.uw[0], 0, 0, rw, 0, 1);
-        if ((fl.tan_mask & 2) && y == g.ny - 1) wall_layer<1, 1, R, RW>(f, fl.uw[1], 0, 0, rw, 0, 1);
-        if ((fl.tan_mask & 4) && z == 0) wall_layer<2, 0, R, RW>(f, fl.uw[0], 0, 0, rw, 0, 1);
-        if ((fl.tan_mask & 4) && z == g.nz - 1) wall_layer<2, 1, R, RW>(f, fl.uw[1], 0, 0, rw, 0, 1);
+        if ((fl.tan_mask & 1) && x == 0) wall_layer<0, 0, R>(f, fl.uw[0], 0, 0, rw, 0, 1);
+        if ((fl.tan_mask & 1) && x == g.nx - 1) wall_layer<0, 1, R>(f, fl.uw[1], 0, 0, rw, 0, 1);
+        if ((fl.tan_mask & 2) && y == 0) wall_layer<1, 0, R>(f, fl.uw[0], 0, 0, rw, 0, 1);
+        if ((fl.tan_mask & 2) && y == g.ny - 1) wall_layer<1, 1, R>(f, fl.uw[1], 0, 0, rw, 0, 1);
+        if ((fl.tan_mask & 4) && z == 0) wall_layer<2, 0, R>(f, fl.uw[0], 0, 0, rw, 0, 1);
+        if ((fl.tan_mask & 4) && z == g.nz - 1) wall_layer<2, 1, R>(f, fl.uw[1], 0, 0, rw, 0, 1);
     }
 }
 
@@ -320,17 +320,18 @@ __device__ __forceinline__ void collide_node(const CollideArgs& a, const void* f
     const int n = x + g.nx * (y + g.ny * z);
 
     R f[Q];
-    gather_pre<ST, WZ>((const T*)fin, g, a.fl, a.bcraw, a.use_bcraw, x, y, z, f);
+    gather_pre<ST, WZ>(to_global<T>(fin), g, a.fl, a.bcraw, a.use_bcraw, x, y, z, f);
 
     R gx = (R)a.fl.gconst[0], gy = (R)a.fl.gconst[1], gz = (R)a.fl.gconst[2];
     const int brick = (x >> 2) + g.bnx * ((y >> 2) + g.bny * (z >> 2));
     uint32_t ep = 0;
-    if (a.has_force || a.use_umask) ep = a.epochs[0] + 1u;  // this step's epoch
+    const auto epochs = to_global<uint32_t>(a.epochs);
+    if (a.has_force || a.use_umask) ep = epochs[0] + 1u;  // this step's epoch
     if (a.has_force && !probe) {
-        const uint32_t fe = a.epochs[1];
-        const uint32_t* fm = mask_of(a.mask0, a.mask1, a.mask2, fe);
+        const uint32_t fe = epochs[1];
+        const auto fm = to_global<uint32_t>(mask_of(a.mask0, a.mask1, a.mask2, fe));
         if (fe != 0u && fm[brick] == fe) {
-            long long* fa = a.facc + n;
+            const auto fa = to_global<long long>(a.facc) + n;
             const long long vx = fa[0], vy = fa[g.nint], vz = fa[2 * (long long)g.nint];
             if ((vx | vy | vz) != 0) {
                 gx += (R)((double)vx * FIX_INV);
@@ -349,8 +350,8 @@ __device__ __forceinline__ void collide_node(const CollideArgs& a, const void* f
     const R ux = (f[1] - f[2] + f[7] + f[8] + f[9] + f[10] - f[11] - f[12] - f[13] - f[14] + (R)0.5 * gx) * inv_rho;
     const R uy = (f[3] - f[4] + f[7] - f[8] + f[11] - f[12] + f[15] + f[16] - f[17] - f[18] + (R)0.5 * gy) * inv_rho;
     const R uz = (f[5] - f[6] + f[9] - f[10] + f[13] - f[14] + f[15] - f[16] + f[17] - f[18] + (R)0.5 * gz) * inv_rho;
-    if (write_u_all || (a.use_umask && mask_of(a.mask0, a.mask1, a.mask2, ep)[brick] == ep)) {
-        R* u = (R*)a.u;
+    if (write_u_all || (a.use_umask && to_global<uint32_t>(mask_of(a.mask0, a.mask1, a.mask2, ep))[brick] == ep)) {
+        const auto u = to_global<R>(a.u);
         u[n] = ux;
         u[n + g.nint] = uy;
         u[n + 2 * (long long)g.nint] = uz;
@@ -376,7 +377,7 @@ __device__ __forceinline__ void collide_node(const CollideArgs& a, const void* f
     });
 
     R tau = (R)a.fl.tau0;
-    R* taup = (R*)a.tau;
+    const auto taup = to_global<R>(a.tau);
     if (VT == 1 || (VT == 2 && a.vt)) {
         const R told = taup[n];
         const R itold = rcp_(told);
@@ -437,7 +438,7 @@ __device__ __forceinline__ void collide_node(const CollideArgs& a, const void* f
             fo[q] = f[q] + o[q] + fg[q];
         });
     }
-    T* out = (T*)fout;
+    const auto out = to_global<T>(fout);
     Unroll<0, Q>::run([&](auto qc) __attribute__((always_inline)) {
         constexpr int q = decltype(qc)::value;
         P::template st<q>(out, (long long)q * g.qstride + n, fo[q]);
@@ -500,23 +501,34 @@ __device__ __forceinline__ void split_node(const CellDesc& cd, int n, int& x, in
 // consecutive nodes of one cell).  Cells are shear cells (periodic x and y, walls normal to z);
 // each cell's tau law, wall speeds, spheres and lattice pointers come from its descriptor.  sp is
 // the step parity: the cell's state is read from f[par ^ sp] and written to the other lattice.
-template <int ST, int COLL>
-__device__ __forceinline__ void collide_batch_tiles(const BatchTables& t, int sp, int write_u_all, int probe, int first, int stride) {
-    const int ntiles = t.cnt->ntiles;
-    for (int k = first; k < ntiles; k += stride) {
+// VT selects the tile range: 0 the constant-tau cells' tiles [0, ntiles_ct), 1 the per-node-tau
+// cells' tiles [ntiles_ct, ntiles).  Two specialised launches keep the constant-tau cells free of
+// the per-node-tau code's registers (occupancy).
+template <int ST, int COLL, int VT>
+__device__ __forceinline__ void collide_batch_tiles(const BatchTables& t, const CellDesc* __restrict__ cells, int sp, int write_u_all,
+                                                    int probe, int first, int stride) {
+    const int lo = VT ? t.cnt->ntiles_ct : 0;
+    const int hi = VT ? t.cnt->ntiles : t.cnt->ntiles_ct;
+    const int span = t.cnt->span;
+    for (int k = lo + first; k < hi; k += stride) {
         const uint2 tl = t.tiles[k];
-        const CellDesc& cd = t.cells[tl.x];
-        const int n = (int)tl.y + (int)threadIdx.x;
-        if (n >= cd.nint) continue;
-        int x, y, z;
-        split_node(cd, n, x, y, z);
-        collide_node<ST, COLL, 2, true>(cd.c, cd.f[cd.par ^ sp], (void*)cd.f[cd.par ^ sp ^ 1], write_u_all, x, y, z, probe);
+        const CellDesc& cd = cells[tl.x];
+        const void* fin = cd.f[cd.par ^ sp];
+        void* fout = (void*)cd.f[cd.par ^ sp ^ 1];
+        for (int j = 0; j < span; j++) {
+            const int n = (int)tl.y + j * (int)blockDim.x + (int)threadIdx.x;
+            if (n >= cd.nint) break;
+            int x, y, z;
+            split_node(cd, n, x, y, z);
+            collide_node<ST, COLL, VT, true>(cd.c, fin, fout, write_u_all, x, y, z, probe);
+        }
     }
 }
 
-template <int ST, int COLL>
-__global__ void __launch_bounds__(256) collide_batch_kernel(BatchTables t, int sp, int write_u_all, int probe) {
-    collide_batch_tiles<ST, COLL>(t, sp, write_u_all, probe, blockIdx.x, gridDim.x);
+template <int ST, int COLL, int VT>
+__global__ void __launch_bounds__(256) collide_batch_kernel(BatchTables t, const CellDesc* __restrict__ cells, int sp, int write_u_all,
+                                                            int probe) {
+    collide_batch_tiles<ST, COLL, VT>(t, cells, sp, write_u_all, probe, blockIdx.x, gridDim.x);
 }
 
 template <int ST, int COLL, int VT>
@@ -601,8 +613,8 @@ static void fused_st(const CollideArgs& a, const PartArgs& p, int nsph, int do_p
 int launch_step_fused(Sim* s, const void* fin, void* fout, int write_u_all, int do_post, hipStream_t st) {
     const CollideArgs a = make_collide_args(s, fin, fout, write_u_all, 1);
     PartArgs p = make_part_args(s);
-    p.pos_out = s->pos_alt;
-    p.vel_out = s->vel_alt;
+    p.pos_out = gp(s->pos_alt);
+    p.vel_out = gp(s->vel_alt);
     const int nsph = s->np;
     const int threads = a.bx * a.by;
     const int coll = s->opt.collision, vt = s->opt.variable_viscosity;
@@ -617,12 +629,20 @@ int launch_step_fused(Sim* s, const void* fin, void* fout, int write_u_all, int 
     return 0;
 }
 
+template <int ST, int COLL>
+static void collide_batch_coll(const BatchLaunch& L, int sp, int write_u_all, int probe, hipStream_t st) {
+    if (L.grid_collide > 0)
+        hipLaunchKernelGGL((collide_batch_kernel<ST, COLL, 0>), dim3(L.grid_collide), dim3(L.tile), 0, st, L.t, L.t.cells, sp,
+                           write_u_all, probe);
+    if (L.grid_collide_vt > 0)
+        hipLaunchKernelGGL((collide_batch_kernel<ST, COLL, 1>), dim3(L.grid_collide_vt), dim3(L.tile), 0, st, L.t, L.t.cells, sp,
+                           write_u_all, probe);
+}
+
 template <int ST>
 static void collide_batch_st(const BatchLaunch& L, int sp, int write_u_all, int probe, hipStream_t st) {
-    if (L.collision == D3Q19_COLL_BGK)
-        hipLaunchKernelGGL((collide_batch_kernel<ST, D3Q19_COLL_BGK>), dim3(L.grid_collide), dim3(L.tile), 0, st, L.t, sp, write_u_all, probe);
-    else
-        hipLaunchKernelGGL((collide_batch_kernel<ST, D3Q19_COLL_MRT>), dim3(L.grid_collide), dim3(L.tile), 0, st, L.t, sp, write_u_all, probe);
+    if (L.collision == D3Q19_COLL_BGK) collide_batch_coll<ST, D3Q19_COLL_BGK>(L, sp, write_u_all, probe, st);
+    else collide_batch_coll<ST, D3Q19_COLL_MRT>(L, sp, write_u_all, probe, st);
 }
 
 int launch_collide_batch(const BatchLaunch& L, int sp, int write_u_all, int probe, hipStream_t st) {
@@ -943,8 +963,8 @@ int launch_check_finite(Sim* s, const void* fsrc, int* flag) { return moments_an
 //   Pi_FA = sum_q c_qF c_qA f_q,  u = (sum_q c_q f_q + g/2)/rho,  g = the constant body force,
 // with tau the relaxation time of the F-A stress moment at the node (the per-node tau, NewtonianTau,
 // or 1/1.4 for the MRT x-y moment, whose rate the reference fixes, GPU_program.cl:453,567-568).
-template <int ST>
-__device__ double node_stress(const CellDesc& cd, const typename Pop<ST>::T* fin, int x, int y, int z) {
+template <int ST, class PIn>
+__device__ double node_stress(const CellDesc& cd, PIn fin, int x, int y, int z) {
     using R = typename Pop<ST>::R;
     const CollideArgs& c = cd.c;
     double f[Q];
@@ -968,7 +988,7 @@ __device__ double node_stress(const CellDesc& cd, const typename Pop<ST>::T* fin
     const double gA = A == 0 ? g3[0] : (A == 1 ? g3[1] : g3[2]);
     double tau = (double)c.fl.tau0;
     if (cd.o.rate_fixed > 0.f) tau = 1.0 / (double)cd.o.rate_fixed;
-    else if (c.vt) tau = (double)((const R*)c.tau)[x + c.g.nx * (y + c.g.ny * z)];
+    else if (c.vt) tau = (double)to_global<R>(c.tau)[x + c.g.nx * (y + c.g.ny * z)];
     return -(1.0 - 0.5 / tau) * (pi - rho * uF * uA + 0.5 * (uF * gA + uA * gF));
 }
 
@@ -988,7 +1008,7 @@ __global__ void __launch_bounds__(256) observables_kernel(const CellDesc* __rest
             const int n1 = A == 0 ? g.ny : g.nx;
             const int n2 = A == 2 ? g.ny : g.nz;
             const int pidx = role == 0 ? 1 : nA - 2;
-            const typename Pop<ST>::T* fin = (const typename Pop<ST>::T*)cd.f[cd.par ^ sp];
+            const auto fin = to_global<typename Pop<ST>::T>(cd.f[cd.par ^ sp]);
             for (int i = threadIdx.x; i < n1 * n2; i += blockDim.x) {
                 const int u1 = i % n1, u2 = i / n1;
                 const int x = A == 0 ? pidx : u1;
@@ -1001,7 +1021,7 @@ __global__ void __launch_bounds__(256) observables_kernel(const CellDesc* __rest
     } else {
         const PartArgs& p = cd.p;
         for (int k = threadIdx.x; k < p.np; k += blockDim.x) {
-            const double* F = p.ffluid + 3 * k;
+            const auto F = p.ffluid + 3 * k;
             v0 += sqrt(F[0] * F[0] + F[1] * F[1] + F[2] * F[2]);
             v1 += 1.0;
         }

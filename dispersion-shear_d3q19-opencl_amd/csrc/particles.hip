@@ -51,7 +51,7 @@ __global__ void __launch_bounds__(256) sphere_turn_batch_kernel(BatchTables t, i
         p.vel = cd.velb[cd.par ^ sp];
         p.pos_out = cd.posb[cd.par ^ sp ^ 1];
         p.vel_out = cd.velb[cd.par ^ sp ^ 1];
-        sphere_turn(p, d.y, do_post, e);
+            sphere_turn(p, d.y, do_post, e);
         __syncthreads();
     }
 }
@@ -85,22 +85,22 @@ PartArgs make_part_args(Sim* s) {
     a.visc_model = s->ip.ViscosityModel;
     a.tau0 = (double)s->fp.NewtonianTau;
     a.zbuf = (double)s->fp.ParticleZBuffer;
-    a.pos = s->pos;
-    a.vel = s->vel;
-    a.pos_out = s->pos;
-    a.vel_out = s->vel;
-    a.quat = s->quat;
-    a.omega = s->omega;
-    a.ffluid = s->ffluid;
-    a.tfluid = s->tfluid;
-    a.fcont = s->fcont;
-    a.tcont = s->tcont;
-    a.mforce = s->mforce;
-    a.mask0 = s->mask[0];
-    a.mask1 = s->mask[1];
-    a.mask2 = s->mask[2];
-    a.flag = s->d_flag;
-    a.epochs = s->epochs;
+    a.pos = gp(s->pos);
+    a.vel = gp(s->vel);
+    a.pos_out = gp(s->pos);
+    a.vel_out = gp(s->vel);
+    a.quat = gp(s->quat);
+    a.omega = gp(s->omega);
+    a.ffluid = gp(s->ffluid);
+    a.tfluid = gp(s->tfluid);
+    a.fcont = gp(s->fcont);
+    a.tcont = gp(s->tcont);
+    a.mforce = gp(s->mforce);
+    a.mask0 = gp(s->mask[0]);
+    a.mask1 = gp(s->mask[1]);
+    a.mask2 = gp(s->mask[2]);
+    a.flag = gp(s->d_flag);
+    a.epochs = gp(s->epochs);
     return a;
 }
 

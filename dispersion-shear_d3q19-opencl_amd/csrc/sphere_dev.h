@@ -24,11 +24,8 @@
 namespace d3q19 {
 
 // brick stamps of epoch e live in array e mod 3
-__host__ __device__ inline uint32_t* mask_of(uint32_t* m0, uint32_t* m1, uint32_t* m2, uint32_t e) {
-    const uint32_t r = e % 3u;
-    return r == 0u ? m0 : (r == 1u ? m1 : m2);
-}
-__host__ __device__ inline const uint32_t* mask_of(const uint32_t* m0, const uint32_t* m1, const uint32_t* m2, uint32_t e) {
+template <class P>
+__host__ __device__ inline P mask_of(P m0, P m1, P m2, uint32_t e) {
     const uint32_t r = e % 3u;
     return r == 0u ? m0 : (r == 1u ? m1 : m2);
 }
@@ -41,7 +38,8 @@ __host__ __device__ inline const uint32_t* mask_of(const uint32_t* m0, const uin
 // x(t+1) while the sphere moves less than pad - 2 lattice units per step and axis).  Periodic axes
 // wrap node indices, wall axes clip them.  The per-axis brick sets are built in LDS, then the block
 // stamps their product.
-__device__ void mark_sphere(const double* c3, const Geom& g, double radius, uint32_t* mk, uint32_t e, int pad) {
+template <class MP>
+__device__ void mark_sphere(const double* c3, const Geom& g, double radius, MP mk, uint32_t e, int pad) {
     __shared__ uint8_t fl[3][1024];
     __shared__ int lists[3][64];
     __shared__ int counts[3];
@@ -101,7 +99,7 @@ __device__ __forceinline__ void dynamics_one(const PartArgs& a, int p) {
         // a displacement that the dilated marks cannot cover is reported, never run silently
         double dx = fabs(x1 - x0);
         dx = fmin(dx, a.sys[k] - dx);
-        if (!(dx < 1.0) && a.flag) atomicOr(a.flag, 1);
+        if (!(dx < 1.0) && a.flag) __hip_atomic_fetch_or(a.flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     double q[4], av[3];
     for (int k = 0; k < 4; k++) q[k] = a.quat[4 * p + k];
@@ -156,7 +154,7 @@ __device__ __forceinline__ void sphere_post(const PartArgs& a, int p, int do_red
     if (do_reduce) {
         double acc[6] = {0, 0, 0, 0, 0, 0};
         for (int m = threadIdx.x; m < a.ppp; m += blockDim.x) {
-            const double* mf = a.mforce + 6 * ((long long)p * a.ppp + m);
+            const auto mf = a.mforce + 6 * ((long long)p * a.ppp + m);
             for (int k = 0; k < 6; k++) acc[k] += mf[k];
         }
         block_sums<6>(acc, sh6);

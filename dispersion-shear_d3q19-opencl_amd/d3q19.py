@@ -123,7 +123,7 @@ class BatchOptions(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("tile", C.c_int32), ("grid_collide", C.c_int32), ("grid_sphere", C.c_int32), ("grid_ibm", C.c_int32),
-                ("collide_ms", C.c_double), ("reserved", C.c_int32 * 4)]
+                ("collide_ms", C.c_double), ("span", C.c_int32), ("reserved", C.c_int32 * 3)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}

@@ -252,15 +252,18 @@ typedef struct {
     int32_t reserved[8];
 } d3q19_batch_options;
 /* Launch shape of the batched kernels, chosen by d3q19_batch_autotune or set by the caller.
- * A grid of 0 means one work item per block (collide tile, sphere) or per wavefront (marker),
- * rounded up to a power of two; a positive grid is a persistent grid that strides over the work. */
+ * A grid of 0 means one work item per block (collide work-list entry, sphere) or per wavefront
+ * (marker), rounded up to a multiple of 1024 blocks; a positive grid is a persistent grid that
+ * strides over the work.  A collide work-list entry is `span` consecutive tiles of one cell, so
+ * that a block reads its cell's descriptor once for span tiles. */
 typedef struct {
     int32_t tile;               /* threads = nodes per collide tile: 64, 128 or 256 */
     int32_t grid_collide;       /* blocks of the collide kernel */
     int32_t grid_sphere;        /* blocks of the sphere kernels */
     int32_t grid_ibm;           /* blocks of the IBM kernel (4 markers at a time each) */
     double collide_ms;          /* measured by the autotuner: mean collide launch */
-    int32_t reserved[4];
+    int32_t span;               /* tiles of one cell per work-list entry (1, 2, 4 or 8); 0 = 1 */
+    int32_t reserved[3];
 } d3q19_tuning;
 int d3q19_batch_create(int device, const d3q19_batch_options* bo, d3q19_batch** out);
 int d3q19_batch_destroy(d3q19_batch* b);

@@ -1,0 +1,83 @@
+"""GPU: long-run agreement with the float64 oracle on the 64^3 single-sphere workload.
+
+Workload: 64^3 box, walls at +-0.01, BGK tau 0.8, one d=16 sphere (256 markers) at the centre,
+Peskin-4.  2000 steps of the GPU, fp32 and fp16-shifted storage, against 2000 steps of the float64
+oracle from the same state.
+
+Bars, 1e-3 for fp32 storage:
+- centreline profile u_x(z) at x = y = centre: relative L-inf;
+- sphere velocity: absolute, in units of the wall speed (the sphere sits on the zero-velocity
+  plane);
+- sphere angular velocity: relative to the oracle's;
+- sphere position: absolute, in lattice units;
+- wall stress: relative;
+- drag coefficient: absolute.
+fp16-shifted storage keeps the sphere and drag bars (measured: velocity 1.3e-5 U, angular velocity
+8.3e-4, position 6.5e-5, drag 4.7e-4).  The stress lives in the non-equilibrium part of the
+populations, about 7e-4 of w_q here, and binary16 rounding of f - w costs a few percent of it per
+node.  So fp16 carries measured envelopes instead: profile 5e-3 (measured 1.7e-3), wall stress
+3e-2 (measured 1.3e-2).
+"""
+import numpy as np
+import pytest
+
+import d3q19
+import oracle_py
+from test_gpu_observables import numpy_wall_stress
+from workloads import c1_deck
+
+pytestmark = pytest.mark.gpu
+
+STEPS = 2000
+U_WALL = float(np.float32(0.01))
+
+
+@pytest.fixture(scope="module")
+def oracle_state():
+    c = d3q19.Case(lines=c1_deck(storage=0))
+    o = oracle_py.Oracle(c, "f64").load_case()
+    print("oracle: %d steps of %s" % (STEPS, c.N), flush=True)
+    o.step(STEPS)
+    N = c.N
+    u = o.arr(oracle_py.U).reshape(3, N[2], N[1], N[0])
+    kin = o.arr(oracle_py.PARKIN).reshape(4, c.np, 4)
+    nfa = max(c.ip.WorkGroupsPerParticle, 1)
+    ff = o.arr(oracle_py.PARFLUID).reshape(2, c.np, nfa, 4).sum(2)[0, :, :3]
+    lo, hi = numpy_wall_stress(o.arr(oracle_py.F_C).astype(np.float64), None, c, float(np.float32(0.8)))
+    a = 0.5 * c.fp.ParticleDiam
+    state = {"ux_centre": u[0, 1:-1, N[1] // 2, N[0] // 2].astype(np.float64).copy(),
+             "pos": kin[0, :, :3].astype(np.float64).copy(), "vel": kin[1, :, :3].astype(np.float64).copy(),
+             "omega": kin[3, :, :3].astype(np.float64).copy(), "wall_stress": 0.5 * (lo + hi),
+             "drag": np.linalg.norm(ff, axis=1).mean() / (0.5 * U_WALL ** 2 * np.pi * a * a)}
+    o.close()
+    return state
+
+
+@pytest.mark.parametrize("storage", [0, 1])
+def test_c1_2000_steps_match_f64_oracle(oracle_state, storage):
+    ref = oracle_state
+    c = d3q19.Case(lines=c1_deck(storage=storage))
+    s = d3q19.Sim(c)
+    s.upload_case()
+    s.run(1, STEPS, d3q19.STEP_WRITE_U)
+    N = c.N
+    u = s.read_u().reshape(3, N[2], N[1], N[0])
+    ux = u[0, 1:-1, N[1] // 2, N[0] // 2].astype(np.float64)
+    pos, vel, om, ff, tf = s.read_particles_f64()
+    o = s.observables()
+    assert s.check_finite() == 0
+    s.close()
+    e_prof = np.abs(ux - ref["ux_centre"]).max() / np.abs(ref["ux_centre"]).max()
+    e_vel = np.abs(vel.reshape(-1, 3)[: c.np] - ref["vel"]).max() / U_WALL
+    e_om = np.abs(om.reshape(-1, 3)[: c.np] - ref["omega"]).max() / np.abs(ref["omega"]).max()
+    e_pos = np.abs(pos.reshape(-1, 3)[: c.np] - ref["pos"]).max()
+    e_ws = abs(o.wall_stress - ref["wall_stress"]) / abs(ref["wall_stress"])
+    e_cd = abs(o.drag_coefficient - ref["drag"])
+    print("storage %d: profile %.2e vel/U %.2e omega %.2e pos %.2e wall stress %.2e drag %.2e (C_d %.6f vs %.6f)" % (
+        storage, e_prof, e_vel, e_om, e_pos, e_ws, e_cd, o.drag_coefficient, ref["drag"]), flush=True)
+    assert e_vel < 1e-3
+    assert e_om < 1e-3
+    assert e_pos < 1e-3
+    assert e_cd < 1e-3
+    assert e_prof < (1e-3 if storage == 0 else 5e-3)
+    assert e_ws < (1e-3 if storage == 0 else 3e-2)
