@@ -252,6 +252,21 @@ def test_abi_library_exports_every_declared_entry_point():
     assert C.sizeof(d3q19.IntParams) == 332 and C.sizeof(d3q19.FlpParams) == 176
 
 
+def test_kernel_parameters_carry_no_address_space():
+    # The global-address-space qualifier exists in the device compilation only; on a kernel
+    # parameter it changes the device-side mangled name, and the launch then finds no kernel.
+    csrc = os.path.join(ROOT, "dispersion-shear_d3q19-opencl_amd", "csrc")
+    bad = []
+    for f in sorted(os.listdir(csrc)):
+        if not f.endswith((".hip", ".h")):
+            continue
+        src = open(os.path.join(csrc, f)).read()
+        for m in re.finditer(r"__global__[^{;]*?\b(\w+)\s*\(([^)]*)\)\s*\{", src):
+            if "D3Q19_GLOBAL" in m.group(2):
+                bad.append("%s:%s" % (f, m.group(1)))
+    assert not bad, bad
+
+
 def test_footprint_and_error_strings():
     L = d3q19.lib()
     c = d3q19.Case(lines=["total_lattice_size 66 66 67", "boundary_conditions_xyz 0 0 1"])
