@@ -61,6 +61,17 @@ static int validate(const int_param_struct* ip, const flp_param_struct* fp, cons
         if (opt->block[0] > 0 && (opt->block[0] * (opt->block[1] > 0 ? opt->block[1] : 1) > 256 ||
                                    (opt->block[0] * (opt->block[1] > 0 ? opt->block[1] : 1)) % 64 != 0))
             return fail(D3Q19_E_INVALID, "block must hold a multiple of 64 and at most 256 threads");
+        if (opt->contact_detection != D3Q19_CONTACT_ZONES && opt->contact_detection != D3Q19_CONTACT_ALL_PAIRS)
+            return fail(D3Q19_E_INVALID, "unknown contact detection");
+        if (opt->contact_detection == D3Q19_CONTACT_ZONES && ip->NumParticles > 0) {
+            long long nzt = 1;
+            for (int d = 0; d < 3; d++) {
+                if (ip->NumZones[d] < 1 || !(fp->ZoneWidth[d] > 0.f))
+                    return fail(D3Q19_E_INVALID, "contact zones need NumZones >= 1 and ZoneWidth > 0 (D3Q19-OpenCL.c:366-375)");
+                nzt *= ip->NumZones[d];
+            }
+            if (nzt >= (1LL << 30)) return fail(D3Q19_E_UNSUPPORTED, "more than 2^30 contact zones");
+        }
     }
     return 0;
 }
@@ -121,7 +132,7 @@ static void* nxt_f(Sim* s) { return s->parity == 0 ? s->fB : s->fA; }
 void sim_release(Sim* s) {
     void* ptrs[] = {s->fA, s->fB, s->u, s->tau, s->facc, s->mask[0], s->mask[1], s->mask[2], s->bcraw, s->pos, s->vel,
                     s->pos_alt, s->vel_alt, s->quat, s->omega, s->ffluid, s->tfluid, s->fcont, s->tcont, s->pts, s->mforce,
-                    s->red, s->d_desc, s->d_obs, s->d_flag};
+                    s->red, s->d_desc, s->d_obs, s->d_flag, s->zone};
     for (void* p : ptrs)
         if (p) {
             if (s->arena) s->arena->release(p);
@@ -278,15 +289,19 @@ static int build_graph(Sim* s) {
     // capture enqueued nothing; the two flips of the lattice and of the position buffers cancel
     s->graph_f = cur_f(s);
     s->graph_pos = s->pos;
+    s->graph_tofs = s->tofs;
     return 0;
 }
 
 // n steps: the prime marks if needed, a first fused step without a pending post, graph-replayed
 // pairs, the remaining steps (the last one carries flags), then the last step's force sums and
 // contacts, so that the state after the call is the reference's state after n steps.
-static int run_steps(Sim* s, int n, int flags) {
+static int run_steps(Sim* s, int t_first, int n, int flags) {
     if (n <= 0) return 0;
     int rc;
+    // the contacts of step t read the zones of generation (t - 1) / rebuild_freq
+    s->tofs = (int)((long long)t_first - (s->h_epoch + 1));
+    s->h_epoch += n;
     if (s->np > 0 && !s->marks_next) {
         if ((rc = launch_sphere_prime(s, s->stream))) return rc;
         s->marks_next = true;
@@ -296,7 +311,7 @@ static int run_steps(Sim* s, int n, int flags) {
     const int tail = (flags && n > 1) ? 1 : 0;
     const int pairs = (n - 1 - tail) / 2;
     if (pairs > 0) {
-        if (!s->graph2 || s->graph_f != cur_f(s) || s->graph_pos != (const void*)s->pos) {
+        if (!s->graph2 || s->graph_f != cur_f(s) || s->graph_pos != (const void*)s->pos || s->graph_tofs != s->tofs) {
             if ((rc = build_graph(s))) return rc;
         }
         for (int i = 0; i < pairs; i++) HIP_OK(hipGraphLaunch(s->graph2, s->stream));
@@ -374,6 +389,7 @@ static void alloc_list(Sim* s, std::vector<AllocItem>& v) {
         v.push_back(item(&s->tcont, 3 * npd * 8));
         v.push_back(item(&s->pts, 4 * (size_t)s->ppp * sizeof(float)));
         v.push_back(item(&s->mforce, 6 * npd * (size_t)s->ppp * 8));
+        v.push_back(item(&s->zone, 2 * npd * sizeof(int)));
     }
 }
 
@@ -456,6 +472,7 @@ int sim_init(Sim* s, int device, const int_param_struct* ip, const flp_param_str
             for (int p = 0; p < s->np; p++) q[4 * p + 3] = 1.0;
             e = hipMemcpyAsync(s->quat, q.data(), q.size() * 8, hipMemcpyHostToDevice, s->stream);
             if (e == hipSuccess) e = hipMemcpyAsync(s->pts, sphere_points, 4 * (size_t)s->ppp * sizeof(float), hipMemcpyHostToDevice, s->stream);
+            if (e == hipSuccess && launch_zone_init(s, s->stream)) e = hipErrorLaunchFailure;
             if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
         }
     }
@@ -701,6 +718,9 @@ int d3q19_write_particles(d3q19_sim* s, const d3q19_float4* pk, const d3q19_floa
         HIP_OK(hipMemcpy(s->vel, b.data(), b.size() * 8, hipMemcpyHostToDevice));
         HIP_OK(hipMemcpy(s->quat, c.data(), c.size() * 8, hipMemcpyHostToDevice));
         HIP_OK(hipMemcpy(s->omega, d.data(), d.size() * 8, hipMemcpyHostToDevice));
+        // the zone lists start from the uploaded positions (initialize_particle_zones)
+        const int rc = launch_zone_init(s, s->stream);
+        if (rc) return rc;
     }
     if (pf) {
         for (int p = 0; p < np; p++) {
@@ -795,6 +815,7 @@ int d3q19_particle_fluid_forces(d3q19_sim* s) {
     CHECK_SIM(s);
     if (s->np == 0) return 0;
     s->marks_next = false;  // this pass commits an epoch
+    s->h_epoch++;
     int rc = launch_sphere_pre(s, 0, 1, s->stream);
     if (!rc) rc = launch_ibm(s, s->stream);
     if (!rc) rc = launch_sphere_post(s, 1, 0, s->stream);
@@ -808,14 +829,12 @@ int d3q19_particle_particle_forces(d3q19_sim* s) {
 
 int d3q19_step(d3q19_sim* s, int t, int flags) {
     CHECK_SIM(s);
-    (void)t;
-    return run_steps(s, 1, flags);
+    return run_steps(s, t, 1, flags);
 }
 
 int d3q19_run(d3q19_sim* s, int t_first, int n, int flags) {
     CHECK_SIM(s);
-    (void)t_first;
-    return run_steps(s, n, flags);
+    return run_steps(s, t_first, n, flags);
 }
 
 int d3q19_sync(d3q19_sim* s) {

@@ -62,6 +62,7 @@ struct Batch {
     int max_cells = 1024;
     std::vector<Member> cells;  // index = cell id
     int phase = 0;
+    long long h_epoch = 0;      // batch steps run (one epoch each)
     // device work lists
     CellDesc* d_cells = nullptr;
     uint2* d_tiles = nullptr;
@@ -172,14 +173,31 @@ static int upload_tables(Batch* b) {
             for (int base = 0; base < s->g.nint; base += tile) b->h_tiles.push_back(make_uint2((unsigned)c, (unsigned)base));
         }
     }
+    int max_ppp = 0;
+    long long nmark = 0;
     for (int c = 0; c < b->max_cells; c++) {
         const Sim* s = b->cells[c].sim;
         if (!s) continue;
         for (int p = 0; p < s->np; p++) {
-            const int sidx = (int)b->h_sph.size();
-            b->h_sph.push_back(make_int4(c, p, (int)b->h_msph.size(), s->ppp));
-            for (int k = 0; k < s->ppp; k++) b->h_msph.push_back(sidx);
+            b->h_sph.push_back(make_int4(c, p, (int)nmark, s->ppp));
+            nmark += s->ppp;
         }
+        max_ppp = std::max(max_ppp, s->ppp);
+    }
+    if (b->h_sph.size() >= (1u << 19) || nmark >= (1LL << 31))
+        return bfail(D3Q19_E_CAPACITY, "batch holds more than 2^19 spheres or 2^31 markers");
+    // marker list, (sphere << 12) | marker: round k takes marker (k * stride) mod ppp of every sphere
+    // with more than k markers, so wavefronts that run together spread into different spheres'
+    // nodes (the spheres of a batch rarely overlap) rather than into one sphere's shared nodes
+    {
+        std::vector<int> stride(b->h_sph.size());
+        for (size_t i = 0; i < b->h_sph.size(); i++) stride[i] = marker_stride(b->h_sph[i].w);
+        b->h_msph.reserve((size_t)nmark);
+        for (int k = 0; k < max_ppp; k++)
+            for (size_t i = 0; i < b->h_sph.size(); i++) {
+                const int ppp = b->h_sph[i].w;
+                if (k < ppp) b->h_msph.push_back(((int)i << 12) | (int)(((long long)k * stride[i]) % ppp));
+            }
     }
     (void)nslots;
     b->h_cnt.ntiles = (int)b->h_tiles.size();
@@ -262,6 +280,7 @@ static int batch_run(Batch* b, int n, int flags) {
     int rc;
     if (b->dirty && (rc = upload_tables(b))) return rc;
     if (b->h_cnt.ncells == 0) return 0;
+    b->h_epoch += n;
     const int wu = (flags & D3Q19_STEP_WRITE_U) ? 1 : 0;
     const BatchLaunch L0 = launch_of(b);
     if ((rc = launch_sphere_prime_batch(L0, b->phase, b->stream))) return rc;
@@ -415,6 +434,9 @@ static int batch_add(Batch* b, const int_param_struct* ip, const flp_param_struc
     m.sub.free_[0] = bytes;
     d3q19_sim* s = new d3q19_sim();
     rc = sim_init(s, b->device, ip, fp, &o, pts, &m.sub, b->stream, b->epochs);
+    // a member's own step numbers start at 1 with the batch's next step
+    s->h_epoch = b->h_epoch;
+    s->tofs = (int)-b->h_epoch;
     if (!rc && kin && s->np > 0) rc = d3q19_write_particles(s, kin, nullptr, nullptr);
     if (rc) {
         sim_release(s);

@@ -153,10 +153,12 @@ __global__ void __launch_bounds__(64 * IBM_WAVES) ibm_marker_kernel(IbmArgs a) {
         a.epochs[0] = e;
         a.epochs[1] = e;
     }
-    const int p = gid / a.ppp;
+    // wavefronts that run together take markers of different spheres, and a sphere's markers are
+    // visited with a stride of about 0.618 ppp, so concurrent atomics rarely meet on one node
+    const int p = gid % a.np, k = gid / a.np;
+    const int m = (int)(((long long)k * a.mstride) % a.ppp);
     const long long nint = a.g.nint;
-    ibm_marker<R, MODE>(a, p, gid - p * a.ppp, lane,
-                        [&](int node, double w, const double* F) { spread_add(a.facc, nint, node, w, F); });
+    ibm_marker<R, MODE>(a, p, m, lane, [&](int node, double w, const double* F) { spread_add(a.facc, nint, node, w, F); });
 }
 
 // Batched: a persistent grid of wavefronts walks the marker list of all cells; the batch's one
@@ -171,21 +173,36 @@ __global__ void __launch_bounds__(64 * IBM_WAVES) ibm_batch_kernel(BatchTables t
     }
     const int nmark = t.cnt->nmark;
     for (int gid = blockIdx.x * IBM_WAVES + (int)(threadIdx.x >> 6); gid < nmark; gid += gridDim.x * IBM_WAVES) {
-        const int sph = t.msph[gid];
-        const int4 d = t.sph[sph];  // cell, local sphere, first global marker, markers per sphere
+        const int v = t.msph[gid];
+        const int4 d = t.sph[v >> 12];  // cell, local sphere, first global marker, markers per sphere
         const CellDesc& cd = t.cells[d.x];
         IbmArgs ia = cd.i;
         ia.pos = cd.posb[cd.par ^ sp ^ 1];  // x(t+1): written by this step's turn
         ia.vel = cd.velb[cd.par ^ sp ^ 1];
         const long long nint = ia.g.nint;
         D3Q19_GLOBAL long long* facc = ia.facc;
-        ibm_marker<R, MODE>(ia, d.y, gid - d.z, lane,
-                            [&](int node, double w, const double* F) { spread_add(facc, nint, node, w, F); });
+        ibm_marker<R, MODE>(ia, d.y, v & 4095, lane, [&](int node, double w, const double* F) { spread_add(facc, nint, node, w, F); });
     }
+}
+
+int marker_stride(int ppp) {
+    if (ppp <= 2) return 1;
+    int st = (int)(0.618034 * ppp + 0.5);
+    auto gcd = [](int x, int y) {
+        while (y) {
+            const int t = x % y;
+            x = y;
+            y = t;
+        }
+        return x;
+    };
+    while (gcd(st, ppp) != 1) st++;
+    return st;
 }
 
 IbmArgs make_ibm_args(Sim* s) {
     IbmArgs a;
+    a.mstride = marker_stride(s->ppp > 0 ? s->ppp : 1);
     a.g = s->g;
     a.np = s->np;
     a.ppp = s->ppp;

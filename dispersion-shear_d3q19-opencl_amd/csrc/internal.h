@@ -158,6 +158,7 @@ struct CollideArgs {
 struct IbmArgs {
     Geom g;
     int np, ppp, mode;
+    int mstride;  // marker visiting order: wavefront k of sphere p takes marker (k * mstride) mod ppp
     double dA;
     const D3Q19_GLOBAL double* pos;
     const D3Q19_GLOBAL double* vel;
@@ -198,6 +199,15 @@ struct PartArgs {
     D3Q19_GLOBAL uint32_t* mask2;
     const D3Q19_GLOBAL uint32_t* epochs;
     D3Q19_GLOBAL int* flag;         // set when a sphere moves a lattice unit or more in one step
+    // contact pairs through the reference's zone lists (D3Q19_CONTACT_ZONES): zone of each sphere
+    // at the last rebuild, two generations (rebuild count mod 2) of np entries; the contacts of
+    // step t = epochs[0] + tofs read generation (t - 1) / rebuild
+    int zone_mode;
+    int rebuild;
+    int tofs;
+    int nz[3];
+    float zw[3];
+    D3Q19_GLOBAL int* zone;
 };
 
 
@@ -241,7 +251,7 @@ struct BatchTables {
     const CellDesc* cells;
     const uint2* tiles;      // collide work list: (cell, first node); an entry covers span tiles
     const int4* sph;         // (cell, local sphere, first global marker, markers per sphere)
-    const int* msph;         // global marker -> global sphere
+    const int* msph;         // marker list: (global sphere << 12) | marker index within the sphere
     const BatchCounts* cnt;
     uint32_t* epochs;        // the batch's epoch counter (shared by all its cells)
 };
@@ -291,6 +301,9 @@ struct Sim {
     int* d_flag = nullptr;     // displacement watchdog of the dilated marks
     bool pending_post = false; // the last step's force sums and contacts are not done yet
     bool marks_next = false;   // marks of the next epoch exist (made from the current positions)
+    int* zone = nullptr;       // 2*np zone of each sphere (contact zones), by rebuild generation
+    long long h_epoch = 0;     // host count of committed epochs (one per step)
+    int tofs = 0;              // step number t of the contacts = committed epoch + tofs
     double* quat = nullptr;    // 4*np
     double* omega = nullptr;   // 3*np
     double* ffluid = nullptr;  // 3*np
@@ -304,6 +317,7 @@ struct Sim {
     hipGraphExec_t graph2 = nullptr;   // two steps (ping-pong closes)
     const void* graph_f = nullptr;     // state the graph was captured from
     const void* graph_pos = nullptr;
+    int graph_tofs = 0;
     int parity = 0;            // 0: current post-collision field in fA
     int block[3] = {64, 4, 1};
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -330,6 +344,7 @@ void derive_observables(const Sim* s, const double* raw, d3q19_observables* out)
 // ---- argument builders and launchers (defined in the .hip files) ---------------------------------
 CollideArgs make_collide_args(Sim* s, const void* fin, void* fout, int write_u_all, int use_maskU);
 IbmArgs make_ibm_args(Sim* s);
+int marker_stride(int ppp);  // coprime with ppp, about 0.618 ppp
 PartArgs make_part_args(Sim* s);
 CellDesc make_cell_desc(Sim* s);
 int launch_collide(Sim* s, const void* fin, void* fout, int write_u_all, int use_maskU, hipStream_t st);
@@ -344,6 +359,7 @@ int launch_check_finite(Sim* s, const void* fsrc, int* dev_flag);
 int launch_ibm(Sim* s, hipStream_t st);
 int launch_sphere_pre(Sim* s, int do_dyn, int do_mark, hipStream_t st);       // dynamics, brick marks
 int launch_sphere_post(Sim* s, int do_reduce, int do_contacts, hipStream_t st); // force sums, contacts
+int launch_zone_init(Sim* s, hipStream_t st);  // zones of the current positions, both generations
 int launch_sphere_prime(Sim* s, hipStream_t st);                                // dilated marks, next epoch
 // fused step: the spheres' turn (post of the last step if pending, dynamics into the other
 // position buffer, dilated marks of the epoch after this one) in np extra blocks of the collide

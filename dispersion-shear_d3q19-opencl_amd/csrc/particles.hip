@@ -14,6 +14,14 @@ __global__ void __launch_bounds__(256) sphere_post_kernel(PartArgs a, int do_red
     sphere_post(a, blockIdx.x, do_reduce, do_contacts);
 }
 
+__global__ void __launch_bounds__(256) zone_init_kernel(PartArgs a) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= a.np) return;
+    const int z = zone_of(a, p);
+    a.zone[p] = z;
+    a.zone[a.np + p] = z;
+}
+
 __global__ void __launch_bounds__(256) sphere_prime_kernel(PartArgs a) {
     sphere_prime(a, blockIdx.x, a.epochs[0] + 1u);
 }
@@ -101,7 +109,22 @@ PartArgs make_part_args(Sim* s) {
     a.mask2 = gp(s->mask[2]);
     a.flag = gp(s->d_flag);
     a.epochs = gp(s->epochs);
+    a.zone_mode = s->opt.contact_detection == D3Q19_CONTACT_ZONES ? 1 : 0;
+    a.rebuild = s->opt.rebuild_freq > 0 ? s->opt.rebuild_freq : 10;
+    a.tofs = s->tofs;
+    for (int d = 0; d < 3; d++) {
+        a.nz[d] = s->ip.NumZones[d];
+        a.zw[d] = s->fp.ZoneWidth[d];
+    }
+    a.zone = gp(s->zone);
     return a;
+}
+
+int launch_zone_init(Sim* s, hipStream_t st) {
+    if (s->np == 0 || s->opt.contact_detection != D3Q19_CONTACT_ZONES) return 0;
+    hipLaunchKernelGGL(zone_init_kernel, dim3((s->np + 255) / 256), dim3(256), 0, st, make_part_args(s));
+    HIP_OK(hipGetLastError());
+    return 0;
 }
 
 int launch_sphere_pre(Sim* s, int do_dyn, int do_mark, hipStream_t st) {

@@ -83,6 +83,49 @@ __device__ void mark_sphere(const double* c3, const Geom& g, double radius, MP m
     }
 }
 
+// ---- contact zones ----------------------------------------------------------------------------------
+// Zone of sphere p from its current position: (int)(x / ZoneWidth) per axis in float, as the
+// reference computes it on its float positions (D3Q19-OpenCL.c:401-404, CPU_program.cl:299-302);
+// an id outside the zone grid (out of bounds in the reference) is -1 and meets no sphere.
+__device__ __forceinline__ int zone_of(const PartArgs& a, int p) {
+    int id[3];
+    for (int d = 0; d < 3; d++) id[d] = (int)((float)a.pos[3 * p + d] / a.zw[d]);
+    const long long z = id[0] + (long long)a.nz[0] * (id[1] + (long long)a.nz[1] * id[2]);
+    return (z >= 0 && z < (long long)a.nz[0] * a.nz[1] * a.nz[2]) ? (int)z : -1;
+}
+
+// Number of times zone zj appears in the neighbour list of zone zi as the reference builds it
+// (D3Q19-OpenCL.c:419-462): offsets -1..1 per axis, none past a velocity-boundary face, indices
+// wrapped (s < 0 -> NumZones - 1, else s % NumZones), the id formed with NumZones[2] where
+// NumZones[1] is meant (line 450).  particle_particle_forces visits pair (pi, pj), pi > pj, once
+// per appearance of pj's zone in pi's list (CPU_program.cl:106-121).
+__device__ __forceinline__ int zone_multiplicity(const PartArgs& a, int zi, int zj) {
+    if (zi < 0 || zj < 0) return 0;
+    const int n0 = a.nz[0], n1 = a.nz[1], n2 = a.nz[2];
+    const int c[3] = {zi % n0, (zi / n0) % n1, zi / (n0 * n1)};
+    const int nd[3] = {n0, n1, n2};
+    int lo[3], hi[3];
+    for (int d = 0; d < 3; d++) {
+        lo[d] = (!a.g.per[d] && c[d] == 0) ? 0 : -1;
+        hi[d] = (!a.g.per[d] && c[d] == nd[d] - 1) ? 0 : 1;
+    }
+    int cnt = 0;
+    for (int n = lo[2]; n <= hi[2]; n++)
+        for (int m = lo[1]; m <= hi[1]; m++)
+            for (int l = lo[0]; l <= hi[0]; l++) {
+                const int ls = c[0] + l, ms = c[1] + m, ns = c[2] + n;
+                const int lw = ls < 0 ? n0 - 1 : ls % n0;
+                const int mw = ms < 0 ? n1 - 1 : ms % n1;
+                const int nw = ns < 0 ? n2 - 1 : ns % n2;
+                cnt += (lw + n0 * (mw + n2 * nw) == zj) ? 1 : 0;
+            }
+    return cnt;
+}
+
+// generation of the zone lists that the contacts of step t read: the lists of the rebuild after
+// step rebuild * ((t - 1) / rebuild), or the initial lists
+__device__ __forceinline__ int zone_generation(const PartArgs& a, long long t) { return t >= 1 ? (int)(((t - 1) / a.rebuild) & 1) : 0; }
+
 // reads pos/vel, writes pos_out/vel_out (the same arrays except in the fused turn)
 __device__ __forceinline__ void dynamics_one(const PartArgs& a, int p) {
     double acc[3], aacc[3];
@@ -165,6 +208,10 @@ __device__ __forceinline__ void sphere_post(const PartArgs& a, int p, int do_red
             }
     }
     if (!do_contacts) return;
+    // step number of these contacts: the step whose spreading committed the current epoch
+    const long long t = (long long)a.epochs[0] + a.tofs;
+    const int zgen = zone_generation(a, t);
+    const int zp = a.zone_mode ? a.zone[zgen * a.np + p] : 0;
     const double rp = 0.5 * a.diam;
     const double xi[3] = {a.pos[3 * p], a.pos[3 * p + 1], a.pos[3 * p + 2]};
     const double vi[3] = {a.vel[3 * p], a.vel[3 * p + 1], a.vel[3 * p + 2]};
@@ -185,6 +232,12 @@ __device__ __forceinline__ void sphere_post(const PartArgs& a, int p, int do_red
         const double rsep = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
         const double minsep = rsep - 2.0 * rp;
         if (minsep >= 0.1 * rp) continue;  // neither overlap nor squeeze range
+        double mult = 1.0;
+        if (a.zone_mode) {
+            const int zj = a.zone[zgen * a.np + j];
+            mult = (double)(ipi ? zone_multiplicity(a, zp, zj) : zone_multiplicity(a, zj, zp));
+            if (mult == 0.0) continue;
+        }
         const double e[3] = {r[0] / rsep, r[1] / rsep, r[2] / rsep};
         const double vrel = -(e[0] * v[0] + e[1] * v[1] + e[2] * v[2]);
         double fmag = 0.0;
@@ -197,7 +250,7 @@ __device__ __forceinline__ void sphere_post(const PartArgs& a, int p, int do_red
         }
         // the reference subtracts e*f from pi and adds it to pj
         const double sgn = ipi ? -1.0 : 1.0;
-        for (int d = 0; d < 3; d++) F[d] += sgn * e[d] * fmag;
+        for (int d = 0; d < 3; d++) F[d] += mult * (sgn * e[d] * fmag);
     }
     block_sums<3>(F, sh3);
     if (threadIdx.x == 0) {
@@ -207,6 +260,9 @@ __device__ __forceinline__ void sphere_post(const PartArgs& a, int p, int do_red
         const double hi = xi[2] - (a.sys[2] - a.zbuf);
         if (hi > 0) F[2] -= a.kcol * hi;
         for (int d = 0; d < 3; d++) a.fcont[3 * p + d] += F[d];
+        // zone lists rebuilt after the contacts of steps t % RebuildFreq == 0 (sim_main.c:428-444),
+        // into the other generation, which no contact of this step reads
+        if (a.zone_mode && t >= 1 && t % a.rebuild == 0) a.zone[(int)((t / a.rebuild) & 1) * a.np + p] = zone_of(a, p);
     }
 }
 

@@ -68,7 +68,9 @@ class Options(C.Structure):
         ("calc_rho", C.c_int32),
         ("tangential_bc", C.c_int32 * 3),
         ("block", C.c_int32 * 3),
-        ("reserved", C.c_int32 * 6),
+        ("contact_detection", C.c_int32),
+        ("rebuild_freq", C.c_int32),
+        ("reserved", C.c_int32 * 4),
     ]
 
 
@@ -98,6 +100,7 @@ class HostParams(C.Structure):
         ("IbmKernel", C.c_int),
         ("Storage", C.c_int),
         ("MassModel", C.c_int),
+        ("ContactDetection", C.c_int),
         ("DataDir", C.c_char * 256),
     ]
 
@@ -142,6 +145,7 @@ class Result(C.Structure):
 COLL_MRT, COLL_BGK = 0, 1
 IBM_LINEAR, IBM_PESKIN4 = 0, 1
 STORE_FP32, STORE_FP16, STORE_BF16, STORE_FP64 = 0, 1, 2, 3
+CONTACT_ZONES, CONTACT_ALL_PAIRS = 0, 1
 STEP_WRITE_U = 1
 E_NONFINITE = -6
 

@@ -14,8 +14,9 @@ import numpy as np
 def deck(size, walls=(0, 0, 1), tau=0.8, uw=0.01, np_=0, diam=8.0, markers=0, dist=4, seed=1, gap=2.0,
          coll=1, ibm=1, visc=1, vparams=(0, 0, 0, 0), local=0, clamp=(0.0, 0.0), storage=0, zbuf=None,
          body=(0.0, 0.0, 0.0), buffer=None, kcol=1.0, mass_model=0, init="zero", u0=(0.0, 0.0, 0.0), tangential=(0, 0, 0),
-         iterations=100):
-    """Lines of an input file.  size = interior box edge(s) (SystemSize for periodic axes)."""
+         iterations=100, contacts=1, rebuild=10):
+    """Lines of an input file.  size = interior box edge(s) (SystemSize for periodic axes).
+    contacts: 1 all sphere pairs (the benchmark workloads), 0 the reference's neighbour-zone lists."""
     if np.isscalar(size):
         size = (size, size, size)
     N = [size[d] + 2 + (1 if walls[d] else 0) for d in range(3)]
@@ -45,6 +46,8 @@ def deck(size, walls=(0, 0, 1), tau=0.8, uw=0.01, np_=0, diam=8.0, markers=0, di
         "mass_model %d" % mass_model,
         "initial_f %s" % init,
         "initial_vel %r %r %r" % tuple(u0),
+        "contact_detection %d" % contacts,
+        "rebuild_neigh_list_freq %d" % rebuild,
     ]
     if zbuf is not None:
         lines.append("z_wall_particle_buffer %r" % zbuf)

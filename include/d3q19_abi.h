@@ -112,8 +112,22 @@ typedef struct {
     int32_t calc_rho;           /* velocity BC density mode, sim_main.c:216 (reference: 0) */
     int32_t tangential_bc[3];   /* hostDat.TangentialVelBC, sim_main.c:388-404 */
     int32_t block[3];           /* collide+stream tile (threads); 0 = library default / autotuned */
-    int32_t reserved[6];
+    int32_t contact_detection;  /* D3Q19_CONTACT_*: which sphere pairs the contact forces visit */
+    int32_t rebuild_freq;       /* hostDat.RebuildFreq: zone lists rebuilt after steps t with
+                                   t % rebuild_freq == 0 (sim_main.c:429-444); <= 0 selects 10 */
+    int32_t reserved[4];
 } d3q19_options;
+
+/* Contact pair enumeration (particle_particle_forces, CPU_program.cl:76-184).
+ * ZONES: the reference's neighbour-zone lists, quirks included (D3Q19-OpenCL.c:344-463,
+ *   CPU_program.cl:280-315): a sphere's zone is (int)(x / ZoneWidth) per axis from the last
+ *   rebuild (int_param_struct.NumZones, flp_param_struct.ZoneWidth), a zone's neighbour ids
+ *   are formed with NumZones[2] where NumZones[1] is meant (D3Q19-OpenCL.c:450), and a
+ *   neighbour zone reached twice (NumZones < 3 on a periodic axis) counts its pairs twice.
+ *   Neighbour ids that fall outside the zone grid match no sphere.
+ * ALL_PAIRS: every pair once. */
+#define D3Q19_CONTACT_ZONES     0
+#define D3Q19_CONTACT_ALL_PAIRS 1
 
 /* Error codes */
 #define D3Q19_OK               0

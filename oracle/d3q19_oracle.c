@@ -53,6 +53,13 @@ struct orc_sim {
     real* ffsum;      /* per-marker partials for the work-group tree reduction, 2*np*ppp*4 */
     int use_s_override;
     real s_override[19];
+    /* contact zones (D3Q19-OpenCL.c:344-463): step counter and the zone lists */
+    int t;            /* steps completed */
+    int nzt;          /* total zones */
+    int* pars_zone;   /* zone of each sphere, np */
+    int* zone_members;/* nzt * np */
+    int* num_in_zone; /* nzt */
+    int* zone_neigh;  /* 28 per zone: count, then neighbour ids */
 };
 typedef struct orc_sim orc_sim;
 
@@ -617,48 +624,150 @@ void ORC(particle_dynamics)(orc_sim* s) {
     }
 }
 
-/* ---- contacts ----------------------------------------------------------------------------------
- * particle_particle_forces + compute_squeeze_force (CPU_program.cl:76-184, 319-343), visiting
- * every pair pi > pj directly instead of through the zone lists. */
-void ORC(particle_forces)(orc_sim* s) {
+/* ---- contact zones ------------------------------------------------------------------------------
+ * The zone of a sphere: (int)(x / ZoneWidth) per axis in float, as parKin is float in the
+ * reference (D3Q19-OpenCL.c:401-404, CPU_program.cl:299-302). */
+static int zone_of(const orc_sim* s, int p) {
+    int id[3];
+    for (int d = 0; d < 3; d++) id[d] = (int)((float)PK(s, p, 0, d) / s->fp.ZoneWidth[d]);
+    return id[0] + s->ip.NumZones[0] * (id[1] + s->ip.NumZones[1] * id[2]);
+}
+
+/* Membership lists from pars_zone, in sphere order: the initial assignment appends p = 0..np-1
+ * (D3Q19-OpenCL.c:383-417); the rebuild runs the reference's threads, each over its own spheres
+ * in increasing order, one after another (CPU_program.cl:280-315, counts reset on the host,
+ * sim_main.c:429-444).  A zone id outside the grid is out of bounds in the reference (an
+ * error at initialisation, an unchecked write at a rebuild); here the sphere joins no zone. */
+static void assign_zones(orc_sim* s) {
     const int np = s->np;
+    memset(s->num_in_zone, 0, sizeof(int) * (size_t)s->nzt);
+    for (int p = 0; p < np; p++) {
+        const int z = zone_of(s, p);
+        s->pars_zone[p] = z;
+        if (z < 0 || z >= s->nzt) continue;
+        s->zone_members[(size_t)z * np + s->num_in_zone[z]++] = p;
+    }
+}
+
+/* Neighbour lists, D3Q19-OpenCL.c:419-462: offsets -1..1 per axis, clipped at the zone grid's
+ * faces on type-1 (velocity) boundaries, wrapped elsewhere (ls < 0 -> NumZones - 1, else
+ * ls % NumZones), the neighbour id formed as lw + NumZones[0]*(mw + NumZones[2]*nw) -- NumZones[2]
+ * where NumZones[1] is meant (line 450) -- and a zone reached twice listed twice. */
+static void build_zone_neighbours(orc_sim* s) {
+    const int* NZ = s->ip.NumZones;
+    const int* BC = s->ip.BoundaryConds;
+    for (int k = 0; k < NZ[2]; k++)
+        for (int j = 0; j < NZ[1]; j++)
+            for (int i = 0; i < NZ[0]; i++) {
+                const int zone = i + NZ[0] * (j + NZ[1] * k);
+                int* list = s->zone_neigh + 28 * (size_t)zone;
+                list[0] = 0;
+                int lm = -1, lp = 1, mm = -1, mp = 1, nm = -1, npp = 1;
+                if (BC[0] == 1 && i == 0) lm = 0;
+                if (BC[1] == 1 && j == 0) mm = 0;
+                if (BC[2] == 1 && k == 0) nm = 0;
+                if (BC[0] == 1 && i == NZ[0] - 1) lp = 0;
+                if (BC[1] == 1 && j == NZ[1] - 1) mp = 0;
+                if (BC[2] == 1 && k == NZ[2] - 1) npp = 0;
+                for (int n = nm; n <= npp; n++)
+                    for (int m = mm; m <= mp; m++)
+                        for (int l = lm; l <= lp; l++) {
+                            const int ls = i + l, ms = j + m, ns = k + n;
+                            const int lw = ls < 0 ? NZ[0] - 1 : ls % NZ[0];
+                            const int mw = ms < 0 ? NZ[1] - 1 : ms % NZ[1];
+                            const int nw = ns < 0 ? NZ[2] - 1 : ns % NZ[2];
+                            list[++list[0]] = lw + NZ[0] * (mw + NZ[2] * nw);
+                        }
+            }
+}
+
+static void init_zones(orc_sim* s) {
+    const int* NZ = s->ip.NumZones;
+    s->nzt = NZ[0] * NZ[1] * NZ[2];
+    s->pars_zone = (int*)calloc((size_t)s->np + 1, sizeof(int));
+    s->zone_members = (int*)calloc((size_t)s->nzt * s->np + 1, sizeof(int));
+    s->num_in_zone = (int*)calloc((size_t)s->nzt + 1, sizeof(int));
+    s->zone_neigh = (int*)calloc(28 * (size_t)s->nzt + 1, sizeof(int));
+    assign_zones(s);
+    build_zone_neighbours(s);
+}
+
+static int rebuild_freq(const orc_sim* s) { return s->opt.rebuild_freq > 0 ? s->opt.rebuild_freq : 10; }
+
+/* ---- contacts ----------------------------------------------------------------------------------
+ * particle_particle_forces + compute_squeeze_force (CPU_program.cl:76-184, 319-343).  With
+ * D3Q19_CONTACT_ZONES every sphere pi visits the members pj < pi of the zones in its zone's
+ * neighbour list, as the reference does; with D3Q19_CONTACT_ALL_PAIRS every pair pi > pj once. */
+static void pair_force(orc_sim* s, int pi, int pj) {
     real* pf = s->arr[ORC_PARFORCE];
     real rp = (real)s->fp.ParticleDiam / (real)2.0;
     real w[3] = {(real)s->ip.SystemSize[0], (real)s->ip.SystemSize[1], (real)s->ip.SystemSize[2]};
     real k = (real)s->fp.ParForceParams[0];
+    real rij[3], vij[3];
+    for (int d = 0; d < 3; d++) {
+        rij[d] = PK(s, pj, 0, d) - PK(s, pi, 0, d);
+        int sg = (rij[d] < 0) ? -1 : (rij[d] > 0);
+        rij[d] = (r_fabs(rij[d]) < w[d] / (real)2.0) ? rij[d] : (rij[d] - (real)sg * w[d]);
+        vij[d] = PK(s, pj, 1, d) - PK(s, pi, 1, d);
+    }
+    real rsep = r_sqrt(rij[0] * rij[0] + rij[1] * rij[1] + rij[2] * rij[2]);
+    real minsep = rsep - 2 * rp;
+    real e[3] = {rij[0] / rsep, rij[1] / rsep, rij[2] / rsep};
+    real vrel = -(e[0] * vij[0] + e[1] * vij[1] + e[2] * vij[2]);
+    real overlap = -minsep;
+    if (overlap > 0 && s->ip.ParForceModel == 1) {
+        real fm = k * overlap;
+        for (int d = 0; d < 3; d++) { pf[4 * pi + d] -= e[d] * fm; pf[4 * pj + d] += e[d] * fm; }
+    }
+    if (minsep > 0 && minsep < (real)0.1 * rp) {
+        real sq = 0;
+        if (s->ip.ViscosityModel == D3Q19_VISC_NEWTONIAN) {
+            real rstar = rp / (real)2.0;
+            real ms = minsep < (real)0.1 ? (real)0.1 : minsep;
+            real eta0 = ((real)s->fp.NewtonianTau - (real)0.5) / (real)3.0;
+            sq = (real)(6.0 * 3.14159265358979323846) * eta0 * rstar * rstar * vrel / ms;
+        }
+        for (int d = 0; d < 3; d++) { pf[4 * pi + d] -= e[d] * sq; pf[4 * pj + d] += e[d] * sq; }
+    }
+}
+
+void ORC(particle_forces)(orc_sim* s) {
+    const int np = s->np;
+    real* pf = s->arr[ORC_PARFORCE];
+    real k = (real)s->fp.ParForceParams[0];
+    const int zones = s->opt.contact_detection == D3Q19_CONTACT_ZONES;
+    if (zones && !s->zone_neigh) init_zones(s);
     for (int pi = 0; pi < np; pi++) {
-        for (int pj = 0; pj < pi; pj++) {
-            real rij[3], vij[3];
-            for (int d = 0; d < 3; d++) {
-                rij[d] = PK(s, pj, 0, d) - PK(s, pi, 0, d);
-                int sg = (rij[d] < 0) ? -1 : (rij[d] > 0);
-                rij[d] = (r_fabs(rij[d]) < w[d] / (real)2.0) ? rij[d] : (rij[d] - (real)sg * w[d]);
-                vij[d] = PK(s, pj, 1, d) - PK(s, pi, 1, d);
-            }
-            real rsep = r_sqrt(rij[0] * rij[0] + rij[1] * rij[1] + rij[2] * rij[2]);
-            real minsep = rsep - 2 * rp;
-            real e[3] = {rij[0] / rsep, rij[1] / rsep, rij[2] / rsep};
-            real vrel = -(e[0] * vij[0] + e[1] * vij[1] + e[2] * vij[2]);
-            real overlap = -minsep;
-            if (overlap > 0 && s->ip.ParForceModel == 1) {
-                real fm = k * overlap;
-                for (int d = 0; d < 3; d++) { pf[4 * pi + d] -= e[d] * fm; pf[4 * pj + d] += e[d] * fm; }
-            }
-            if (minsep > 0 && minsep < (real)0.1 * rp) {
-                real sq = 0;
-                if (s->ip.ViscosityModel == D3Q19_VISC_NEWTONIAN) {
-                    real rstar = rp / (real)2.0;
-                    real ms = minsep < (real)0.1 ? (real)0.1 : minsep;
-                    real eta0 = ((real)s->fp.NewtonianTau - (real)0.5) / (real)3.0;
-                    sq = (real)(6.0 * 3.14159265358979323846) * eta0 * rstar * rstar * vrel / ms;
+        if (zones) {
+            const int pz = s->pars_zone[pi];
+            if (pz >= 0 && pz < s->nzt) {
+                const int* list = s->zone_neigh + 28 * (size_t)pz;
+                for (int a = 1; a <= list[0]; a++) {
+                    const int z = list[a];
+                    if (z < 0 || z >= s->nzt) continue;
+                    for (int j = 0; j < s->num_in_zone[z]; j++) {
+                        const int pj = s->zone_members[(size_t)z * np + j];
+                        if (pi > pj) pair_force(s, pi, pj);
+                    }
                 }
-                for (int d = 0; d < 3; d++) { pf[4 * pi + d] -= e[d] * sq; pf[4 * pj + d] += e[d] * sq; }
             }
+        } else {
+            for (int pj = 0; pj < pi; pj++) pair_force(s, pi, pj);
         }
         real lo = (real)s->fp.ParticleZBuffer - PK(s, pi, 0, 2);
         if (lo > 0) pf[4 * pi + 2] += k * lo;
         real hi = PK(s, pi, 0, 2) - ((real)s->ip.SystemSize[2] - (real)s->fp.ParticleZBuffer);
         if (hi > 0) pf[4 * pi + 2] -= k * hi;
+    }
+}
+
+/* End of step t: the zone lists are rebuilt from the current positions when t % RebuildFreq == 0
+ * (sim_main.c:428-444). */
+void ORC(end_step)(orc_sim* s) {
+    s->t++;
+    if (s->np > 0 && s->opt.contact_detection == D3Q19_CONTACT_ZONES && s->t % rebuild_freq(s) == 0) {
+        if (!s->zone_neigh) init_zones(s);
+        assign_zones(s);
     }
 }
 
@@ -671,6 +780,7 @@ static void swap_f(orc_sim* s) {
 /* One time step in sim_main.c:315-471 order.  particle_dynamics runs on the reference's CPU
  * queue concurrently with collide; it reads nothing collide writes. */
 void ORC(step)(orc_sim* s) {
+    if (s->np > 0 && s->opt.contact_detection == D3Q19_CONTACT_ZONES && !s->zone_neigh) init_zones(s);
     ORC(collide_stream)(s);
     if (s->np > 0) {
         ORC(particle_dynamics)(s);
@@ -689,6 +799,7 @@ void ORC(step)(orc_sim* s) {
         ORC(ibm)(s);
         ORC(particle_forces)(s);
     }
+    ORC(end_step)(s);
     swap_f(s);
 }
 
@@ -738,6 +849,10 @@ void ORC(destroy)(orc_sim* s) {
     if (!s) return;
     for (int a = 0; a < ORC_NARRAYS; a++) free(s->arr[a]);
     free(s->ffsum);
+    free(s->pars_zone);
+    free(s->zone_members);
+    free(s->num_in_zone);
+    free(s->zone_neigh);
     free(s);
 }
 
@@ -766,6 +881,20 @@ void ORC(guo)(const double* u, const double* g, double* out) {
 }
 double ORC(tau_of)(orc_sim* s, double srt) { return (double)compute_tau(s, (real)srt); }
 double ORC(peskin)(double r) { return (double)peskin4((real)r); }
+/* zone lists from the current positions, copied out (arrays of np, nzt, nzt*np and 28*nzt ints) */
+int ORC(zone_lists)(orc_sim* s, int* pars_zone, int* num_in_zone, int* members, int* neigh) {
+    if (s->np == 0) return 0;
+    if (!s->zone_neigh) init_zones(s);
+    else assign_zones(s);
+    memcpy(pars_zone, s->pars_zone, sizeof(int) * (size_t)s->np);
+    memcpy(num_in_zone, s->num_in_zone, sizeof(int) * (size_t)s->nzt);
+    memcpy(members, s->zone_members, sizeof(int) * (size_t)s->nzt * s->np);
+    memcpy(neigh, s->zone_neigh, sizeof(int) * 28 * (size_t)s->nzt);
+    return s->nzt;
+}
+int ORC(steps_done)(orc_sim* s) { return s->t; }
+/* one pair's contact contribution added to the contact-force array (test hook) */
+void ORC(pair_force_of)(orc_sim* s, int pi, int pj) { pair_force(s, pi, pj); }
 void ORC(set_s_override)(orc_sim* s, const double* sv) {
     if (!sv) { s->use_s_override = 0; return; }
     s->use_s_override = 1;

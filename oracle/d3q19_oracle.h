@@ -13,7 +13,8 @@
  *   wall BC          GPU_program.cl:737-865, tangential passes sim_main.c:388-404
  *   IBM              GPU_program.cl:81-256 (linear), Peskin 4-point (configuration mode)
  *   dynamics         CPU_program.cl:187-277
- *   contacts         CPU_program.cl:76-184, 319-343 (all pairs instead of zone lists)
+ *   contacts         CPU_program.cl:76-184, 319-343, over the zone lists of
+ *                    D3Q19-OpenCL.c:344-463 and CPU_program.cl:280-315, or over all pairs
  *   step order       sim_main.c:315-471
  * Built twice: orc_*_f32 (float, reference arithmetic) and orc_*_f64 (double).
  */

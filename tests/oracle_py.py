@@ -49,6 +49,11 @@ def load(sfx):
     getattr(L, "orc_peskin_" + sfx).argtypes = [C.c_double]
     getattr(L, "orc_peskin_" + sfx).restype = C.c_double
     getattr(L, "orc_set_s_override_" + sfx).argtypes = [V, V]
+    getattr(L, "orc_zone_lists_" + sfx).argtypes = [V, V, V, V, V]
+    getattr(L, "orc_zone_lists_" + sfx).restype = C.c_int
+    getattr(L, "orc_steps_done_" + sfx).argtypes = [V]
+    getattr(L, "orc_steps_done_" + sfx).restype = C.c_int
+    getattr(L, "orc_pair_force_of_" + sfx).argtypes = [V, C.c_int, C.c_int]
     _libs[sfx] = L
     return L
 
@@ -93,6 +98,15 @@ class Oracle:
 
     def call(self, name, *args):
         self.fn(name)(self.h, *args)
+
+    def zone_lists(self):
+        """(pars_zone, num_in_zone, members[nzt, np], neigh[nzt, 28]) from the current positions."""
+        c = self.case
+        nzt = int(np.prod(list(c.ip.NumZones)))
+        pz, nz = np.zeros(max(c.np, 1), np.int32), np.zeros(nzt, np.int32)
+        mem, nb = np.zeros(nzt * max(c.np, 1), np.int32), np.zeros(28 * nzt, np.int32)
+        self.fn("zone_lists")(self.h, *[a.ctypes.data_as(C.c_void_p) for a in (pz, nz, mem, nb)])
+        return pz[: c.np], nz, mem.reshape(nzt, max(c.np, 1)), nb.reshape(nzt, 28)
 
     def close(self):
         if self.h:

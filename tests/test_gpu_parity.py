@@ -163,6 +163,23 @@ def test_contacts_many_spheres():
     run_pair(c, 20)
 
 
+@pytest.mark.parametrize("grid", [0, 4])
+def test_zone_contacts_fp64(grid):
+    # contacts through the reference's zone lists (zone grids (2, 4, 2) and (4, 3, 5): pairs
+    # visited 0, 1 or 2 times), lists rebuilt every 3 steps; fp64 build against the f64 oracle
+    from test_zones import GRIDS, zone_case
+    box, walls, np_, diam, _ = GRIDS[grid]
+    c = zone_case(box, walls, np_, diam, 3, storage=3)
+    run_pair(c, 13)
+
+
+def test_zone_contacts_fp32_and_kernel_level():
+    from test_zones import GRIDS, zone_case
+    box, walls, np_, diam, _ = GRIDS[0]
+    run_pair(zone_case(box, walls, np_, diam, 4), 17)
+    run_pair(zone_case(box, walls, np_, diam, 2, coll=0), 7, kernel_level=True)
+
+
 FP64_CASES = {
     "mrt-linear-ibm-walls": dict(size=(16, 14, 12), coll=0, ibm=0, np_=1, diam=6.0, markers=128, tau=0.8, uw=0.02),
     "bgk-peskin-walls": dict(size=(16, 14, 12), coll=1, ibm=1, np_=1, diam=6.0, markers=128, tau=0.8, uw=0.02),
