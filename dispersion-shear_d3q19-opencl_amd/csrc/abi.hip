@@ -7,6 +7,7 @@
 // are returned, never exit()).
 #include <stdio.h>
 #include <string.h>
+#include <algorithm>
 #include <cmath>
 #include <utility>
 #include <string>
@@ -132,7 +133,7 @@ static void* nxt_f(Sim* s) { return s->parity == 0 ? s->fB : s->fA; }
 void sim_release(Sim* s) {
     void* ptrs[] = {s->fA, s->fB, s->u, s->tau, s->facc, s->mask[0], s->mask[1], s->mask[2], s->bcraw, s->pos, s->vel,
                     s->pos_alt, s->vel_alt, s->quat, s->omega, s->ffluid, s->tfluid, s->fcont, s->tcont, s->pts, s->mforce,
-                    s->red, s->d_desc, s->d_obs, s->d_flag, s->zone};
+                    s->red, s->d_desc, s->d_obs, s->d_flag, s->zone, s->perm};
     for (void* p : ptrs)
         if (p) {
             if (s->arena) s->arena->release(p);
@@ -390,6 +391,7 @@ static void alloc_list(Sim* s, std::vector<AllocItem>& v) {
         v.push_back(item(&s->pts, 4 * (size_t)s->ppp * sizeof(float)));
         v.push_back(item(&s->mforce, 6 * npd * (size_t)s->ppp * 8));
         v.push_back(item(&s->zone, 2 * npd * sizeof(int)));
+        v.push_back(item(&s->perm, (size_t)s->ppp * sizeof(int)));
     }
 }
 
@@ -456,6 +458,36 @@ int sim_init(Sim* s, int device, const int_param_struct* ip, const flp_param_str
         free_all(s);
         return rc;
     }
+    // marker groups of the LDS-staged spreading: markers in Morton order of their offsets, so that
+    // consecutive ones are neighbours on the sphere; enabled where every periodic extent exceeds
+    // twice the stencil reach (|offset| + 3 nodes) plus a margin
+    std::vector<int> perm;
+    if (s->np > 0) {
+        double rmax = 0.0;
+        for (int m = 0; m < s->ppp; m++) {
+            const d3q19_float4 q = sphere_points[m];
+            rmax = std::max(rmax, std::sqrt((double)q.x * q.x + (double)q.y * q.y + (double)q.z * q.z));
+        }
+        std::vector<std::pair<uint64_t, int>> key((size_t)s->ppp);
+        for (int m = 0; m < s->ppp; m++) {
+            const float c[3] = {sphere_points[m].x, sphere_points[m].y, sphere_points[m].z};
+            uint64_t code = 0;
+            for (int d = 0; d < 3; d++) {
+                const double u = rmax > 0 ? (c[d] / rmax + 1.0) * 0.5 : 0.5;
+                const uint64_t qd = (uint64_t)std::min(1023.0, std::max(0.0, u * 1023.0));
+                for (int bit = 0; bit < 10; bit++) code |= ((qd >> bit) & 1ull) << (3 * bit + d);
+            }
+            key[(size_t)m] = {code, m};
+        }
+        std::stable_sort(key.begin(), key.end());
+        perm.resize((size_t)s->ppp);
+        for (int m = 0; m < s->ppp; m++) perm[(size_t)m] = key[(size_t)m].second;
+        const int need = 2 * ((int)std::ceil(rmax) + 3) + 4;
+        s->tile_ok = true;
+        const int nd[3] = {s->g.nx, s->g.ny, s->g.nz};
+        for (int d = 0; d < 3; d++)
+            if (s->g.per[d] && nd[d] < need) s->tile_ok = false;
+    }
     e = hipMemsetAsync(s->u, 0, 3 * nint * rs, s->stream);
     if (e == hipSuccess) e = hipMemsetAsync(s->d_flag, 0, sizeof(int), s->stream);
     if (e == hipSuccess && s->np > 0) {
@@ -472,6 +504,7 @@ int sim_init(Sim* s, int device, const int_param_struct* ip, const flp_param_str
             for (int p = 0; p < s->np; p++) q[4 * p + 3] = 1.0;
             e = hipMemcpyAsync(s->quat, q.data(), q.size() * 8, hipMemcpyHostToDevice, s->stream);
             if (e == hipSuccess) e = hipMemcpyAsync(s->pts, sphere_points, 4 * (size_t)s->ppp * sizeof(float), hipMemcpyHostToDevice, s->stream);
+            if (e == hipSuccess) e = hipMemcpyAsync(s->perm, perm.data(), perm.size() * sizeof(int), hipMemcpyHostToDevice, s->stream);
             if (e == hipSuccess && launch_zone_init(s, s->stream)) e = hipErrorLaunchFailure;
             if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
         }

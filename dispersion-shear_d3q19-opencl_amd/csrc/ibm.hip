@@ -23,12 +23,13 @@ __device__ __forceinline__ double peskin4(double r) {
     return 0.0;
 }
 
-// contribution -w F of one stencil node, rounded to fixed point as a whole and added with 64-bit
-// integer atomics (order independent)
+// contribution -w F[c] of one stencil node, rounded to fixed point as a whole
+__device__ __forceinline__ long long spread_value(double w, double Fc) { return __double2ll_rn((-w * Fc) * FIX_SCALE); }
+
+// ... added with 64-bit integer atomics (order independent)
 __device__ __forceinline__ void spread_add(D3Q19_GLOBAL long long* facc, long long nint, int node, double w, const double* F) {
     for (int c = 0; c < 3; c++) {
-        const double v = -w * F[c];
-        const long long iv = __double2ll_rn(v * FIX_SCALE);
+        const long long iv = spread_value(w, F[c]);
         if (iv != 0)
             __hip_atomic_fetch_add((D3Q19_GLOBAL unsigned long long*)(facc + c * nint + node), (unsigned long long)iv,
                                    __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -45,7 +46,7 @@ __device__ __forceinline__ double wave_sum(double v) {
 // marker's 4x4x4 support.  Interpolation is a wave reduction of the lanes' weighted velocities;
 // spreading is three integer atomics per lane.
 // Marker m of sphere p (mforce slot p*ppp + m), executed by one whole wavefront; each lane whose
-// stencil node receives force calls sink(node, weight, F).
+// stencil node receives force calls sink(sn, weight, F) with the node's coordinates sn[3].
 template <class R, int MODE, class Sink>
 __device__ __forceinline__ void ibm_marker(const IbmArgs& a, int p, int m, int lane, Sink&& sink) {
     const int gid = p * a.ppp + m;
@@ -129,7 +130,7 @@ __device__ __forceinline__ void ibm_marker(const IbmArgs& a, int p, int m, int l
     const double wr[3] = {om[1] * r0[2] - om[2] * r0[1], om[2] * r0[0] - om[0] * r0[2], om[0] * r0[1] - om[1] * r0[0]};
     double F[3];
     for (int k = 0; k < 3; k++) F[k] = (up[k] - (a.vel[3 * p + k] + wr[k])) * a.dA;
-    if (sok) sink(sn[0] + g.nx * (sn[1] + g.ny * sn[2]), sw[0] * sw[1] * sw[2], F);
+    if (sok) sink(sn, sw[0] * sw[1] * sw[2], F);
     if (lane == 0) {
         D3Q19_GLOBAL double* mf = a.mforce + 6 * (long long)gid;
         mf[0] = F[0];
@@ -158,11 +159,129 @@ __global__ void __launch_bounds__(64 * IBM_WAVES) ibm_marker_kernel(IbmArgs a) {
     const int p = gid % a.np, k = gid / a.np;
     const int m = (int)(((long long)k * a.mstride) % a.ppp);
     const long long nint = a.g.nint;
-    ibm_marker<R, MODE>(a, p, m, lane, [&](int node, double w, const double* F) { spread_add(a.facc, nint, node, w, F); });
+    ibm_marker<R, MODE>(a, p, m, lane, [&](const int* sn, double w, const double* F) {
+        spread_add(a.facc, nint, sn[0] + a.g.nx * (sn[1] + a.g.ny * sn[2]), w, F);
+    });
+}
+
+// ---- spreading staged in LDS -----------------------------------------------------------------------
+// A block of TILE_MARKERS wavefronts takes TILE_MARKERS markers of one sphere that lie next to each
+// other on its surface (a Morton-ordered permutation of the marker set): each wavefront runs its
+// marker as above, but adds its fixed-point contributions into an LDS box covering the group's
+// stencils (64-bit integer LDS atomics); the block then adds the box's nonzero nodes to the spread
+// force with one global atomic per node and component.  Stencils of neighbouring markers overlap,
+// so the global atomics, which bound the spreading, drop by the overlap factor; integer sums keep
+// the result bitwise that of per-lane atomics.  Nodes are placed in the box by their offset from
+// the sphere's centre node, taken into (-n/2, n/2] on periodic axes (used only where every periodic
+// extent exceeds twice the sphere's stencil reach).  A box above TILE_NODES spreads directly.
+__device__ __forceinline__ int rel_offset(int n, int c, int nd, int per) {
+    int o = n - c;
+    if (per) {
+        if (2 * o > nd) o -= nd;
+        else if (2 * o <= -nd) o += nd;
+    }
+    return o;
+}
+
+template <class R, int MODE>
+__device__ __forceinline__ void ibm_tile_group(const IbmArgs& a, int p, int m, bool has, int lane, long long (*tile)[TILE_NODES],
+                                               int* box) {
+    const Geom& g = a.g;
+    const int nd[3] = {g.nx, g.ny, g.nz};
+    if (threadIdx.x < 3) {
+        box[threadIdx.x] = 1 << 30;
+        box[3 + threadIdx.x] = -(1 << 30);
+        // the sphere's centre node (positions lie in [0, SystemSize))
+        box[6 + threadIdx.x] = (int)floor(a.pos[3 * p + threadIdx.x]);
+    }
+    __syncthreads();
+    if (has && lane == 0) {
+        double rpp[3];
+        bool finite = true;
+        for (int d = 0; d < 3; d++) {
+            const double L = (double)g.sys[d];
+            rpp[d] = fmod((a.pos[3 * p + d] + (double)a.pts[4 * m + d]) + L, L);  // as in ibm_marker
+            if (!(rpp[d] >= 0.0 && rpp[d] < L)) finite = false;
+        }
+        if (finite)
+            for (int d = 0; d < 3; d++) {
+                const int ob = rel_offset((int)floor(rpp[d]), box[6 + d], nd[d], g.per[d]);
+                atomicMin(&box[d], ob - 1);
+                atomicMax(&box[3 + d], ob + 2);
+            }
+    }
+    __syncthreads();
+    const int lo[3] = {box[0], box[1], box[2]}, ctr[3] = {box[6], box[7], box[8]};
+    const int ex = box[3] - box[0] + 1, ey = box[4] - box[1] + 1, ez = box[5] - box[2] + 1;
+    const bool use_tile = a.tile_ok && ex > 0 && ey > 0 && ez > 0 && ex * ey * ez <= TILE_NODES;  // uniform over the block
+    const int vol = use_tile ? ex * ey * ez : 0;
+    for (int i = threadIdx.x; i < vol; i += blockDim.x) {
+        tile[0][i] = 0;
+        tile[1][i] = 0;
+        tile[2][i] = 0;
+    }
+    __syncthreads();
+    if (has)
+        ibm_marker<R, MODE>(a, p, m, lane, [&](const int* sn, double w, const double* F) {
+            int t = -1;
+            if (use_tile) {
+                const int tx = rel_offset(sn[0], ctr[0], nd[0], g.per[0]) - lo[0];
+                const int ty = rel_offset(sn[1], ctr[1], nd[1], g.per[1]) - lo[1];
+                const int tz = rel_offset(sn[2], ctr[2], nd[2], g.per[2]) - lo[2];
+                if ((unsigned)tx < (unsigned)ex && (unsigned)ty < (unsigned)ey && (unsigned)tz < (unsigned)ez) t = tx + ex * (ty + ey * tz);
+            }
+            if (t >= 0) {
+                for (int c = 0; c < 3; c++) {
+                    const long long iv = spread_value(w, F[c]);
+                    if (iv != 0) atomicAdd((unsigned long long*)&tile[c][t], (unsigned long long)iv);
+                }
+            } else {
+                spread_add(a.facc, g.nint, sn[0] + g.nx * (sn[1] + g.ny * sn[2]), w, F);
+            }
+        });
+    __syncthreads();
+    for (int i = threadIdx.x; i < vol; i += blockDim.x) {
+        const long long v0 = tile[0][i], v1 = tile[1][i], v2 = tile[2][i];
+        if ((v0 | v1 | v2) == 0) continue;
+        const int t3[3] = {i % ex, (i / ex) % ey, i / (ex * ey)};
+        int n3[3];
+        for (int d = 0; d < 3; d++) {
+            int n = ctr[d] + lo[d] + t3[d];
+            if (g.per[d]) n = ((n % nd[d]) + nd[d]) % nd[d];
+            n3[d] = n;
+        }
+        const long long node = n3[0] + (long long)g.nx * (n3[1] + (long long)g.ny * n3[2]);
+        const long long vs[3] = {v0, v1, v2};
+        for (int c = 0; c < 3; c++)
+            if (vs[c] != 0)
+                __hip_atomic_fetch_add((D3Q19_GLOBAL unsigned long long*)(a.facc + c * (long long)g.nint + node), (unsigned long long)vs[c],
+                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+// one block per (sphere, group of TILE_MARKERS markers); blocks of consecutive index take different
+// spheres
+template <class R, int MODE>
+__global__ void __launch_bounds__(64 * TILE_MARKERS) ibm_tile_kernel(IbmArgs a) {
+    __shared__ long long tile[3][TILE_NODES];
+    __shared__ int box[9];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int b = blockIdx.x;
+    if (b == 0 && threadIdx.x == 0) {
+        const uint32_t e = a.epochs[0] + 1u;  // commits the step's epoch, as ibm_marker_kernel does
+        a.epochs[0] = e;
+        a.epochs[1] = e;
+    }
+    const int p = b % a.np, grp = b / a.np;
+    const int j = grp * TILE_MARKERS + w;
+    const bool has = j < a.ppp;
+    ibm_tile_group<R, MODE>(a, p, has ? a.perm[j] : 0, has, lane, tile, box);
 }
 
 // Batched: a persistent grid of wavefronts walks the marker list of all cells; the batch's one
-// epoch counter is committed by the first lane of the grid.
+// epoch counter is committed by the first lane of the grid.  (Staging the spreading in LDS, as for
+// a single simulation, measured slower here: the batch's smaller spheres give small groups, and
+// blocks of TILE_MARKERS wavefronts hide less latency than independent wavefronts.)
 template <class R, int MODE>
 __global__ void __launch_bounds__(64 * IBM_WAVES) ibm_batch_kernel(BatchTables t, int sp) {
     const int lane = threadIdx.x & 63;
@@ -181,7 +300,9 @@ __global__ void __launch_bounds__(64 * IBM_WAVES) ibm_batch_kernel(BatchTables t
         ia.vel = cd.velb[cd.par ^ sp ^ 1];
         const long long nint = ia.g.nint;
         D3Q19_GLOBAL long long* facc = ia.facc;
-        ibm_marker<R, MODE>(ia, d.y, v & 4095, lane, [&](int node, double w, const double* F) { spread_add(facc, nint, node, w, F); });
+        ibm_marker<R, MODE>(ia, d.y, v & 4095, lane, [&](const int* sn, double w, const double* F) {
+            spread_add(facc, nint, sn[0] + ia.g.nx * (sn[1] + ia.g.ny * sn[2]), w, F);
+        });
     }
 }
 
@@ -203,6 +324,8 @@ int marker_stride(int ppp) {
 IbmArgs make_ibm_args(Sim* s) {
     IbmArgs a;
     a.mstride = marker_stride(s->ppp > 0 ? s->ppp : 1);
+    a.tile_ok = s->tile_ok ? 1 : 0;
+    a.perm = gp(s->perm);
     a.g = s->g;
     a.np = s->np;
     a.ppp = s->ppp;
@@ -222,9 +345,22 @@ IbmArgs make_ibm_args(Sim* s) {
 int launch_ibm(Sim* s, hipStream_t st) {
     if (s->np == 0) return 0;
     const IbmArgs a = make_ibm_args(s);
+    const bool dbl = s->storage == D3Q19_STORE_FP64;
+    if (s->tile_ok) {
+        const int blocks = s->np * ((s->ppp + TILE_MARKERS - 1) / TILE_MARKERS);
+        const dim3 th(64 * TILE_MARKERS);
+        if (a.mode == D3Q19_IBM_PESKIN4) {
+            if (dbl) hipLaunchKernelGGL((ibm_tile_kernel<double, D3Q19_IBM_PESKIN4>), dim3(blocks), th, 0, st, a);
+            else hipLaunchKernelGGL((ibm_tile_kernel<float, D3Q19_IBM_PESKIN4>), dim3(blocks), th, 0, st, a);
+        } else {
+            if (dbl) hipLaunchKernelGGL((ibm_tile_kernel<double, D3Q19_IBM_LINEAR_SMOOTHED>), dim3(blocks), th, 0, st, a);
+            else hipLaunchKernelGGL((ibm_tile_kernel<float, D3Q19_IBM_LINEAR_SMOOTHED>), dim3(blocks), th, 0, st, a);
+        }
+        HIP_OK(hipGetLastError());
+        return 0;
+    }
     const int total = s->np * s->ppp;
     const int th = 64 * IBM_WAVES, blocks = (total + IBM_WAVES - 1) / IBM_WAVES;  // one wavefront per marker
-    const bool dbl = s->storage == D3Q19_STORE_FP64;
     if (a.mode == D3Q19_IBM_PESKIN4) {
         if (dbl) hipLaunchKernelGGL((ibm_marker_kernel<double, D3Q19_IBM_PESKIN4>), dim3(blocks), dim3(th), 0, st, a);
         else hipLaunchKernelGGL((ibm_marker_kernel<float, D3Q19_IBM_PESKIN4>), dim3(blocks), dim3(th), 0, st, a);

@@ -33,6 +33,8 @@ constexpr double FIX_SCALE = 1099511627776.0;       // 2^40
 constexpr double FIX_INV = 1.0 / 1099511627776.0;
 constexpr int BRICK = 4;                              // brick edge (nodes)
 constexpr int IBM_WAVES = 4;                          // markers (wavefronts) per IBM block
+constexpr int TILE_MARKERS = 16;                      // markers per block of the LDS-staged spreading
+constexpr int TILE_NODES = 1024;                      // LDS box of that spreading (nodes)
 constexpr float FP16_SCALE = 16384.0f;                // 2^14, relative-deviation scale of fp16 storage
 
 // ---- velocity set (reference order, D3Q19-OpenCL.c:9-13) ---------------------------------
@@ -159,6 +161,8 @@ struct IbmArgs {
     Geom g;
     int np, ppp, mode;
     int mstride;  // marker visiting order: wavefront k of sphere p takes marker (k * mstride) mod ppp
+    int tile_ok;  // spreading may be staged in LDS (every periodic extent exceeds twice the stencil reach)
+    const D3Q19_GLOBAL int* perm;  // ppp marker indices in Morton order of their offsets
     double dA;
     const D3Q19_GLOBAL double* pos;
     const D3Q19_GLOBAL double* vel;
@@ -302,6 +306,8 @@ struct Sim {
     bool pending_post = false; // the last step's force sums and contacts are not done yet
     bool marks_next = false;   // marks of the next epoch exist (made from the current positions)
     int* zone = nullptr;       // 2*np zone of each sphere (contact zones), by rebuild generation
+    int* perm = nullptr;       // ppp marker indices in Morton order of their offsets (LDS-staged spreading)
+    bool tile_ok = false;      // every periodic extent exceeds twice the spheres' stencil reach
     long long h_epoch = 0;     // host count of committed epochs (one per step)
     int tofs = 0;              // step number t of the contacts = committed epoch + tofs
     double* quat = nullptr;    // 4*np
