@@ -211,6 +211,11 @@ def lib():
         L.d3q19_time_steps.argtypes = [V, C.c_int, C.c_int, P(C.c_double), P(C.c_double)]
         L.d3q19_footprint.argtypes = [P(IntParams), P(Options), P(C.c_size_t)]
         L.d3q19_observables_get.argtypes = [V, P(Observables)]
+        L.d3q19_checkpoint_size.argtypes = [V, P(C.c_size_t)]
+        L.d3q19_checkpoint_save.argtypes = [V, V, C.c_size_t]
+        L.d3q19_checkpoint_load.argtypes = [V, V, C.c_size_t]
+        L.d3q19_checkpoint_write.argtypes = [V, C.c_char_p]
+        L.d3q19_checkpoint_read.argtypes = [V, C.c_char_p]
         L.d3q19_batch_create.argtypes = [C.c_int, P(BatchOptions), P(V)]
         L.d3q19_batch_destroy.argtypes = [V]
         L.d3q19_batch_add.argtypes = [V, P(IntParams), P(FlpParams), P(Options), V, V, P(C.c_int)]
@@ -455,6 +460,25 @@ class Sim:
         o = Observables()
         check(self.L.d3q19_observables_get(self.h, C.byref(o)), "observables_get")
         return o
+
+    def checkpoint(self):
+        """The whole simulation state as bytes (d3q19_checkpoint_save)."""
+        n = C.c_size_t()
+        check(self.L.d3q19_checkpoint_size(self.h, C.byref(n)), "checkpoint_size")
+        buf = C.create_string_buffer(n.value)
+        check(self.L.d3q19_checkpoint_save(self.h, buf, n.value), "checkpoint_save")
+        return buf.raw
+
+    def restore(self, image):
+        """Load a state written by checkpoint() (d3q19_checkpoint_load)."""
+        buf = C.create_string_buffer(bytes(image), len(image))
+        check(self.L.d3q19_checkpoint_load(self.h, buf, len(image)), "checkpoint_load")
+
+    def checkpoint_write(self, path):
+        check(self.L.d3q19_checkpoint_write(self.h, str(path).encode()), "checkpoint_write")
+
+    def checkpoint_read(self, path):
+        check(self.L.d3q19_checkpoint_read(self.h, str(path).encode()), "checkpoint_read")
 
     def upload_case(self):
         c = self.case

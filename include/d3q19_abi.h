@@ -199,8 +199,22 @@ int d3q19_particle_dynamics(d3q19_sim* sim);
  * (GPU_program.cl:81-313): interpolation, penalty force, deterministic spreading, FP64
  * per-sphere force and torque sums. */
 int d3q19_particle_fluid_forces(d3q19_sim* sim);
-/* particle_particle_forces, CPU_program.cl:76-184 (all pairs, FP64). */
+/* particle_particle_forces, CPU_program.cl:76-184 (FP64; pairs as d3q19_options.contact_detection
+ * selects, the zone lists rebuilt as in sim_main.c:428-444). */
 int d3q19_particle_particle_forces(d3q19_sim* sim);
+
+/* Checkpoint / restart (no counterpart in the reference, which writes only u at the end,
+ * sim_main.c:476-481).  The image holds the whole state -- populations in their storage format,
+ * u, tau, the pending spread force, brick marks and epochs, the sphere state, contact zones and
+ * the step bookkeeping -- so that a simulation created with the same parameter structs, options
+ * and marker offsets continues bit for bit after a load.  `size` gives the image bytes; save and
+ * load move it through host memory (a server can evict and resume a cell), write and read
+ * through a file.  Loads reject images of other parameters (MaxIterations aside). */
+int d3q19_checkpoint_size(d3q19_sim* sim, size_t* bytes);
+int d3q19_checkpoint_save(d3q19_sim* sim, void* buf, size_t bytes);
+int d3q19_checkpoint_load(d3q19_sim* sim, const void* buf, size_t bytes);
+int d3q19_checkpoint_write(d3q19_sim* sim, const char* path);
+int d3q19_checkpoint_read(d3q19_sim* sim, const char* path);
 
 /* One full time step t in the order of sim_main.c:315-471. */
 int d3q19_step(d3q19_sim* sim, int t, int flags);
