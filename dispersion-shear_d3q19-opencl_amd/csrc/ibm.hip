@@ -48,7 +48,8 @@ __device__ __forceinline__ double wave_sum(double v) {
 // Marker m of sphere p (mforce slot p*ppp + m), executed by one whole wavefront; each lane whose
 // stencil node receives force calls sink(sn, weight, F) with the node's coordinates sn[3].
 template <class R, int MODE, class Sink>
-__device__ __forceinline__ void ibm_marker(const IbmArgs& a, int p, int m, int lane, Sink&& sink) {
+__device__ __forceinline__ void ibm_marker(const IbmArgs& a, const D3Q19_GLOBAL double* pos, const D3Q19_GLOBAL double* vel, int p,
+                                           int m, int lane, Sink&& sink) {
     const int gid = p * a.ppp + m;
     const Geom& g = a.g;
     const double r0[3] = {(double)a.pts[4 * m], (double)a.pts[4 * m + 1], (double)a.pts[4 * m + 2]};
@@ -57,7 +58,7 @@ __device__ __forceinline__ void ibm_marker(const IbmArgs& a, int p, int m, int l
     bool finite = true;
     for (int d = 0; d < 3; d++) {
         const double L = (double)g.sys[d];
-        rpp[d] = fmod((a.pos[3 * p + d] + r0[d]) + L, L);  // GPU_program.cl:141-144
+        rpp[d] = fmod((pos[3 * p + d] + r0[d]) + L, L);  // GPU_program.cl:141-144
         if (!(rpp[d] >= 0.0 && rpp[d] < L)) finite = false;  // NaN or out of range
     }
     if (!finite) {
@@ -129,7 +130,7 @@ __device__ __forceinline__ void ibm_marker(const IbmArgs& a, int p, int m, int l
     const D3Q19_GLOBAL double* om = a.omega + 3 * p;
     const double wr[3] = {om[1] * r0[2] - om[2] * r0[1], om[2] * r0[0] - om[0] * r0[2], om[0] * r0[1] - om[1] * r0[0]};
     double F[3];
-    for (int k = 0; k < 3; k++) F[k] = (up[k] - (a.vel[3 * p + k] + wr[k])) * a.dA;
+    for (int k = 0; k < 3; k++) F[k] = (up[k] - (vel[3 * p + k] + wr[k])) * a.dA;
     if (sok) sink(sn, sw[0] * sw[1] * sw[2], F);
     if (lane == 0) {
         D3Q19_GLOBAL double* mf = a.mforce + 6 * (long long)gid;
@@ -159,7 +160,7 @@ __global__ void __launch_bounds__(64 * IBM_WAVES) ibm_marker_kernel(IbmArgs a) {
     const int p = gid % a.np, k = gid / a.np;
     const int m = (int)(((long long)k * a.mstride) % a.ppp);
     const long long nint = a.g.nint;
-    ibm_marker<R, MODE>(a, p, m, lane, [&](const int* sn, double w, const double* F) {
+    ibm_marker<R, MODE>(a, a.pos, a.vel, p, m, lane, [&](const int* sn, double w, const double* F) {
         spread_add(a.facc, nint, sn[0] + a.g.nx * (sn[1] + a.g.ny * sn[2]), w, F);
     });
 }
@@ -222,7 +223,7 @@ __device__ __forceinline__ void ibm_tile_group(const IbmArgs& a, int p, int m, b
     }
     __syncthreads();
     if (has)
-        ibm_marker<R, MODE>(a, p, m, lane, [&](const int* sn, double w, const double* F) {
+        ibm_marker<R, MODE>(a, a.pos, a.vel, p, m, lane, [&](const int* sn, double w, const double* F) {
             int t = -1;
             if (use_tile) {
                 const int tx = rel_offset(sn[0], ctr[0], nd[0], g.per[0]) - lo[0];
@@ -291,18 +292,20 @@ __global__ void __launch_bounds__(64 * IBM_WAVES) ibm_batch_kernel(BatchTables t
         t.epochs[1] = e;
     }
     const int nmark = t.cnt->nmark;
+    // the work item is uniform over the wavefront: read-first-lane keeps its indices, and so the
+    // descriptor fields read through them, in scalar registers
     for (int gid = blockIdx.x * IBM_WAVES + (int)(threadIdx.x >> 6); gid < nmark; gid += gridDim.x * IBM_WAVES) {
-        const int v = t.msph[gid];
+        const int v = __builtin_amdgcn_readfirstlane(t.msph[__builtin_amdgcn_readfirstlane(gid)]);
         const int4 d = t.sph[v >> 12];  // cell, local sphere, first global marker, markers per sphere
-        const CellDesc& cd = t.cells[d.x];
-        IbmArgs ia = cd.i;
-        ia.pos = cd.posb[cd.par ^ sp ^ 1];  // x(t+1): written by this step's turn
-        ia.vel = cd.velb[cd.par ^ sp ^ 1];
-        const long long nint = ia.g.nint;
-        D3Q19_GLOBAL long long* facc = ia.facc;
-        ibm_marker<R, MODE>(ia, d.y, v & 4095, lane, [&](const int* sn, double w, const double* F) {
-            spread_add(facc, nint, sn[0] + ia.g.nx * (sn[1] + ia.g.ny * sn[2]), w, F);
-        });
+        const int cell = __builtin_amdgcn_readfirstlane(d.x), sph = __builtin_amdgcn_readfirstlane(d.y);
+        const CellDesc& cd = t.cells[cell];
+        const IbmArgs& ia = cd.i;
+        const int par = __builtin_amdgcn_readfirstlane(cd.par);
+        // x(t+1), v(t+1): written by this step's turn
+        ibm_marker<R, MODE>(ia, cd.posb[par ^ sp ^ 1], cd.velb[par ^ sp ^ 1], sph, v & 4095, lane,
+                            [&](const int* sn, double w, const double* F) {
+                                spread_add(ia.facc, ia.g.nint, sn[0] + ia.g.nx * (sn[1] + ia.g.ny * sn[2]), w, F);
+                            });
     }
 }
 

@@ -469,7 +469,7 @@ __global__ void __launch_bounds__(256) collide_kernel(CollideArgs a) {
 // sphere block writes is read by a collide block of the same launch: the collide reads the marks of
 // this epoch and of the pending force, the turn writes the marks of the next epoch (a third array).
 template <int ST, int COLL, int VT, bool WZ>
-__global__ void __launch_bounds__(256) step_kernel(CollideArgs a, PartArgs p, int nsph, int do_post) {
+__device__ __forceinline__ void step_body(const CollideArgs& a, const PartArgs& p, int nsph, int do_post) {
     if ((int)blockIdx.x < nsph) {
         sphere_turn(p, blockIdx.x, do_post, p.epochs[0] + 2u);
         return;
@@ -484,6 +484,20 @@ __global__ void __launch_bounds__(256) step_kernel(CollideArgs a, PartArgs p, in
     const int y = tyb * a.by + (int)(threadIdx.x / a.bx);
     if (x >= a.g.nx || y >= a.g.ny || z >= a.g.nz) return;
     collide_node<ST, COLL, VT, WZ>(a, a.fin, a.fout, a.write_u_all, x, y, z, 0);
+}
+
+template <int ST, int COLL, int VT, bool WZ>
+__global__ void __launch_bounds__(256) step_kernel(CollideArgs a, PartArgs p, int nsph, int do_post) {
+    step_body<ST, COLL, VT, WZ>(a, p, nsph, do_post);
+}
+
+// The same step with the register budget of 6 waves per SIMD (80 VGPRs), which the BGK forms in
+// fp32 / fp16 / bf16 storage meet without spilling; left alone, the sphere turn's code raises the
+// whole kernel to 89 VGPRs (5 waves) while the collide tiles need 77.
+template <int ST, int COLL, int VT, bool WZ>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) step_kernel_w6(CollideArgs a, PartArgs p, int nsph,
+                                                                                            int do_post) {
+    step_body<ST, COLL, VT, WZ>(a, p, nsph, do_post);
 }
 
 // flat node index of a cell -> (x, y, z); exact for n < 2^24 (float estimate within +-1, corrected)
@@ -595,8 +609,13 @@ template <int ST, int COLL, int VT>
 static void launch_fused_one(const CollideArgs& a, const PartArgs& p, int nsph, int do_post, int threads, hipStream_t st) {
     const bool wz = a.g.wall == 2 && a.g.per[0] && a.g.per[1] && a.fl.tan_mask == 0;
     const dim3 grid(a.nblk + nsph);
-    if (wz) hipLaunchKernelGGL((step_kernel<ST, COLL, VT, true>), grid, dim3(threads), 0, st, a, p, nsph, do_post);
-    else hipLaunchKernelGGL((step_kernel<ST, COLL, VT, false>), grid, dim3(threads), 0, st, a, p, nsph, do_post);
+    if constexpr (COLL == D3Q19_COLL_BGK && ST != D3Q19_STORE_FP64) {
+        if (wz) hipLaunchKernelGGL((step_kernel_w6<ST, COLL, VT, true>), grid, dim3(threads), 0, st, a, p, nsph, do_post);
+        else hipLaunchKernelGGL((step_kernel_w6<ST, COLL, VT, false>), grid, dim3(threads), 0, st, a, p, nsph, do_post);
+    } else {
+        if (wz) hipLaunchKernelGGL((step_kernel<ST, COLL, VT, true>), grid, dim3(threads), 0, st, a, p, nsph, do_post);
+        else hipLaunchKernelGGL((step_kernel<ST, COLL, VT, false>), grid, dim3(threads), 0, st, a, p, nsph, do_post);
+    }
 }
 
 template <int ST>

@@ -4,12 +4,14 @@ Scaled-down form of the 256x256x128 precision workload: a 64x64x32 box, 20 spher
 fp32, fp16-shifted and bf16-shifted storage (all fp32 arithmetic) are compared with the fp64 build
 of the same kernels on the same seeded input.
 
-Bars:
-- fp32: u within 1e-3 relative L-inf (float32 rounding alone moves u by ~1e-4 relative within a
-  hundred steps); drag coefficient within 1e-3 absolute.
-- fp16-shifted: u within 5e-2 relative L-inf; drag coefficient within 1e-2 absolute.
-- bf16-shifted: a 7-bit mantissa on f - w gives ~4e-3 relative rounding per population and step,
-  so it carries no bar beyond finiteness; its errors are reported.
+Bars (about 4x the measured errors; the runs are deterministic):
+- fp32: u within 1e-4 relative L-inf (measured 2.7e-5); drag coefficient within 2e-4 absolute
+  (measured 3.5e-5, C_d ~ 5).
+- fp16-shifted: u within 2e-3 relative L-inf (measured 5.6e-4); drag coefficient within 1e-3
+  relative (measured 2.9e-4), the target for reduced-precision storage.
+- bf16-shifted: a 7-bit mantissa on f - w gives ~4e-3 relative rounding per population and step:
+  u within 2e-2 (measured 4.9e-3) and drag within 5e-3 relative (measured 1.0e-3); it misses the
+  1e-3 drag target on the full workload.
 """
 import json
 import math
@@ -28,9 +30,11 @@ def small_c3(storage=0):
 def test_storage_precision_against_fp64_build():
     r = precision_study(small_c3, 300)
     print(json.dumps(r, indent=1), flush=True)
-    assert r["fp32"]["u_linf_rel"] < 1e-3
-    assert r["fp32"]["drag_abs_err"] < 1e-3
-    assert r["fp16_shift"]["u_linf_rel"] < 5e-2
-    assert r["fp16_shift"]["drag_abs_err"] < 1e-2
+    assert r["fp32"]["u_linf_rel"] < 1e-4
+    assert r["fp32"]["drag_abs_err"] < 2e-4
+    assert r["fp16_shift"]["u_linf_rel"] < 2e-3
+    assert r["fp16_shift"]["drag_rel_err"] < 1e-3
     for k in ("u_l2_rel", "u_linf_rel", "drag_abs_err"):
         assert math.isfinite(r["bf16_shift"][k])
+    assert r["bf16_shift"]["u_linf_rel"] < 2e-2
+    assert r["bf16_shift"]["drag_rel_err"] < 5e-3
