@@ -111,15 +111,18 @@ static void geom_from(Sim* s) {
             if (s->opt.tangential_bc[d] == 1) f.tan_mask |= (1 << d);
 }
 
-// Default tile: one 256-thread block along x (full rows of 256, or rows of nx rounded up to 64
-// with several rows per block).  256x1 measured 3-8% faster than 64x4 on MI355X.
+// Default tile: one block along x (a row segment of up to 256 nodes, or rows of nx rounded up to 64
+// stacked to 256 threads).  Row tiles measured 3-8% faster than 64x4 on MI355X.  The sphere sums of
+// the fused step do not depend on the block size, so the tile changes no result bit.
 static void default_block(Sim* s) {
     int bx = s->opt.block[0], by = s->opt.block[1];
     if (bx <= 0) {
+        // one x-row segment of up to 256 nodes per block; rows of 128 and more alone (measured on
+        // the 128^3 workload: 128x1 tiles 2% faster than 128x2, fp32 and fp16), shorter rows stacked
         const int nx = s->g.nx;
         bx = nx >= 256 ? 256 : ((nx + 63) / 64) * 64;
         if (bx > 256) bx = 256;
-        by = 256 / bx;
+        by = bx >= 128 ? 1 : 256 / bx;
     }
     if (by <= 0) by = 256 / bx;
     s->block[0] = bx;

@@ -33,7 +33,7 @@ constexpr double FIX_SCALE = 1099511627776.0;       // 2^40
 constexpr double FIX_INV = 1.0 / 1099511627776.0;
 constexpr int BRICK = 4;                              // brick edge (nodes)
 constexpr int IBM_WAVES = 4;                          // markers (wavefronts) per IBM block
-constexpr int TILE_MARKERS = 16;                      // markers per block of the LDS-staged spreading
+constexpr int TILE_MARKERS = 8;                       // markers per block of the LDS-staged spreading
 constexpr int TILE_NODES = 1024;                      // LDS box of that spreading (nodes)
 constexpr float FP16_SCALE = 16384.0f;                // 2^14, relative-deviation scale of fp16 storage
 

@@ -172,19 +172,27 @@ __device__ __forceinline__ void sphere_pre(const PartArgs& a, int p, int do_dyn,
     }
 }
 
-// fixed-order sums over the block of K values per thread (wave butterfly, then the waves in
-// order): deterministic
+// Fixed-order sums over 256 virtual threads, whatever the block size (a multiple of 64, at most
+// 256): virtual thread vt (threadIdx.x, threadIdx.x + blockDim.x, ...) accumulates items vt,
+// vt + 256, ... in order, each virtual wave sums by butterfly (wave_store), then every thread adds
+// the four waves in order (wave_total).  The bits do not depend on the launch shape, so the fused
+// step (collide-tile blocks) and the separate sphere kernels (256 threads) agree exactly.
+constexpr int VTHREADS = 256;
+
 template <int K>
-__device__ __forceinline__ void block_sums(double* v, double (*sh)[K]) {
+__device__ __forceinline__ void wave_store(double* v, double (*sh)[K], int vt) {
     for (int k = 0; k < K; k++)
         for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_xor(v[k], off, 64);
-    const int w = threadIdx.x >> 6, nw = blockDim.x >> 6;
-    if ((threadIdx.x & 63) == 0)
-        for (int k = 0; k < K; k++) sh[w][k] = v[k];
+    if ((vt & 63) == 0)
+        for (int k = 0; k < K; k++) sh[vt >> 6][k] = v[k];
+}
+
+template <int K>
+__device__ __forceinline__ void wave_total(double* v, double (*sh)[K]) {
     __syncthreads();
     for (int k = 0; k < K; k++) {
         double t = 0;
-        for (int i = 0; i < nw; i++) t += sh[i][k];
+        for (int i = 0; i < VTHREADS / 64; i++) t += sh[i][k];
         v[k] = t;
     }
     __syncthreads();
@@ -195,12 +203,16 @@ __device__ __forceinline__ void sphere_post(const PartArgs& a, int p, int do_red
     __shared__ double sh6[4][6];
     __shared__ double sh3[4][3];
     if (do_reduce) {
-        double acc[6] = {0, 0, 0, 0, 0, 0};
-        for (int m = threadIdx.x; m < a.ppp; m += blockDim.x) {
-            const auto mf = a.mforce + 6 * ((long long)p * a.ppp + m);
-            for (int k = 0; k < 6; k++) acc[k] += mf[k];
+        double acc[6];
+        for (int vt = threadIdx.x; vt < VTHREADS; vt += blockDim.x) {  // wave-uniform trip count
+            for (int k = 0; k < 6; k++) acc[k] = 0;
+            for (int m = vt; m < a.ppp; m += VTHREADS) {
+                const auto mf = a.mforce + 6 * ((long long)p * a.ppp + m);
+                for (int k = 0; k < 6; k++) acc[k] += mf[k];
+            }
+            wave_store<6>(acc, sh6, vt);
         }
-        block_sums<6>(acc, sh6);
+        wave_total<6>(acc, sh6);
         if (threadIdx.x == 0)
             for (int k = 0; k < 3; k++) {
                 a.ffluid[3 * p + k] = acc[k];
@@ -215,44 +227,48 @@ __device__ __forceinline__ void sphere_post(const PartArgs& a, int p, int do_red
     const double rp = 0.5 * a.diam;
     const double xi[3] = {a.pos[3 * p], a.pos[3 * p + 1], a.pos[3 * p + 2]};
     const double vi[3] = {a.vel[3 * p], a.vel[3 * p + 1], a.vel[3 * p + 2]};
-    double F[3] = {0, 0, 0};
-    for (int j = threadIdx.x; j < a.np; j += blockDim.x) {
-        if (j == p) continue;
-        // pair (pi, pj) = (max, min) as in the reference: r = x_pj - x_pi, v = v_pj - v_pi
-        const bool ipi = p > j;
-        double r[3], v[3];
-        for (int d = 0; d < 3; d++) {
-            const double xj = a.pos[3 * j + d], vj = a.vel[3 * j + d];
-            double rd = ipi ? xj - xi[d] : xi[d] - xj;
-            const double sg = rd < 0 ? -1.0 : (rd > 0 ? 1.0 : 0.0);
-            rd = (fabs(rd) < a.sys[d] / 2.0) ? rd : (rd - sg * a.sys[d]);
-            r[d] = rd;
-            v[d] = ipi ? vj - vi[d] : vi[d] - vj;
+    double F[3];
+    for (int vt = threadIdx.x; vt < VTHREADS; vt += blockDim.x) {  // wave-uniform trip count
+        F[0] = F[1] = F[2] = 0;
+        for (int j = vt; j < a.np; j += VTHREADS) {
+            if (j == p) continue;
+            // pair (pi, pj) = (max, min) as in the reference: r = x_pj - x_pi, v = v_pj - v_pi
+            const bool ipi = p > j;
+            double r[3], v[3];
+            for (int d = 0; d < 3; d++) {
+                const double xj = a.pos[3 * j + d], vj = a.vel[3 * j + d];
+                double rd = ipi ? xj - xi[d] : xi[d] - xj;
+                const double sg = rd < 0 ? -1.0 : (rd > 0 ? 1.0 : 0.0);
+                rd = (fabs(rd) < a.sys[d] / 2.0) ? rd : (rd - sg * a.sys[d]);
+                r[d] = rd;
+                v[d] = ipi ? vj - vi[d] : vi[d] - vj;
+            }
+            const double rsep = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+            const double minsep = rsep - 2.0 * rp;
+            if (minsep >= 0.1 * rp) continue;  // neither overlap nor squeeze range
+            double mult = 1.0;
+            if (a.zone_mode) {
+                const int zj = a.zone[zgen * a.np + j];
+                mult = (double)(ipi ? zone_multiplicity(a, zp, zj) : zone_multiplicity(a, zj, zp));
+                if (mult == 0.0) continue;
+            }
+            const double e[3] = {r[0] / rsep, r[1] / rsep, r[2] / rsep};
+            const double vrel = -(e[0] * v[0] + e[1] * v[1] + e[2] * v[2]);
+            double fmag = 0.0;
+            if (-minsep > 0 && a.force_model == 1) fmag += a.kcol * (-minsep);
+            if (minsep > 0 && minsep < 0.1 * rp && a.visc_model == D3Q19_VISC_NEWTONIAN) {
+                const double rstar = rp / 2.0;
+                const double ms = minsep < 0.1 ? 0.1 : minsep;
+                const double eta0 = (a.tau0 - 0.5) / 3.0;
+                fmag += 6.0 * 3.14159265358979323846 * eta0 * rstar * rstar * vrel / ms;
+            }
+            // the reference subtracts e*f from pi and adds it to pj
+            const double sgn = ipi ? -1.0 : 1.0;
+            for (int d = 0; d < 3; d++) F[d] += mult * (sgn * e[d] * fmag);
         }
-        const double rsep = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
-        const double minsep = rsep - 2.0 * rp;
-        if (minsep >= 0.1 * rp) continue;  // neither overlap nor squeeze range
-        double mult = 1.0;
-        if (a.zone_mode) {
-            const int zj = a.zone[zgen * a.np + j];
-            mult = (double)(ipi ? zone_multiplicity(a, zp, zj) : zone_multiplicity(a, zj, zp));
-            if (mult == 0.0) continue;
-        }
-        const double e[3] = {r[0] / rsep, r[1] / rsep, r[2] / rsep};
-        const double vrel = -(e[0] * v[0] + e[1] * v[1] + e[2] * v[2]);
-        double fmag = 0.0;
-        if (-minsep > 0 && a.force_model == 1) fmag += a.kcol * (-minsep);
-        if (minsep > 0 && minsep < 0.1 * rp && a.visc_model == D3Q19_VISC_NEWTONIAN) {
-            const double rstar = rp / 2.0;
-            const double ms = minsep < 0.1 ? 0.1 : minsep;
-            const double eta0 = (a.tau0 - 0.5) / 3.0;
-            fmag += 6.0 * 3.14159265358979323846 * eta0 * rstar * rstar * vrel / ms;
-        }
-        // the reference subtracts e*f from pi and adds it to pj
-        const double sgn = ipi ? -1.0 : 1.0;
-        for (int d = 0; d < 3; d++) F[d] += mult * (sgn * e[d] * fmag);
+        wave_store<3>(F, sh3, vt);
     }
-    block_sums<3>(F, sh3);
+    wave_total<3>(F, sh3);
     if (threadIdx.x == 0) {
         // z-wall repulsion (CPU_program.cl:175-182)
         const double lo = a.zbuf - xi[2];

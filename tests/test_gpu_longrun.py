@@ -81,3 +81,64 @@ def test_c1_2000_steps_match_f64_oracle(oracle_state, storage):
     assert e_cd < 1e-3
     assert e_prof < (1e-3 if storage == 0 else 5e-3)
     assert e_ws < (1e-3 if storage == 0 else 3e-2)
+
+
+@pytest.fixture(scope="module")
+def c2_runs():
+    """5000 steps of the 128^3 power-law workload per storage format, run on first use."""
+    from workloads import c2_deck
+    res = {}
+
+    def get(st):
+        if st not in res:
+            c = d3q19.Case(lines=c2_deck(storage=st))
+            s = d3q19.Sim(c)
+            s.upload_case()
+            s.run(1, C2_STEPS)
+            o = s.observables()
+            pos = s.read_particles_f64()[0].reshape(-1, 3)[: c.np].copy()
+            assert s.check_finite() == 0
+            s.close()
+            res[st] = (o.relative_viscosity, o.drag_coefficient, pos)
+        return res[st]
+    return get
+
+
+C2_STEPS = 5000
+
+
+def c2_errors(c2_runs, st):
+    L = 128.0
+    ref = c2_runs(3)
+    rv, cd, pos = c2_runs(st)
+    d = np.abs(pos - ref[2])
+    d = np.minimum(d, L - d).max()
+    e_rv, e_cd = abs(rv / ref[0] - 1.0), abs(cd / ref[1] - 1.0)
+    print("storage %d: relative viscosity %.6f (fp64 %.6f, err %.2e)  C_d %.6f (fp64 %.6f, err %.2e)  pos %.2e" % (
+        st, rv, ref[0], e_rv, cd, ref[1], e_cd, d), flush=True)
+    return e_rv, e_cd, d
+
+
+def test_c2_5000_steps_fp32_against_fp64_build(c2_runs):
+    """The 128^3 power-law workload (32 spheres, 192 markers each, n = 0.7, tau clamp [0.51, 5]):
+    5000 steps in fp32 storage against the fp64 build of the same kernels (the GPU's FP64 oracle,
+    itself within 1e-9 of the float64 oracle in test_gpu_parity.py).  Bars: the relative suspension
+    viscosity and the drag coefficient within 1e-3 relative (measured 9.2e-5, 1.4e-4); every
+    sphere's position within 2.5e-2 lattice units (measured 5.9e-3, 0.05% of the diameter: contacts
+    and shear amplify rounding along the trajectories)."""
+    e_rv, e_cd, d = c2_errors(c2_runs, 0)
+    assert e_rv < 1e-3
+    assert e_cd < 1e-3
+    assert d < 2.5e-2
+
+
+@pytest.mark.xfail(reason="fp16-shifted storage with round-to-nearest drifts on this developing flow: "
+                          "relative viscosity 1.8e-2 and drag 1.2e-2 off after 5000 steps, growing linearly",
+                   strict=False)
+def test_c2_5000_steps_fp16_against_fp64_build(c2_runs):
+    """As above for fp16-shifted storage, with the same 1e-3 bars on relative viscosity and drag
+    and 0.25 lattice units on positions."""
+    e_rv, e_cd, d = c2_errors(c2_runs, 1)
+    assert e_rv < 1e-3
+    assert e_cd < 1e-3
+    assert d < 0.25
