@@ -62,6 +62,8 @@ static int validate(const int_param_struct* ip, const flp_param_struct* fp, cons
         if (opt->block[0] > 0 && (opt->block[0] * (opt->block[1] > 0 ? opt->block[1] : 1) > 256 ||
                                    (opt->block[0] * (opt->block[1] > 0 ? opt->block[1] : 1)) % 64 != 0))
             return fail(D3Q19_E_INVALID, "block must hold a multiple of 64 and at most 256 threads");
+        if (opt->rounding != D3Q19_ROUND_STOCHASTIC && opt->rounding != D3Q19_ROUND_NEAREST)
+            return fail(D3Q19_E_INVALID, "unknown rounding mode");
         if (opt->contact_detection != D3Q19_CONTACT_ZONES && opt->contact_detection != D3Q19_CONTACT_ALL_PAIRS)
             return fail(D3Q19_E_INVALID, "unknown contact detection");
         if (opt->contact_detection == D3Q19_CONTACT_ZONES && ip->NumParticles > 0) {
@@ -106,6 +108,8 @@ static void geom_from(Sim* s) {
     f.tau_max = s->opt.tau_max > 0.f ? s->opt.tau_max : 0.f;
     f.calc_rho = s->opt.calc_rho;
     f.tan_mask = 0;
+    f.sr = (s->storage == D3Q19_STORE_FP16_SHIFT || s->storage == D3Q19_STORE_BF16_SHIFT) &&
+           s->opt.rounding == D3Q19_ROUND_STOCHASTIC;
     if (g.wall >= 0)  // tangential passes only run when a velocity boundary exists (sim_main.c:383-405)
         for (int d = 0; d < 3; d++)
             if (s->opt.tangential_bc[d] == 1) f.tan_mask |= (1 << d);

@@ -110,6 +110,7 @@ struct FluidParams {
     float tau_min, tau_max;
     int calc_rho;
     int tan_mask;         // bit a: tangential velocity BC on axis a
+    int sr;               // stochastic rounding of 16-bit population stores
 };
 
 // Pointer fields that device code dereferences in place are declared in the global address space

@@ -24,20 +24,27 @@ __device__ __forceinline__ uint16_t f32_to_bf16(float f) {
     return (uint16_t)(u >> 16);
 }
 
+// Storage formats.  st rounds to nearest; st_sr takes 32 random bits r and rounds stochastically
+// (unbiased: the result's expectation over uniform r is the stored value), which only the 16-bit
+// formats need.
 template <int ST> struct Pop;
 template <> struct Pop<D3Q19_STORE_FP32> {
     using T = float;
     using R = float;
+    static constexpr bool SR = false;
     template <int QI, class PT> __device__ static R ld(PT p, unsigned off) { return p[off]; }
     template <int QI, class PT> __device__ static double ldd(PT p, unsigned off) { return (double)p[off]; }
     template <int QI, class PT> __device__ static void st(PT p, long long off, R v) { p[off] = v; }
+    template <int QI, class PT> __device__ static void st_sr(PT p, long long off, R v, uint32_t) { p[off] = v; }
 };
 template <> struct Pop<D3Q19_STORE_FP64> {
     using T = double;
     using R = double;
+    static constexpr bool SR = false;
     template <int QI, class PT> __device__ static R ld(PT p, unsigned off) { return p[off]; }
     template <int QI, class PT> __device__ static double ldd(PT p, unsigned off) { return p[off]; }
     template <int QI, class PT> __device__ static void st(PT p, long long off, R v) { p[off] = v; }
+    template <int QI, class PT> __device__ static void st_sr(PT p, long long off, R v, uint32_t) { p[off] = v; }
 };
 // binary16 of the relative deviation (f_q - w_q)/w_q scaled by 2^14: deviations down to ~1e-8
 // stay normal numbers and the headroom reaches |dev| = 4.
@@ -52,10 +59,21 @@ template <> struct Pop<D3Q19_STORE_FP16_SHIFT> {
     template <int QI, class PT> __device__ static double ldd(PT p, unsigned off) {
         return wq(QI) + (double)p[off] * (wq(QI) / 16384.0);
     }
+    static constexpr bool SR = true;
     template <int QI, class PT> __device__ static void st(PT p, long long off, R v) {
         constexpr float w = (float)wq(QI);
         constexpr float isc = (float)(16384.0 / wq(QI));
         p[off] = (_Float16)((v - w) * isc);
+    }
+    // the 13 magnitude bits that binary16 drops, plus 13 random bits, carry into the kept ones with
+    // probability equal to the dropped fraction; the truncated value then converts exactly (normal
+    // binary16 range; |f - w|/w below 2^-28 also rounds to nearest, which is immaterial).  The
+    // collide measured 2% faster this way than with v_cvt_sr_f16_f32.
+    template <int QI, class PT> __device__ static void st_sr(PT p, long long off, R v, uint32_t r) {
+        constexpr float w = (float)wq(QI);
+        constexpr float isc = (float)(16384.0 / wq(QI));
+        const uint32_t b = (__float_as_uint((v - w) * isc) + (r >> 19)) & 0xFFFFE000u;
+        p[off] = (_Float16)__uint_as_float(b);
     }
 };
 // bfloat16 of f_q - w_q
@@ -69,11 +87,40 @@ template <> struct Pop<D3Q19_STORE_BF16_SHIFT> {
     template <int QI, class PT> __device__ static double ldd(PT p, unsigned off) {
         return wq(QI) + (double)bf16_to_f32(p[off]);
     }
+    static constexpr bool SR = true;
     template <int QI, class PT> __device__ static void st(PT p, long long off, R v) {
         constexpr float w = (float)wq(QI);
         p[off] = f32_to_bf16(v - w);
     }
+    // the 16 discarded magnitude bits plus 16 random bits carry into the kept ones with
+    // probability equal to the discarded fraction (finite inputs)
+    template <int QI, class PT> __device__ static void st_sr(PT p, long long off, R v, uint32_t r) {
+        constexpr float w = (float)wq(QI);
+        p[off] = (uint16_t)((__float_as_uint(v - w) + (r >> 16)) >> 16);
+    }
 };
+
+// 32-bit finaliser of MurmurHash3 (Appleby): a bijection whose output bits each depend on every
+// input bit
+__device__ __forceinline__ uint32_t fmix32(uint32_t h) {
+    h ^= h >> 16;
+    h *= 0x85EBCA6Bu;
+    h ^= h >> 13;
+    h *= 0xC2B2AE35u;
+    h ^= h >> 16;
+    return h;
+}
+
+// Random word of population q for stochastic rounding at one node: seven products of the node's
+// hash h with fixed odd constants, each read through three rotations (11 bits apart) so that the
+// high bits that decide a rounding differ between the populations.
+template <int q> __device__ __forceinline__ uint32_t sr_word(uint32_t h) {
+    constexpr uint32_t K[7] = {0x9E3779B1u, 0x85EBCA77u, 0xC2B2AE3Du, 0x27D4EB2Fu, 0x165667B1u, 0xD3A2646Du, 0xFD7046C5u};
+    const uint32_t x = h * K[q / 3];
+    constexpr int s = 11 * (q % 3);
+    if constexpr (s == 0) return x;
+    else return (x << s) | (x >> (32 - s));
+}
 
 // compile-time loop
 template <int I, int N> struct Unroll {
@@ -439,10 +486,22 @@ __device__ __forceinline__ void collide_node(const CollideArgs& a, const void* f
         });
     }
     const auto out = to_global<T>(fout);
-    Unroll<0, Q>::run([&](auto qc) __attribute__((always_inline)) {
-        constexpr int q = decltype(qc)::value;
-        P::template st<q>(out, (long long)q * g.qstride + n, fo[q]);
-    });
+    if (P::SR && a.fl.sr) {
+        // random bits from the node index and the bit patterns of the incoming populations: the
+        // same for a cell alone and in a batch, and new whenever an input changes
+        uint32_t x = (uint32_t)n * 0x9E3779B1u;
+        Unroll<0, Q>::run([&](auto qc) __attribute__((always_inline)) { x ^= __float_as_uint((float)f[decltype(qc)::value]); });
+        const uint32_t h = fmix32(x);
+        Unroll<0, Q>::run([&](auto qc) __attribute__((always_inline)) {
+            constexpr int q = decltype(qc)::value;
+            P::template st_sr<q>(out, (long long)q * g.qstride + n, fo[q], sr_word<q>(h));
+        });
+    } else {
+        Unroll<0, Q>::run([&](auto qc) __attribute__((always_inline)) {
+            constexpr int q = decltype(qc)::value;
+            P::template st<q>(out, (long long)q * g.qstride + n, fo[q]);
+        });
+    }
 }
 
 template <int ST, int COLL, int VT, bool WZ>

@@ -70,7 +70,8 @@ class Options(C.Structure):
         ("block", C.c_int32 * 3),
         ("contact_detection", C.c_int32),
         ("rebuild_freq", C.c_int32),
-        ("reserved", C.c_int32 * 4),
+        ("rounding", C.c_int32),
+        ("reserved", C.c_int32 * 3),
     ]
 
 
@@ -101,6 +102,7 @@ class HostParams(C.Structure):
         ("Storage", C.c_int),
         ("MassModel", C.c_int),
         ("ContactDetection", C.c_int),
+        ("Rounding", C.c_int),
         ("DataDir", C.c_char * 256),
     ]
 
@@ -146,6 +148,7 @@ COLL_MRT, COLL_BGK = 0, 1
 IBM_LINEAR, IBM_PESKIN4 = 0, 1
 STORE_FP32, STORE_FP16, STORE_BF16, STORE_FP64 = 0, 1, 2, 3
 CONTACT_ZONES, CONTACT_ALL_PAIRS = 0, 1
+ROUND_STOCHASTIC, ROUND_NEAREST = 0, 1
 STEP_WRITE_U = 1
 E_NONFINITE = -6
 

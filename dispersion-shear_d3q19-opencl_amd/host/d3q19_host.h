@@ -45,6 +45,7 @@ typedef struct {
     int Storage;              /* "population_storage": 0 fp32, 1 fp16 shifted, 2 bf16 shifted, 3 fp64 */
     int MassModel;            /* "mass_model": 0 reference (4/3)pi rho d^3, 1 sphere (pi/6) rho d^3 */
     int ContactDetection;     /* "contact_detection": 0 zone lists (reference), 1 all pairs */
+    int Rounding;             /* "population_rounding": 16-bit stores, 0 stochastic, 1 nearest */
     char DataDir[256];        /* "marker_directory": folder holding unit_sphere_partition_N.txt */
 } host_param_struct;
 

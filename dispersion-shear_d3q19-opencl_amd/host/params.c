@@ -83,6 +83,7 @@ static const keyword_t KEYS[] = {
     {"population_storage", T_INT, HP(Storage), "0"},
     {"mass_model", T_INT, HP(MassModel), "0"},
     {"contact_detection", T_INT, HP(ContactDetection), "0"},
+    {"population_rounding", T_INT, HP(Rounding), "0"},
     {"marker_directory", T_STRING, HP(DataDir), "unit_sphere_partitions"},
 };
 #define NKEYS ((int)(sizeof(KEYS) / sizeof(KEYS[0])))
@@ -471,4 +472,5 @@ void host_fill_options(const host_param_struct* hp, d3q19_options* opt) {
     for (int d = 0; d < 3; d++) opt->tangential_bc[d] = hp->TangentialVelBC[d];
     opt->contact_detection = hp->ContactDetection;
     opt->rebuild_freq = hp->RebuildFreq;
+    opt->rounding = hp->Rounding;
 }

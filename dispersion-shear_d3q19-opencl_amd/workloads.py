@@ -14,9 +14,10 @@ import numpy as np
 def deck(size, walls=(0, 0, 1), tau=0.8, uw=0.01, np_=0, diam=8.0, markers=0, dist=4, seed=1, gap=2.0,
          coll=1, ibm=1, visc=1, vparams=(0, 0, 0, 0), local=0, clamp=(0.0, 0.0), storage=0, zbuf=None,
          body=(0.0, 0.0, 0.0), buffer=None, kcol=1.0, mass_model=0, init="zero", u0=(0.0, 0.0, 0.0), tangential=(0, 0, 0),
-         iterations=100, contacts=1, rebuild=10):
+         iterations=100, contacts=1, rebuild=10, rounding=0):
     """Lines of an input file.  size = interior box edge(s) (SystemSize for periodic axes).
-    contacts: 1 all sphere pairs (the benchmark workloads), 0 the reference's neighbour-zone lists."""
+    contacts: 1 all sphere pairs (the benchmark workloads), 0 the reference's neighbour-zone lists.
+    rounding: 16-bit population stores, 0 stochastic, 1 round to nearest."""
     if np.isscalar(size):
         size = (size, size, size)
     N = [size[d] + 2 + (1 if walls[d] else 0) for d in range(3)]
@@ -48,6 +49,7 @@ def deck(size, walls=(0, 0, 1), tau=0.8, uw=0.01, np_=0, diam=8.0, markers=0, di
         "initial_vel %r %r %r" % tuple(u0),
         "contact_detection %d" % contacts,
         "rebuild_neigh_list_freq %d" % rebuild,
+        "population_rounding %d" % rounding,
     ]
     if zbuf is not None:
         lines.append("z_wall_particle_buffer %r" % zbuf)
@@ -152,14 +154,16 @@ def poisson_arrivals(rate, count, seed=6):
     return np.cumsum(rng.exponential(1.0 / rate, size=count))
 
 
-def precision_study(deck_fn, steps, storages=(0, 1, 2), reference=3):
+def precision_study(deck_fn, steps, storages=(0, 1, 2), reference=3, nearest=(1, 2)):
     """Run the same workload with each population storage and with the fp64 build; report L2 and
     L-inf errors of u (relative to the fp64 run, over interior nodes), and the errors of the drag
-    coefficient, wall stress and sphere positions."""
+    coefficient, wall stress and sphere positions.  The 16-bit storages round stochastically (the
+    default); those listed in `nearest` are also run with round-to-nearest stores ("*_nearest")."""
     import d3q19
     runs = {}
-    for st in (reference,) + tuple(storages):
-        c = d3q19.Case(lines=deck_fn(storage=st))
+    variants = [(reference, 0)] + [(st, 0) for st in storages] + [(st, 1) for st in nearest]
+    for st, rnd in variants:
+        c = d3q19.Case(lines=deck_fn(storage=st) + ["population_rounding %d" % rnd])
         s = d3q19.Sim(c)
         if c.np:
             s.write_particles(c.parKin)
@@ -169,20 +173,20 @@ def precision_study(deck_fn, steps, storages=(0, 1, 2), reference=3):
         o = s.observables()
         pos = s.read_particles_f64()[0].reshape(-1, 3)[: c.np] if c.np else np.zeros((0, 3))
         s.close()
-        runs[st] = (u, o, pos, c)
+        runs[(st, rnd)] = (u, o, pos, c)
         del c
-    u64, o64, p64, c = runs[reference]
+    u64, o64, p64, c = runs[(reference, 0)]
     L = np.array(list(c.ip.SystemSize), dtype=np.float64)
     out = {"reference": "fp64 build", "steps": steps,
            "fp64": {"drag_coefficient": o64.drag_coefficient, "wall_stress": o64.wall_stress,
                     "relative_viscosity": o64.relative_viscosity}}
     names = {0: "fp32", 1: "fp16_shift", 2: "bf16_shift", 3: "fp64"}
-    for st in storages:
-        u, o, pos, _ = runs[st]
+    for st, rnd in variants[1:]:
+        u, o, pos, _ = runs[(st, rnd)]
         d = u - u64
         dp = np.abs(pos - p64)
         dp = np.minimum(dp, L - dp) if len(pos) else dp
-        out[names[st]] = {
+        out[names[st] + ("_nearest" if rnd else "")] = {
             "u_l2_rel": float(np.sqrt((d * d).sum() / (u64 * u64).sum())),
             "u_linf_rel": float(np.abs(d).max() / np.abs(u64).max()),
             "drag_coefficient": o.drag_coefficient,

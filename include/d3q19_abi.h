@@ -115,8 +115,19 @@ typedef struct {
     int32_t contact_detection;  /* D3Q19_CONTACT_*: which sphere pairs the contact forces visit */
     int32_t rebuild_freq;       /* hostDat.RebuildFreq: zone lists rebuilt after steps t with
                                    t % rebuild_freq == 0 (sim_main.c:429-444); <= 0 selects 10 */
-    int32_t reserved[4];
+    int32_t rounding;           /* D3Q19_ROUND_*: how the collide rounds fp16 / bf16 shifted stores */
+    int32_t reserved[3];
 } d3q19_options;
+
+/* Rounding of the collide's 16-bit stores (no counterpart in the reference, which stores fp32).
+ * STOCHASTIC: unbiased stochastic rounding (gfx950 v_cvt_sr_f16_f32 for fp16, integer dither
+ *   for bf16) with random bits hashed from the node index and the node's incoming populations:
+ *   deterministic, the same for a cell alone and in a batch, fresh whenever the inputs change.
+ *   Nearly uniform fields (the populations next to a moving wall) otherwise get the same rounding
+ *   error at every node, which does not average out, and updates below half an ulp are lost.
+ * NEAREST: round to nearest even. */
+#define D3Q19_ROUND_STOCHASTIC 0
+#define D3Q19_ROUND_NEAREST    1
 
 /* Contact pair enumeration (particle_particle_forces, CPU_program.cl:76-184).
  * ZONES: the reference's neighbour-zone lists, quirks included (D3Q19-OpenCL.c:344-463,

@@ -12,11 +12,14 @@ Bars, 1e-3 for fp32 storage:
 - sphere position: absolute, in lattice units;
 - wall stress: relative;
 - drag coefficient: absolute.
-fp16-shifted storage keeps the sphere and drag bars (measured: velocity 1.3e-5 U, angular velocity
-8.3e-4, position 6.5e-5, drag 4.7e-4).  The stress lives in the non-equilibrium part of the
-populations, about 7e-4 of w_q here, and binary16 rounding of f - w costs a few percent of it per
-node.  So fp16 carries measured envelopes instead: profile 5e-3 (measured 1.7e-3), wall stress
-3e-2 (measured 1.3e-2).
+fp16-shifted storage (stochastic rounding, the default) keeps the same bars (measured: profile
+1.9e-4, velocity 1.3e-5 U, angular velocity 1.3e-4, position 9.9e-5, wall stress 8.5e-6) except
+the drag coefficient.  The net fluid force on the centred sphere vanishes by symmetry (the
+oracle's C_d is 1.7e-5), so C_d = |F| / (rho U^2 pi a^2 / 2) reads the force's noise floor: fp32
+1.0e-4, fp16 1.8e-3, held to 5e-3.  (With round-to-nearest stores, whose errors are frozen in
+space and cancel in the symmetric sum, the drag reads 4.7e-4, but the wall stress is off by 1.3e-2
+and the profile by 1.7e-3: the stress lives in the non-equilibrium part of the populations, about
+7e-4 of w_q here, and every node of a wall plane then makes the same rounding error.)
 """
 import numpy as np
 import pytest
@@ -78,9 +81,9 @@ def test_c1_2000_steps_match_f64_oracle(oracle_state, storage):
     assert e_vel < 1e-3
     assert e_om < 1e-3
     assert e_pos < 1e-3
-    assert e_cd < 1e-3
-    assert e_prof < (1e-3 if storage == 0 else 5e-3)
-    assert e_ws < (1e-3 if storage == 0 else 3e-2)
+    assert e_cd < (1e-3 if storage == 0 else 5e-3)
+    assert e_prof < 1e-3
+    assert e_ws < 1e-3
 
 
 @pytest.fixture(scope="module")
@@ -132,13 +135,11 @@ def test_c2_5000_steps_fp32_against_fp64_build(c2_runs):
     assert d < 2.5e-2
 
 
-@pytest.mark.xfail(reason="fp16-shifted storage with round-to-nearest drifts on this developing flow: "
-                          "relative viscosity 1.8e-2 and drag 1.2e-2 off after 5000 steps, growing linearly",
-                   strict=False)
 def test_c2_5000_steps_fp16_against_fp64_build(c2_runs):
-    """As above for fp16-shifted storage, with the same 1e-3 bars on relative viscosity and drag
-    and 0.25 lattice units on positions."""
+    """As above for fp16-shifted storage with stochastic rounding (the default), with the same bars
+    (measured: relative viscosity 7.3e-5, drag 2.0e-4, positions 4.9e-3).  With round-to-nearest
+    stores the same run drifts: 1.8e-2, 1.2e-2 and 5.9e-2 after 5000 steps, growing with time."""
     e_rv, e_cd, d = c2_errors(c2_runs, 1)
     assert e_rv < 1e-3
     assert e_cd < 1e-3
-    assert d < 0.25
+    assert d < 2.5e-2

@@ -284,6 +284,11 @@ def test_invalid_parameters_rejected_before_device_use():
     h = C.c_void_p()
     rc = L.d3q19_create(0, C.byref(c.ip), C.byref(c.fp), C.byref(c.opt), None, C.byref(h))
     assert rc == -1 and b"SystemSize" in L.d3q19_last_error()
+    c = d3q19.Case(lines=["total_lattice_size 12 12 13", "boundary_conditions_xyz 0 0 1", "population_storage 1",
+                          "population_rounding 2"])
+    assert c.opt.storage == 1 and c.opt.rounding == 2
+    rc = L.d3q19_create(0, C.byref(c.ip), C.byref(c.fp), C.byref(c.opt), None, C.byref(h))
+    assert rc == -1 and b"rounding" in L.d3q19_last_error()
 
 
 def test_parameter_checking_rejects_two_wall_axes():
