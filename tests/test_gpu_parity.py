@@ -203,3 +203,26 @@ def test_c1_geometry_100_steps():
     # markers at the centre, Peskin 4-point; 100 steps; fields and sphere state within 1e-5.
     c = case(size=64, coll=1, ibm=1, np_=1, diam=16.0, markers=256, dist=3, buffer=32.0, tau=0.8, uw=0.01)
     run_pair(c, 100)
+
+
+@pytest.mark.parametrize("coll,storage,visc", [(0, 0, 1), (1, 0, 2), (1, 1, 2), (0, 3, 1)])
+def test_tile_shape_changes_no_result_bit(coll, storage, visc):
+    """No result may depend on which nodes share a wavefront or a block (the sphere sums of the fused
+    step run over 256 virtual threads whatever the block size): 40 steps of a sphere workload
+    (forced and unforced nodes in the same rows) with 64x4, 128x2, 256x1 and the default 128x1
+    tiles give the same populations, velocities and sphere state bit for bit."""
+    vp = (0, 0, 0, 0) if visc == 1 else (2.0e-2, 0.7, 0, 0)
+    runs = []
+    for block in ((64, 4), (128, 2), (256, 1), (0, 0)):
+        c = case(size=(96, 24, 24), coll=coll, ibm=1, np_=6, diam=8.0, markers=128, dist=4, seed=31, tau=0.8, uw=0.03,
+                 storage=storage, visc=visc, vparams=vp, local=1 if visc != 1 else 0, clamp=(0.51, 5.0) if visc != 1 else (0, 0))
+        c.opt.block[0], c.opt.block[1] = block
+        s = d3q19.Sim(c)
+        s.upload_case()
+        s.run(1, 40)
+        runs.append((s.read_f(), s.read_u(), s.read_particles_f64()[0].copy()))
+        assert s.check_finite() == 0
+        s.close()
+    for r in runs[1:]:
+        for a, b in zip(runs[0], r):
+            assert np.array_equal(a, b)
