@@ -470,11 +470,15 @@ int sim_init(Sim* s, int device, const int_param_struct* ip, const flp_param_str
     // twice the stencil reach (|offset| + 3 nodes) plus a margin
     std::vector<int> perm;
     if (s->np > 0) {
-        double rmax = 0.0;
+        double rmax = 0.0, rmin = 1e300;
         for (int m = 0; m < s->ppp; m++) {
             const d3q19_float4 q = sphere_points[m];
-            rmax = std::max(rmax, std::sqrt((double)q.x * q.x + (double)q.y * q.y + (double)q.z * q.z));
+            const double r = std::sqrt((double)q.x * q.x + (double)q.y * q.y + (double)q.z * q.z);
+            rmax = std::max(rmax, r);
+            rmin = std::min(rmin, r);
         }
+        s->mark_rin = std::max(0.0, rmin * (1.0 - 1e-6) - 1e-6);
+        s->mark_rout = rmax * (1.0 + 1e-6) + 1e-6;
         std::vector<std::pair<uint64_t, int>> key((size_t)s->ppp);
         for (int m = 0; m < s->ppp; m++) {
             const float c[3] = {sphere_points[m].x, sphere_points[m].y, sphere_points[m].z};

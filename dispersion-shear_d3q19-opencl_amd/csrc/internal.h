@@ -183,7 +183,8 @@ struct PartArgs {
     double mass, inertia;
     double diam;
     double kcol;
-    double radius;  // brick-marking radius
+    double radius;  // brick-marking radius (cube extent)
+    double rin, rout;  // bounds of the marker offsets' lengths: bricks outside the shell are not marked
     int force_model;
     int visc_model;
     double tau0;
@@ -309,6 +310,7 @@ struct Sim {
     int* zone = nullptr;       // 2*np zone of each sphere (contact zones), by rebuild generation
     int* perm = nullptr;       // ppp marker indices in Morton order of their offsets (LDS-staged spreading)
     bool tile_ok = false;      // every periodic extent exceeds twice the spheres' stencil reach
+    double mark_rin = 0.0, mark_rout = 0.0;  // shortest / longest marker offset, with a margin
     long long h_epoch = 0;     // host count of committed epochs (one per step)
     int tofs = 0;              // step number t of the contacts = committed epoch + tofs
     double* quat = nullptr;    // 4*np

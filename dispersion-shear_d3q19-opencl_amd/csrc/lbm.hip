@@ -366,18 +366,28 @@ __device__ __forceinline__ void collide_node(const CollideArgs& a, const void* f
     const Geom& g = a.g;
     const int n = x + g.nx * (y + g.ny * z);
 
+    // The brick marks are loaded before the populations, so that their latency overlaps the
+    // population loads.  Both loads are unconditional (a flag that is off points them at the first
+    // word of fin, always readable, and the value is then ignored): a load made on one branch only
+    // makes the compiler wait for it where the branches join.
+    const int brick = (x >> 2) + g.bnx * ((y >> 2) + g.bny * (z >> 2));
+    uint32_t ep = 0, fe = 0;
+    const auto epochs = to_global<uint32_t>(a.epochs);
+    if (a.has_force || a.use_umask) ep = epochs[0] + 1u;  // this step's epoch
+    const bool force_on = a.has_force && !probe;
+    if (force_on) fe = epochs[1];
+    const D3Q19_GLOBAL uint32_t* fmp = force_on ? to_global<uint32_t>(mask_of(a.mask0, a.mask1, a.mask2, fe)) + brick
+                                                : to_global<uint32_t>(fin);
+    const D3Q19_GLOBAL uint32_t* ump = a.use_umask ? to_global<uint32_t>(mask_of(a.mask0, a.mask1, a.mask2, ep)) + brick
+                                                   : to_global<uint32_t>(fin);
+    const uint32_t fmark = *fmp, umark = *ump;
+
     R f[Q];
     gather_pre<ST, WZ>(to_global<T>(fin), g, a.fl, a.bcraw, a.use_bcraw, x, y, z, f);
 
     R gx = (R)a.fl.gconst[0], gy = (R)a.fl.gconst[1], gz = (R)a.fl.gconst[2];
-    const int brick = (x >> 2) + g.bnx * ((y >> 2) + g.bny * (z >> 2));
-    uint32_t ep = 0;
-    const auto epochs = to_global<uint32_t>(a.epochs);
-    if (a.has_force || a.use_umask) ep = epochs[0] + 1u;  // this step's epoch
-    if (a.has_force && !probe) {
-        const uint32_t fe = epochs[1];
-        const auto fm = to_global<uint32_t>(mask_of(a.mask0, a.mask1, a.mask2, fe));
-        if (fe != 0u && fm[brick] == fe) {
+    if (force_on) {
+        if (fe != 0u && fmark == fe) {
             const auto fa = to_global<long long>(a.facc) + n;
             const long long vx = fa[0], vy = fa[g.nint], vz = fa[2 * (long long)g.nint];
             if ((vx | vy | vz) != 0) {
@@ -397,7 +407,7 @@ __device__ __forceinline__ void collide_node(const CollideArgs& a, const void* f
     const R ux = (f[1] - f[2] + f[7] + f[8] + f[9] + f[10] - f[11] - f[12] - f[13] - f[14] + (R)0.5 * gx) * inv_rho;
     const R uy = (f[3] - f[4] + f[7] - f[8] + f[11] - f[12] + f[15] + f[16] - f[17] - f[18] + (R)0.5 * gy) * inv_rho;
     const R uz = (f[5] - f[6] + f[9] - f[10] + f[13] - f[14] + f[15] - f[16] + f[17] - f[18] + (R)0.5 * gz) * inv_rho;
-    if (write_u_all || (a.use_umask && to_global<uint32_t>(mask_of(a.mask0, a.mask1, a.mask2, ep))[brick] == ep)) {
+    if (write_u_all || (a.use_umask && umark == ep)) {
         const auto u = to_global<R>(a.u);
         u[n] = ux;
         u[n + g.nint] = uy;

@@ -87,7 +87,9 @@ PartArgs make_part_args(Sim* s) {
     a.inertia = (double)s->fp.ParticleMomInertia;
     a.diam = (double)s->fp.ParticleDiam;
     // marker offsets are scaled unit vectors (|r0|^2 within 1% of 1, D3Q19-OpenCL.c:597-601)
-    a.radius = 0.5 * (double)s->fp.ParticleDiam * 1.01 + 1e-3;
+    a.radius = std::max(0.5 * (double)s->fp.ParticleDiam * 1.01 + 1e-3, s->mark_rout);
+    a.rin = s->mark_rin;
+    a.rout = s->mark_rout;
     a.kcol = (double)s->fp.ParForceParams[0];
     a.force_model = s->ip.ParForceModel;
     a.visc_model = s->ip.ViscosityModel;

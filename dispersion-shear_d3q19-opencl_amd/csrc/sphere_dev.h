@@ -39,9 +39,10 @@ __host__ __device__ inline P mask_of(P m0, P m1, P m2, uint32_t e) {
 // wrap node indices, wall axes clip them.  The per-axis brick sets are built in LDS, then the block
 // stamps their product.
 template <class MP>
-__device__ void mark_sphere(const double* c3, const Geom& g, double radius, MP mk, uint32_t e, int pad) {
+__device__ void mark_sphere(const double* c3, const Geom& g, double radius, double rin, double rout, MP mk, uint32_t e, int pad) {
     __shared__ uint8_t fl[3][1024];
     __shared__ int lists[3][64];
+    __shared__ float dn[3][64], df[3][64];
     __shared__ int counts[3];
     const int nd[3] = {g.nx, g.ny, g.nz};
     const int bn[3] = {g.bnx, g.bny, g.bnz};
@@ -68,17 +69,41 @@ __device__ void mark_sphere(const double* c3, const Geom& g, double radius, MP m
         }
     }
     __syncthreads();
+    // Shell test: a brick is needed only if its nodes, widened by pad on each axis, come as close
+    // to the centre as the longest marker offset and as far as the shortest one (a stencil node
+    // lies within 2 nodes of its marker on each axis, plus 1 for a step's motion when pad = 3).
+    // Per axis, the distance from the centre to the widened brick (nearest periodic image) is
+    // kept as its nearest (dn) and farthest (df) extent.  On a periodic axis shorter than the
+    // marked span plus margins, images are ambiguous and every brick of the box is kept.
+    const bool shell = ok && rout > 0.0;
     if (threadIdx.x < 3) {
         const int d = threadIdx.x;
         int cnt = 0;
         for (int b = 0; b < bn[d] && cnt < 64; b++)
-            if (fl[d][b]) lists[d][cnt++] = b;
+            if (fl[d][b]) {
+                double lo_r = 4.0 * b - pad - c3[d], hi_r = 4.0 * b + 3 + pad - c3[d];
+                if (g.per[d]) {
+                    const double k = rint(0.5 * (lo_r + hi_r) / nd[d]);
+                    lo_r -= k * nd[d];
+                    hi_r -= k * nd[d];
+                }
+                const bool amb = g.per[d] && nd[d] < 2.0 * (rout + pad) + 16.0;
+                dn[d][cnt] = amb ? 0.0f : (float)(lo_r > 0 ? lo_r : (hi_r < 0 ? -hi_r : 0.0));
+                df[d][cnt] = amb ? 3.0e38f : (float)fmax(fabs(lo_r), fabs(hi_r));
+                lists[d][cnt++] = b;
+            }
         counts[d] = cnt;
     }
     __syncthreads();
     const int total = counts[0] * counts[1] * counts[2];
+    const float rin2 = (float)(rin * rin) * 0.999f, rout2 = (float)(rout * rout) * 1.001f;
     for (int i = threadIdx.x; i < total; i += blockDim.x) {
         const int a0 = i % counts[0], a1 = (i / counts[0]) % counts[1], a2 = i / (counts[0] * counts[1]);
+        if (shell) {
+            const float dmin2 = dn[0][a0] * dn[0][a0] + dn[1][a1] * dn[1][a1] + dn[2][a2] * dn[2][a2];
+            const float dmax2 = fminf(df[0][a0] * df[0][a0] + df[1][a1] * df[1][a1] + df[2][a2] * df[2][a2], 3.0e38f);
+            if (dmin2 > rout2 || dmax2 < rin2) continue;
+        }
         mk[lists[0][a0] + g.bnx * (lists[1][a1] + g.bny * lists[2][a2])] = e;
     }
 }
@@ -168,7 +193,7 @@ __device__ __forceinline__ void sphere_pre(const PartArgs& a, int p, int do_dyn,
     __syncthreads();
     if (do_mark) {
         const uint32_t e = a.epochs[0] + 1u;
-        mark_sphere(c3, a.g, a.radius, mask_of(a.mask0, a.mask1, a.mask2, e), e, 2);
+        mark_sphere(c3, a.g, a.radius, a.rin, a.rout, mask_of(a.mask0, a.mask1, a.mask2, e), e, 2);
     }
 }
 
@@ -292,7 +317,7 @@ __device__ __forceinline__ void sphere_turn(const PartArgs& a, int p, int do_pos
         for (int k = 0; k < 3; k++) c3n[k] = a.pos_out[3 * p + k];
     }
     __syncthreads();
-    mark_sphere(c3n, a.g, a.radius, mask_of(a.mask0, a.mask1, a.mask2, e_mark), e_mark, 3);
+    mark_sphere(c3n, a.g, a.radius, a.rin, a.rout, mask_of(a.mask0, a.mask1, a.mask2, e_mark), e_mark, 3);
 }
 
 // marks of epoch e for the current positions, dilated (the first step of a fused sequence)
@@ -301,7 +326,7 @@ __device__ __forceinline__ void sphere_prime(const PartArgs& a, int p, uint32_t 
     if (threadIdx.x == 0)
         for (int k = 0; k < 3; k++) c3p[k] = a.pos[3 * p + k];
     __syncthreads();
-    mark_sphere(c3p, a.g, a.radius, mask_of(a.mask0, a.mask1, a.mask2, e), e, 3);
+    mark_sphere(c3p, a.g, a.radius, a.rin, a.rout, mask_of(a.mask0, a.mask1, a.mask2, e), e, 3);
 }
 
 }  // namespace d3q19
