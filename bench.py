@@ -199,6 +199,8 @@ def serving(args):
             nxt += 1
         for r in srv.progress():
             done[rid_of[r.id]] = r
+            print("serving: batched %d/%d done (request %d, %d steps)" % (len(done), len(cases), rid_of[r.id], r.steps),
+                  flush=True)
         q, res, _ = srv.status()
         if q == 0 and res == 0 and nxt < len(cases):
             time.sleep(max(0.0, arr[nxt] - (d3q19.clock() - t0)))
