@@ -205,6 +205,44 @@ def test_c1_geometry_100_steps():
     run_pair(c, 100)
 
 
+@pytest.mark.parametrize("ibm", [0, 1])
+def test_marked_bricks_cover_every_stencil_across_periodic_faces(ibm):
+    """Brick marks decide where the collide writes u and consumes the spread force; on boxes whose
+    periodic extents resolve the nearest image they are cut to the bricks a marker stencil can
+    reach.  36x36 periodic (> 2 (r + 3) + 16 for d = 8), z walls, a seeded random f_eq(rho, u)
+    start (|u| <= 0.02) so that the spheres move, four spheres placed across the x face, the y face,
+    an xy corner and inside; 30 fused steps without STEP_WRITE_U (so u is written only in marked
+    bricks), fp64 build: populations within 1e-10 of the float64 oracle, positions within 1e-9
+    lattice units, velocities within 1e-9 of the wall speed.  A stencil node in an unmarked brick
+    would read a stale u or leave its force unconsumed, far outside these bars."""
+    c = perturb(case(size=(36, 36, 32), coll=1, ibm=ibm, np_=4, diam=8.0, markers=128, dist=4, seed=21, tau=0.8,
+                     uw=0.03, storage=3), seed=5)
+    kin = np.asarray(c.parKin)
+    cen = ((0.6, 18.2, 15.5), (17.9, 35.5, 16.4), (35.3, 0.4, 13.8), (18.5, 9.0, 19.2))
+    for p, xyz in enumerate(cen):
+        kin[4 * p: 4 * p + 3] = xyz  # parKin[0]: positions, one float4 per sphere
+    s = d3q19.Sim(c)
+    s.upload_case()
+    s.run(1, 30)
+    s.sync()
+    o64 = oracle_py.Oracle(c, "f64").load_case()
+    o64.step(30)
+    N = c.N
+    ef = rel_linf(interior(s.read_f64(), N, 19), interior(o64.arr(oracle_py.F_C), N, 19))
+    pos, vel, om, ff, tf = s.read_particles_f64()
+    k64 = o64.arr(oracle_py.PARKIN).reshape(4, c.np, 4)
+    L = np.array(list(c.ip.SystemSize), dtype=np.float64)
+    d = np.abs(pos.reshape(-1, 3)[: c.np] - k64[0, :, :3])
+    ep = np.minimum(d, L - d).max()
+    ev = np.abs(vel.reshape(-1, 3)[: c.np] - k64[1, :, :3]).max() / 0.03
+    print("ibm %d: f %.3e  pos %.3e  vel/U %.3e (|v| up to %.2e U)" % (ibm, ef, ep, ev, np.abs(k64[1, :, :3]).max() / 0.03),
+          flush=True)
+    assert s.check_finite() == 0
+    s.close()
+    assert ef < 1e-10
+    assert ep < 1e-9 and ev < 1e-9
+
+
 @pytest.mark.parametrize("coll,storage,visc", [(0, 0, 1), (1, 0, 2), (1, 1, 2), (0, 3, 1)])
 def test_tile_shape_changes_no_result_bit(coll, storage, visc):
     """No result may depend on which nodes share a wavefront or a block (the sphere sums of the fused
