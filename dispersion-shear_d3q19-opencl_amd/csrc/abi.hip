@@ -951,18 +951,30 @@ int d3q19_check_finite(d3q19_sim* s) {
 
 int d3q19_time_collide(d3q19_sim* s, int n, int flags, double* ms) {
     CHECK_SIM(s);
-    if (n <= 0 || !ms) return fail(D3Q19_E_INVALID, "bad timing request");
-    HIP_OK(hipEventRecord(s->ev0, s->stream));
-    for (int i = 0; i < n; i++) {
-        int rc = launch_collide(s, cur_f(s), nxt_f(s), (flags & D3Q19_STEP_WRITE_U) ? 1 : 0, 1, s->stream);
-        if (rc) return rc;
-        s->parity ^= 1;
+    if (n <= 0 || n > 4096 || !ms) return fail(D3Q19_E_INVALID, "bad timing request");
+    // each launch's own start and end (no dispatch gaps), averaged
+    std::vector<hipEvent_t> evs(2 * (size_t)n, nullptr);
+    int rc = 0;
+    for (auto& ev : evs)
+        if (hipEventCreate(&ev) != hipSuccess) {
+            rc = fail(D3Q19_E_DEVICE, "hipEventCreate failed");
+            break;
+        }
+    for (int i = 0; i < n && !rc; i++) {
+        rc = launch_collide(s, cur_f(s), nxt_f(s), (flags & D3Q19_STEP_WRITE_U) ? 1 : 0, 1, s->stream, evs[2 * i], evs[2 * i + 1]);
+        if (!rc) s->parity ^= 1;
     }
-    HIP_OK(hipEventRecord(s->ev1, s->stream));
-    HIP_OK(hipEventSynchronize(s->ev1));
-    float t = 0;
-    HIP_OK(hipEventElapsedTime(&t, s->ev0, s->ev1));
-    *ms = (double)t / n;
+    double acc = 0;
+    if (!rc && hipStreamSynchronize(s->stream) != hipSuccess) rc = fail(D3Q19_E_DEVICE, "hipStreamSynchronize failed");
+    for (int i = 0; i < n && !rc; i++) {
+        float dt = 0;
+        if (hipEventElapsedTime(&dt, evs[2 * i], evs[2 * i + 1]) != hipSuccess) rc = fail(D3Q19_E_DEVICE, "hipEventElapsedTime failed");
+        acc += dt;
+    }
+    for (auto& ev : evs)
+        if (ev) (void)hipEventDestroy(ev);
+    if (rc) return rc;
+    *ms = acc / n;
     return 0;
 }
 
@@ -984,11 +996,9 @@ int d3q19_time_steps(d3q19_sim* s, int t_first, int n, double* total_ms, double*
         for (auto& ev : evs) HIP_OK(hipEventCreate(&ev));
         double acc = 0;
         for (int i = 0; i < 16; i++) {
-            HIP_OK(hipEventRecord(evs[2 * i], s->stream));
-            rc = launch_collide(s, cur_f(s), nxt_f(s), 0, 1, s->stream);
+            rc = launch_collide(s, cur_f(s), nxt_f(s), 0, 1, s->stream, evs[2 * i], evs[2 * i + 1]);
             if (rc) break;
             s->parity ^= 1;
-            HIP_OK(hipEventRecord(evs[2 * i + 1], s->stream));
         }
         HIP_OK(hipStreamSynchronize(s->stream));
         for (int i = 0; i < 16 && !rc; i++) {

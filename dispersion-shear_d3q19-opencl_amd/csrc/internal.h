@@ -356,7 +356,9 @@ IbmArgs make_ibm_args(Sim* s);
 int marker_stride(int ppp);  // coprime with ppp, about 0.618 ppp
 PartArgs make_part_args(Sim* s);
 CellDesc make_cell_desc(Sim* s);
-int launch_collide(Sim* s, const void* fin, void* fout, int write_u_all, int use_maskU, hipStream_t st);
+// ev0/ev1 (optional): recorded at the kernel's own start and end, for timing
+int launch_collide(Sim* s, const void* fin, void* fout, int write_u_all, int use_maskU, hipStream_t st, hipEvent_t ev0 = nullptr,
+                   hipEvent_t ev1 = nullptr);
 int launch_import_f(Sim* s, const void* ref_f, int is_double, void* fdst);
 int launch_export_f(Sim* s, const void* fsrc, void* ref_out, int as_double);
 int launch_import_field(Sim* s, const float* ref, int ncomp, void* dst);

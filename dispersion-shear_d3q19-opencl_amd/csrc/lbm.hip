@@ -12,6 +12,7 @@
 // (152 B fp32, 76 B fp16/bf16, 304 B fp64); +8 B when tau is per node; +24 B read (+24 B
 // written) only at nodes of bricks holding pending spread force; +12 B u stores only at nodes
 // the next immersed-boundary pass reads (or everywhere when requested).
+#include <hip/hip_ext.h>
 #include "sphere_dev.h"
 
 namespace d3q19 {
@@ -614,21 +615,28 @@ __global__ void __launch_bounds__(256) collide_batch_kernel(BatchTables t, const
     collide_batch_tiles<ST, COLL, VT>(t, cells, sp, write_u_all, probe, blockIdx.x, gridDim.x);
 }
 
+// With events ev0/ev1, they are recorded at the kernel's own start and end (hipExtLaunchKernelGGL),
+// so their interval is the kernel's duration without the dispatch gap before it.
 template <int ST, int COLL, int VT>
-static void launch_one(const CollideArgs& a, int threads, hipStream_t st) {
+static void launch_one(const CollideArgs& a, int threads, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1) {
     const bool wz = a.g.wall == 2 && a.g.per[0] && a.g.per[1] && a.fl.tan_mask == 0;
+    if (ev0) {
+        if (wz) hipExtLaunchKernelGGL((collide_kernel<ST, COLL, VT, true>), dim3(a.nblk), dim3(threads), 0, st, ev0, ev1, 0, a);
+        else hipExtLaunchKernelGGL((collide_kernel<ST, COLL, VT, false>), dim3(a.nblk), dim3(threads), 0, st, ev0, ev1, 0, a);
+        return;
+    }
     if (wz) hipLaunchKernelGGL((collide_kernel<ST, COLL, VT, true>), dim3(a.nblk), dim3(threads), 0, st, a);
     else hipLaunchKernelGGL((collide_kernel<ST, COLL, VT, false>), dim3(a.nblk), dim3(threads), 0, st, a);
 }
 
 template <int ST>
-static void dispatch_coll(const CollideArgs& a, int coll, int vt, int threads, hipStream_t st) {
+static void dispatch_coll(const CollideArgs& a, int coll, int vt, int threads, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1) {
     if (coll == D3Q19_COLL_BGK) {
-        if (vt) launch_one<ST, D3Q19_COLL_BGK, 1>(a, threads, st);
-        else launch_one<ST, D3Q19_COLL_BGK, 0>(a, threads, st);
+        if (vt) launch_one<ST, D3Q19_COLL_BGK, 1>(a, threads, st, ev0, ev1);
+        else launch_one<ST, D3Q19_COLL_BGK, 0>(a, threads, st, ev0, ev1);
     } else {
-        if (vt) launch_one<ST, D3Q19_COLL_MRT, 1>(a, threads, st);
-        else launch_one<ST, D3Q19_COLL_MRT, 0>(a, threads, st);
+        if (vt) launch_one<ST, D3Q19_COLL_MRT, 1>(a, threads, st, ev0, ev1);
+        else launch_one<ST, D3Q19_COLL_MRT, 0>(a, threads, st, ev0, ev1);
     }
 }
 
@@ -660,14 +668,16 @@ CollideArgs make_collide_args(Sim* s, const void* fin, void* fout, int write_u_a
     return a;
 }
 
-int launch_collide(Sim* s, const void* fin, void* fout, int write_u_all, int use_maskU, hipStream_t st) {
+int launch_collide(Sim* s, const void* fin, void* fout, int write_u_all, int use_maskU, hipStream_t st, hipEvent_t ev0,
+                   hipEvent_t ev1) {
     const CollideArgs a = make_collide_args(s, fin, fout, write_u_all, use_maskU);
     const int threads = a.bx * a.by;
+    const int c = s->opt.collision, vt = s->opt.variable_viscosity;
     switch (s->storage) {
-    case D3Q19_STORE_FP16_SHIFT: dispatch_coll<D3Q19_STORE_FP16_SHIFT>(a, s->opt.collision, s->opt.variable_viscosity, threads, st); break;
-    case D3Q19_STORE_BF16_SHIFT: dispatch_coll<D3Q19_STORE_BF16_SHIFT>(a, s->opt.collision, s->opt.variable_viscosity, threads, st); break;
-    case D3Q19_STORE_FP64: dispatch_coll<D3Q19_STORE_FP64>(a, s->opt.collision, s->opt.variable_viscosity, threads, st); break;
-    default: dispatch_coll<D3Q19_STORE_FP32>(a, s->opt.collision, s->opt.variable_viscosity, threads, st); break;
+    case D3Q19_STORE_FP16_SHIFT: dispatch_coll<D3Q19_STORE_FP16_SHIFT>(a, c, vt, threads, st, ev0, ev1); break;
+    case D3Q19_STORE_BF16_SHIFT: dispatch_coll<D3Q19_STORE_BF16_SHIFT>(a, c, vt, threads, st, ev0, ev1); break;
+    case D3Q19_STORE_FP64: dispatch_coll<D3Q19_STORE_FP64>(a, c, vt, threads, st, ev0, ev1); break;
+    default: dispatch_coll<D3Q19_STORE_FP32>(a, c, vt, threads, st, ev0, ev1); break;
     }
     HIP_OK(hipGetLastError());
     s->bcraw_valid = 0;

@@ -378,8 +378,9 @@ int d3q19_solve(int device, const int_param_struct* ip, const flp_param_struct* 
 /* Monotonic clock (seconds) used for the result time stamps. */
 double d3q19_clock(void);
 
-/* Timing on the simulation's own stream (HIP events). Runs n collide_stream launches and
- * returns the mean duration per launch in milliseconds. */
+/* Timing on the simulation's own stream (HIP events recorded at each kernel's own start and end,
+ * so dispatch gaps are not counted). Runs n (1..4096) collide_stream launches and returns the
+ * mean kernel duration in milliseconds. */
 int d3q19_time_collide(d3q19_sim* sim, int n, int flags, double* ms_per_launch);
 /* Runs n full steps from t_first and returns total milliseconds plus the mean collide launch
  * duration (events recorded around every collide launch on the same stream). */
