@@ -114,7 +114,9 @@ def run(args):
                        "collide_ms": round(col16, 5), "collide_achieved_GBs": round(ach16, 1),
                        "collide_traffic": measured_traffic("traffic_bytes_per_launch_fp16"),
                        "speedup_vs_fp32": round(mlups16 / mlups, 3),
-                       "rounding": "stochastic (drag within 2e-4 of the fp64 build after 5000 steps)"},
+                       "rounding": "stochastic (drag within 2e-4 of the fp64 build after 5000 steps)",
+                       "note": "the two 80 MB fp16 lattices fit the 256 MB Infinity Cache, so this collide reads partly "
+                               "from cache; see roofline_256x256x128 for a cache-independent lattice"},
     }
     if not args.no_large:
         out["roofline_256x256x128"] = large_lattice_collide()
