@@ -18,7 +18,7 @@ using namespace d3q19;
 namespace {
 
 constexpr char MAGIC[8] = {'D', '3', 'Q', '1', '9', 'C', 'K', '1'};
-constexpr uint32_t VERSION = 1;
+constexpr uint32_t VERSION = 2;  // 2: u stored with its components interleaved per node
 
 struct Header {
     char magic[8];

@@ -119,12 +119,12 @@ __device__ __forceinline__ void ibm_marker(const IbmArgs& a, const D3Q19_GLOBAL 
         inode = sok ? sn[0] + g.nx * (sn[1] + g.ny * sn[2]) : 0;
     }
     const D3Q19_GLOBAL R* u = (const D3Q19_GLOBAL R*)a.u;
-    const long long nint = g.nint;
     double c0 = 0, c1 = 0, c2 = 0;
     if (iok) {
-        c0 = iw * (double)u[inode];
-        c1 = iw * (double)u[nint + inode];
-        c2 = iw * (double)u[2 * nint + inode];
+        const D3Q19_GLOBAL R* un = u + 3 * (long long)inode;  // components interleaved per node
+        c0 = iw * (double)un[0];
+        c1 = iw * (double)un[1];
+        c2 = iw * (double)un[2];
     }
     const double up[3] = {wave_sum(c0), wave_sum(c1), wave_sum(c2)};
     const D3Q19_GLOBAL double* om = a.omega + 3 * p;
