@@ -185,36 +185,17 @@ __device__ __forceinline__ int rel_offset(int n, int c, int nd, int per) {
 }
 
 template <class R, int MODE>
-__device__ __forceinline__ void ibm_tile_group(const IbmArgs& a, int p, int m, bool has, int lane, long long (*tile)[TILE_NODES],
-                                               int* box) {
+__device__ __forceinline__ void ibm_tile_group(const IbmArgs& a, int p, int grp, int m, bool has, int lane,
+                                               long long (*tile)[TILE_NODES]) {
     const Geom& g = a.g;
     const int nd[3] = {g.nx, g.ny, g.nz};
-    if (threadIdx.x < 3) {
-        box[threadIdx.x] = 1 << 30;
-        box[3 + threadIdx.x] = -(1 << 30);
-        // the sphere's centre node (positions lie in [0, SystemSize))
-        box[6 + threadIdx.x] = (int)floor(a.pos[3 * p + threadIdx.x]);
-    }
-    __syncthreads();
-    if (has && lane == 0) {
-        double rpp[3];
-        bool finite = true;
-        for (int d = 0; d < 3; d++) {
-            const double L = (double)g.sys[d];
-            rpp[d] = fmod((a.pos[3 * p + d] + (double)a.pts[4 * m + d]) + L, L);  // as in ibm_marker
-            if (!(rpp[d] >= 0.0 && rpp[d] < L)) finite = false;
-        }
-        if (finite)
-            for (int d = 0; d < 3; d++) {
-                const int ob = rel_offset((int)floor(rpp[d]), box[6 + d], nd[d], g.per[d]);
-                atomicMin(&box[d], ob - 1);
-                atomicMax(&box[3 + d], ob + 2);
-            }
-    }
-    __syncthreads();
-    const int lo[3] = {box[0], box[1], box[2]}, ctr[3] = {box[6], box[7], box[8]};
-    const int ex = box[3] - box[0] + 1, ey = box[4] - box[1] + 1, ez = box[5] - box[2] + 1;
-    const bool use_tile = a.tile_ok && ex > 0 && ey > 0 && ez > 0 && ex * ey * ez <= TILE_NODES;  // uniform over the block
+    // the group's box, relative to the sphere's centre node, was computed from the offsets on the
+    // host; the centre node is floor of the position (positions lie in [0, SystemSize))
+    const D3Q19_GLOBAL int* gb = a.gbox + 8 * grp;
+    const int lo[3] = {gb[0], gb[1], gb[2]};
+    const int ex = gb[3], ey = gb[4], ez = gb[5];
+    const int ctr[3] = {(int)floor(a.pos[3 * p]), (int)floor(a.pos[3 * p + 1]), (int)floor(a.pos[3 * p + 2])};
+    const bool use_tile = a.tile_ok && gb[6] <= TILE_NODES;  // uniform over the block
     const int vol = use_tile ? ex * ey * ez : 0;
     for (int i = threadIdx.x; i < vol; i += blockDim.x) {
         tile[0][i] = 0;
@@ -265,7 +246,6 @@ __device__ __forceinline__ void ibm_tile_group(const IbmArgs& a, int p, int m, b
 template <class R, int MODE>
 __global__ void __launch_bounds__(64 * TILE_MARKERS) ibm_tile_kernel(IbmArgs a) {
     __shared__ long long tile[3][TILE_NODES];
-    __shared__ int box[9];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int b = blockIdx.x;
     if (b == 0 && threadIdx.x == 0) {
@@ -276,7 +256,7 @@ __global__ void __launch_bounds__(64 * TILE_MARKERS) ibm_tile_kernel(IbmArgs a) 
     const int p = b % a.np, grp = b / a.np;
     const int j = grp * TILE_MARKERS + w;
     const bool has = j < a.ppp;
-    ibm_tile_group<R, MODE>(a, p, has ? a.perm[j] : 0, has, lane, tile, box);
+    ibm_tile_group<R, MODE>(a, p, grp, has ? a.perm[j] : 0, has, lane, tile);
 }
 
 // Batched: a persistent grid of wavefronts walks the marker list of all cells; the batch's one
@@ -329,6 +309,7 @@ IbmArgs make_ibm_args(Sim* s) {
     a.mstride = marker_stride(s->ppp > 0 ? s->ppp : 1);
     a.tile_ok = s->tile_ok ? 1 : 0;
     a.perm = gp(s->perm);
+    a.gbox = gp(s->gbox);
     a.g = s->g;
     a.np = s->np;
     a.ppp = s->ppp;

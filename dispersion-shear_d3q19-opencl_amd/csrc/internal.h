@@ -164,6 +164,7 @@ struct IbmArgs {
     int mstride;  // marker visiting order: wavefront k of sphere p takes marker (k * mstride) mod ppp
     int tile_ok;  // spreading may be staged in LDS (every periodic extent exceeds twice the stencil reach)
     const D3Q19_GLOBAL int* perm;  // ppp marker indices in Morton order of their offsets
+    const D3Q19_GLOBAL int* gbox;  // per group of TILE_MARKERS: LDS box relative to the centre node (lo[3], extent[3], volume, 0)
     double dA;
     const D3Q19_GLOBAL double* pos;
     const D3Q19_GLOBAL double* vel;
@@ -309,6 +310,7 @@ struct Sim {
     bool marks_next = false;   // marks of the next epoch exist (made from the current positions)
     int* zone = nullptr;       // 2*np zone of each sphere (contact zones), by rebuild generation
     int* perm = nullptr;       // ppp marker indices in Morton order of their offsets (LDS-staged spreading)
+    int* gbox = nullptr;       // the LDS box of each group of TILE_MARKERS markers (8 ints per group)
     bool tile_ok = false;      // every periodic extent exceeds twice the spheres' stencil reach
     double mark_rin = 0.0, mark_rout = 0.0;  // shortest / longest marker offset, with a margin
     long long h_epoch = 0;     // host count of committed epochs (one per step)
