@@ -186,18 +186,14 @@ static int upload_tables(Batch* b) {
     }
     if (b->h_sph.size() >= (1u << 19) || nmark >= (1LL << 31))
         return bfail(D3Q19_E_CAPACITY, "batch holds more than 2^19 spheres or 2^31 markers");
-    // marker list, (sphere << 12) | marker: round k takes marker (k * stride) mod ppp of every sphere
-    // with more than k markers, so wavefronts that run together spread into different spheres'
-    // nodes (the spheres of a batch rarely overlap) rather than into one sphere's shared nodes
+    // LDS-staged IBM: group list, (sphere << 12) | first position of the group in the sphere's
+    // group order; round k takes group k of every sphere with more than k groups, so that blocks
+    // that run together spread into different spheres' nodes
     {
-        std::vector<int> stride(b->h_sph.size());
-        for (size_t i = 0; i < b->h_sph.size(); i++) stride[i] = marker_stride(b->h_sph[i].w);
-        b->h_msph.reserve((size_t)nmark);
-        for (int k = 0; k < max_ppp; k++)
-            for (size_t i = 0; i < b->h_sph.size(); i++) {
-                const int ppp = b->h_sph[i].w;
-                if (k < ppp) b->h_msph.push_back(((int)i << 12) | (int)(((long long)k * stride[i]) % ppp));
-            }
+        b->h_msph.reserve((size_t)(nmark / TILE_MARKERS + b->h_sph.size()));
+        for (int k = 0; k * TILE_MARKERS < max_ppp; k++)
+            for (size_t i = 0; i < b->h_sph.size(); i++)
+                if (k * TILE_MARKERS < b->h_sph[i].w) b->h_msph.push_back(((int)i << 12) | (k * TILE_MARKERS));
     }
     (void)nslots;
     b->h_cnt.ntiles = (int)b->h_tiles.size();
@@ -209,7 +205,7 @@ static int upload_tables(Batch* b) {
     const int ec = n_ct == 0 ? 0 : (b->tuning.grid_collide > 0 ? b->tuning.grid_collide : grid_for(n_ct));
     const int ev = n_vt == 0 ? 0 : (b->tuning.grid_collide > 0 ? b->tuning.grid_collide : grid_for(n_vt));
     const int es = b->tuning.grid_sphere > 0 ? b->tuning.grid_sphere : grid_for(b->h_cnt.nsph);
-    const int ei = b->tuning.grid_ibm > 0 ? b->tuning.grid_ibm : grid_for((b->h_cnt.nmark + IBM_WAVES - 1) / IBM_WAVES);
+    const int ei = b->tuning.grid_ibm > 0 ? b->tuning.grid_ibm : grid_for(b->h_cnt.nmark);  // one group per block
     if (ec != b->eff_collide || ev != b->eff_collide_vt || es != b->eff_sphere || ei != b->eff_ibm) {
         b->eff_collide = ec;
         b->eff_collide_vt = ev;
@@ -309,7 +305,7 @@ static void default_tuning(Batch* b) {
     b->tuning.span = 1;
     b->tuning.grid_collide = 0;  // one work-list entry per block
     b->tuning.grid_sphere = 0;   // one sphere per block
-    b->tuning.grid_ibm = 0;      // one marker per wavefront
+    b->tuning.grid_ibm = 0;      // one marker group per block
     b->tuning.collide_ms = 0.0;
 }
 

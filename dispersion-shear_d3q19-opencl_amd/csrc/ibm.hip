@@ -185,8 +185,8 @@ __device__ __forceinline__ int rel_offset(int n, int c, int nd, int per) {
 }
 
 template <class R, int MODE>
-__device__ __forceinline__ void ibm_tile_group(const IbmArgs& a, int p, int grp, int m, bool has, int lane,
-                                               long long (*tile)[TILE_NODES]) {
+__device__ __forceinline__ void ibm_tile_group(const IbmArgs& a, const D3Q19_GLOBAL double* pos, const D3Q19_GLOBAL double* vel, int p,
+                                               int grp, int m, bool has, int lane, long long (*tile)[TILE_NODES]) {
     const Geom& g = a.g;
     const int nd[3] = {g.nx, g.ny, g.nz};
     // the group's box, relative to the sphere's centre node, was computed from the offsets on the
@@ -194,7 +194,7 @@ __device__ __forceinline__ void ibm_tile_group(const IbmArgs& a, int p, int grp,
     const D3Q19_GLOBAL int* gb = a.gbox + 8 * grp;
     const int lo[3] = {gb[0], gb[1], gb[2]};
     const int ex = gb[3], ey = gb[4], ez = gb[5];
-    const int ctr[3] = {(int)floor(a.pos[3 * p]), (int)floor(a.pos[3 * p + 1]), (int)floor(a.pos[3 * p + 2])};
+    const int ctr[3] = {(int)floor(pos[3 * p]), (int)floor(pos[3 * p + 1]), (int)floor(pos[3 * p + 2])};
     const bool use_tile = a.tile_ok && gb[6] <= TILE_NODES;  // uniform over the block
     const int vol = use_tile ? ex * ey * ez : 0;
     for (int i = threadIdx.x; i < vol; i += blockDim.x) {
@@ -204,7 +204,7 @@ __device__ __forceinline__ void ibm_tile_group(const IbmArgs& a, int p, int grp,
     }
     __syncthreads();
     if (has)
-        ibm_marker<R, MODE>(a, a.pos, a.vel, p, m, lane, [&](const int* sn, double w, const double* F) {
+        ibm_marker<R, MODE>(a, pos, vel, p, m, lane, [&](const int* sn, double w, const double* F) {
             int t = -1;
             if (use_tile) {
                 const int tx = rel_offset(sn[0], ctr[0], nd[0], g.per[0]) - lo[0];
@@ -256,36 +256,37 @@ __global__ void __launch_bounds__(64 * TILE_MARKERS) ibm_tile_kernel(IbmArgs a) 
     const int p = b % a.np, grp = b / a.np;
     const int j = grp * TILE_MARKERS + w;
     const bool has = j < a.ppp;
-    ibm_tile_group<R, MODE>(a, p, grp, has ? a.perm[j] : 0, has, lane, tile);
+    ibm_tile_group<R, MODE>(a, a.pos, a.vel, p, grp, has ? a.perm[j] : 0, has, lane, tile);
 }
 
-// Batched: a persistent grid of wavefronts walks the marker list of all cells; the batch's one
-// epoch counter is committed by the first lane of the grid.  (Staging the spreading in LDS, as for
-// a single simulation, measured slower here: the batch's smaller spheres give small groups, and
-// blocks of TILE_MARKERS wavefronts hide less latency than independent wavefronts.)
+// Batched, spreading staged in LDS: a persistent grid of TILE_MARKERS-wavefront blocks walks the
+// group list of all cells, (sphere << 12) | first position of the group in the sphere's group
+// order, and runs each group as ibm_tile_group does for a single simulation (integer sums: the same
+// bits as one marker per wavefront).  The batch's epoch counter is committed by the first thread.
 template <class R, int MODE>
-__global__ void __launch_bounds__(64 * IBM_WAVES) ibm_batch_kernel(BatchTables t, int sp) {
-    const int lane = threadIdx.x & 63;
+__global__ void __launch_bounds__(64 * TILE_MARKERS) ibm_batch_tile_kernel(BatchTables t, int sp) {
+    __shared__ long long tile[3][TILE_NODES];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         const uint32_t e = t.epochs[0] + 1u;
         t.epochs[0] = e;
         t.epochs[1] = e;
     }
-    const int nmark = t.cnt->nmark;
-    // the work item is uniform over the wavefront: read-first-lane keeps its indices, and so the
-    // descriptor fields read through them, in scalar registers
-    for (int gid = blockIdx.x * IBM_WAVES + (int)(threadIdx.x >> 6); gid < nmark; gid += gridDim.x * IBM_WAVES) {
-        const int v = __builtin_amdgcn_readfirstlane(t.msph[__builtin_amdgcn_readfirstlane(gid)]);
+    const int ngrp = t.cnt->nmark;
+    for (int k = blockIdx.x; k < ngrp; k += gridDim.x) {  // uniform over the block
+        const int v = __builtin_amdgcn_readfirstlane(t.msph[k]);
         const int4 d = t.sph[v >> 12];  // cell, local sphere, first global marker, markers per sphere
         const int cell = __builtin_amdgcn_readfirstlane(d.x), sph = __builtin_amdgcn_readfirstlane(d.y);
+        const int ppp = __builtin_amdgcn_readfirstlane(d.w);
         const CellDesc& cd = t.cells[cell];
         const IbmArgs& ia = cd.i;
         const int par = __builtin_amdgcn_readfirstlane(cd.par);
+        const int j0 = v & 4095, j = j0 + w;
+        const bool has = j < ppp;
         // x(t+1), v(t+1): written by this step's turn
-        ibm_marker<R, MODE>(ia, cd.posb[par ^ sp ^ 1], cd.velb[par ^ sp ^ 1], sph, v & 4095, lane,
-                            [&](const int* sn, double w, const double* F) {
-                                spread_add(ia.facc, ia.g.nint, sn[0] + ia.g.nx * (sn[1] + ia.g.ny * sn[2]), w, F);
-                            });
+        ibm_tile_group<R, MODE>(ia, cd.posb[par ^ sp ^ 1], cd.velb[par ^ sp ^ 1], sph, j0 / TILE_MARKERS, has ? ia.perm[j] : 0, has,
+                                lane, tile);
+        __syncthreads();  // the next group reuses the tile
     }
 }
 
@@ -358,13 +359,13 @@ int launch_ibm(Sim* s, hipStream_t st) {
 
 int launch_ibm_batch(const BatchLaunch& L, int sp, hipStream_t st) {
     const bool dbl = L.storage == D3Q19_STORE_FP64;
-    const dim3 grid(L.grid_ibm), block(64 * IBM_WAVES);
+    const dim3 grid(L.grid_ibm), block(64 * TILE_MARKERS);
     if (L.ibm_mode == D3Q19_IBM_PESKIN4) {
-        if (dbl) hipLaunchKernelGGL((ibm_batch_kernel<double, D3Q19_IBM_PESKIN4>), grid, block, 0, st, L.t, sp);
-        else hipLaunchKernelGGL((ibm_batch_kernel<float, D3Q19_IBM_PESKIN4>), grid, block, 0, st, L.t, sp);
+        if (dbl) hipLaunchKernelGGL((ibm_batch_tile_kernel<double, D3Q19_IBM_PESKIN4>), grid, block, 0, st, L.t, sp);
+        else hipLaunchKernelGGL((ibm_batch_tile_kernel<float, D3Q19_IBM_PESKIN4>), grid, block, 0, st, L.t, sp);
     } else {
-        if (dbl) hipLaunchKernelGGL((ibm_batch_kernel<double, D3Q19_IBM_LINEAR_SMOOTHED>), grid, block, 0, st, L.t, sp);
-        else hipLaunchKernelGGL((ibm_batch_kernel<float, D3Q19_IBM_LINEAR_SMOOTHED>), grid, block, 0, st, L.t, sp);
+        if (dbl) hipLaunchKernelGGL((ibm_batch_tile_kernel<double, D3Q19_IBM_LINEAR_SMOOTHED>), grid, block, 0, st, L.t, sp);
+        else hipLaunchKernelGGL((ibm_batch_tile_kernel<float, D3Q19_IBM_LINEAR_SMOOTHED>), grid, block, 0, st, L.t, sp);
     }
     HIP_OK(hipGetLastError());
     return 0;

@@ -250,7 +250,7 @@ struct BatchCounts {
     int ntiles_ct;  // the first ntiles_ct entries belong to cells of constant tau
     int span;       // tiles per entry
     int nsph;     // spheres over all cells
-    int nmark;    // markers over all cells
+    int nmark;    // entries of the group list (groups of TILE_MARKERS markers over all spheres)
     int ncells;
 };
 
@@ -258,7 +258,7 @@ struct BatchTables {
     const CellDesc* cells;
     const uint2* tiles;      // collide work list: (cell, first node); an entry covers span tiles
     const int4* sph;         // (cell, local sphere, first global marker, markers per sphere)
-    const int* msph;         // marker list: (global sphere << 12) | marker index within the sphere
+    const int* msph;         // group list: (global sphere << 12) | first position of the group in the sphere's group order
     const BatchCounts* cnt;
     uint32_t* epochs;        // the batch's epoch counter (shared by all its cells)
 };
