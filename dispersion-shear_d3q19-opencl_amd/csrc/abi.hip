@@ -47,6 +47,9 @@ static int validate(const int_param_struct* ip, const flp_param_struct* fp, cons
         return fail(D3Q19_E_INVALID, "more than one pair of velocity-boundary faces");
     const long long nint = (long long)(ip->LatticeSize[0] - 2) * (ip->LatticeSize[1] - 2) * (ip->LatticeSize[2] - 2);
     if (nint >= (1LL << 31)) return fail(D3Q19_E_UNSUPPORTED, "more than 2^31 interior nodes");
+    // kernels address a population plane by a 32-bit byte offset (no options: fp32 storage)
+    if (nint * (long long)pop_size(opt ? opt->storage : 0) >= (1LL << 32))
+        return fail(D3Q19_E_UNSUPPORTED, "a population plane above 4 GiB (interior nodes x bytes per value)");
     if (!(fp->NewtonianTau > 0.5f)) return fail(D3Q19_E_INVALID, "NewtonianTau must exceed 0.5");
     if (ip->NumParticles < 0) return fail(D3Q19_E_INVALID, "NumParticles < 0");
     if (ip->NumParticles > 0) {

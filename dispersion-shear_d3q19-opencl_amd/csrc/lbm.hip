@@ -33,19 +33,19 @@ template <> struct Pop<D3Q19_STORE_FP32> {
     using T = float;
     using R = float;
     static constexpr bool SR = false;
-    template <int QI, class PT> __device__ static R ld(PT p, unsigned off) { return p[off]; }
-    template <int QI, class PT> __device__ static double ldd(PT p, unsigned off) { return (double)p[off]; }
-    template <int QI, class PT> __device__ static void st(PT p, long long off, R v) { p[off] = v; }
-    template <int QI, class PT> __device__ static void st_sr(PT p, long long off, R v, uint32_t) { p[off] = v; }
+    template <int QI, class PT> __device__ static R ld(PT p, unsigned off) { return *plane_at(p, off); }
+    template <int QI, class PT> __device__ static double ldd(PT p, unsigned off) { return (double)*plane_at(p, off); }
+    template <int QI, class PT> __device__ static void st(PT p, unsigned off, R v) { *plane_at(p, off) = v; }
+    template <int QI, class PT> __device__ static void st_sr(PT p, unsigned off, R v, uint32_t) { *plane_at(p, off) = v; }
 };
 template <> struct Pop<D3Q19_STORE_FP64> {
     using T = double;
     using R = double;
     static constexpr bool SR = false;
-    template <int QI, class PT> __device__ static R ld(PT p, unsigned off) { return p[off]; }
-    template <int QI, class PT> __device__ static double ldd(PT p, unsigned off) { return p[off]; }
-    template <int QI, class PT> __device__ static void st(PT p, long long off, R v) { p[off] = v; }
-    template <int QI, class PT> __device__ static void st_sr(PT p, long long off, R v, uint32_t) { p[off] = v; }
+    template <int QI, class PT> __device__ static R ld(PT p, unsigned off) { return *plane_at(p, off); }
+    template <int QI, class PT> __device__ static double ldd(PT p, unsigned off) { return *plane_at(p, off); }
+    template <int QI, class PT> __device__ static void st(PT p, unsigned off, R v) { *plane_at(p, off) = v; }
+    template <int QI, class PT> __device__ static void st_sr(PT p, unsigned off, R v, uint32_t) { *plane_at(p, off) = v; }
 };
 // binary16 of the relative deviation (f_q - w_q)/w_q scaled by 2^14: deviations down to ~1e-8
 // stay normal numbers and the headroom reaches |dev| = 4.
@@ -55,26 +55,26 @@ template <> struct Pop<D3Q19_STORE_FP16_SHIFT> {
     template <int QI, class PT> __device__ static R ld(PT p, unsigned off) {
         constexpr float w = (float)wq(QI);
         constexpr float sc = (float)(wq(QI) / 16384.0);
-        return fmaf((float)p[off], sc, w);
+        return fmaf((float)*plane_at(p, off), sc, w);
     }
     template <int QI, class PT> __device__ static double ldd(PT p, unsigned off) {
-        return wq(QI) + (double)p[off] * (wq(QI) / 16384.0);
+        return wq(QI) + (double)*plane_at(p, off) * (wq(QI) / 16384.0);
     }
     static constexpr bool SR = true;
-    template <int QI, class PT> __device__ static void st(PT p, long long off, R v) {
+    template <int QI, class PT> __device__ static void st(PT p, unsigned off, R v) {
         constexpr float w = (float)wq(QI);
         constexpr float isc = (float)(16384.0 / wq(QI));
-        p[off] = (_Float16)((v - w) * isc);
+        *plane_at(p, off) = (_Float16)((v - w) * isc);
     }
     // the 13 magnitude bits that binary16 drops, plus 13 random bits, carry into the kept ones with
     // probability equal to the dropped fraction; the truncated value then converts exactly (normal
     // binary16 range; |f - w|/w below 2^-28 also rounds to nearest, which is immaterial).  The
     // collide measured 2% faster this way than with v_cvt_sr_f16_f32.
-    template <int QI, class PT> __device__ static void st_sr(PT p, long long off, R v, uint32_t r) {
+    template <int QI, class PT> __device__ static void st_sr(PT p, unsigned off, R v, uint32_t r) {
         constexpr float w = (float)wq(QI);
         constexpr float isc = (float)(16384.0 / wq(QI));
         const uint32_t b = (__float_as_uint((v - w) * isc) + (r >> 19)) & 0xFFFFE000u;
-        p[off] = (_Float16)__uint_as_float(b);
+        *plane_at(p, off) = (_Float16)__uint_as_float(b);
     }
 };
 // bfloat16 of f_q - w_q
@@ -83,21 +83,21 @@ template <> struct Pop<D3Q19_STORE_BF16_SHIFT> {
     using R = float;
     template <int QI, class PT> __device__ static R ld(PT p, unsigned off) {
         constexpr float w = (float)wq(QI);
-        return w + bf16_to_f32(p[off]);
+        return w + bf16_to_f32(*plane_at(p, off));
     }
     template <int QI, class PT> __device__ static double ldd(PT p, unsigned off) {
-        return wq(QI) + (double)bf16_to_f32(p[off]);
+        return wq(QI) + (double)bf16_to_f32(*plane_at(p, off));
     }
     static constexpr bool SR = true;
-    template <int QI, class PT> __device__ static void st(PT p, long long off, R v) {
+    template <int QI, class PT> __device__ static void st(PT p, unsigned off, R v) {
         constexpr float w = (float)wq(QI);
-        p[off] = f32_to_bf16(v - w);
+        *plane_at(p, off) = f32_to_bf16(v - w);
     }
     // the 16 discarded magnitude bits plus 16 random bits carry into the kept ones with
     // probability equal to the discarded fraction (finite inputs)
-    template <int QI, class PT> __device__ static void st_sr(PT p, long long off, R v, uint32_t r) {
+    template <int QI, class PT> __device__ static void st_sr(PT p, unsigned off, R v, uint32_t r) {
         constexpr float w = (float)wq(QI);
-        p[off] = (uint16_t)((__float_as_uint(v - w) + (r >> 16)) >> 16);
+        *plane_at(p, off) = (uint16_t)((__float_as_uint(v - w) + (r >> 16)) >> 16);
     }
 };
 
@@ -225,7 +225,7 @@ __device__ __forceinline__ void gather_pre(PIn fin, const Geom& g, const FluidPa
         const int ox = cx(q) == 0 ? 0 : (cx(q) > 0 ? oxm : oxp);
         const int oy = cy(q) == 0 ? 0 : (cy(q) > 0 ? oym : oyp);
         const int oz = cz(q) == 0 ? 0 : (cz(q) > 0 ? ozm : ozp);
-        const auto base = fin + (long long)q * qs;
+        const auto base = (const D3Q19_GLOBAL typename P::T*)fin + (long long)q * qs;
         if constexpr (std::is_same<RO, typename P::R>::value) f[q] = P::template ld<q>(base, (unsigned)((int)n + ox + oy + oz));
         else f[q] = P::template ldd<q>(base, (unsigned)((int)n + ox + oy + oz));
     });
@@ -508,12 +508,12 @@ __device__ __forceinline__ void collide_node(const CollideArgs& a, const void* f
         const uint32_t h = fmix32(x);
         Unroll<0, Q>::run([&](auto qc) __attribute__((always_inline)) {
             constexpr int q = decltype(qc)::value;
-            P::template st_sr<q>(out, (long long)q * g.qstride + n, fo[q], sr_word<q>(h));
+            P::template st_sr<q>(out + (long long)q * g.qstride, (unsigned)n, fo[q], sr_word<q>(h));
         });
     } else {
         Unroll<0, Q>::run([&](auto qc) __attribute__((always_inline)) {
             constexpr int q = decltype(qc)::value;
-            P::template st<q>(out, (long long)q * g.qstride + n, fo[q]);
+            P::template st<q>(out + (long long)q * g.qstride, (unsigned)n, fo[q]);
         });
     }
 }
@@ -785,7 +785,7 @@ __global__ void import_kernel(const IN* __restrict__ ref, Geom g, int Nx, int Ny
         }
         if (!ok) { s3[0] = x; s3[1] = y; s3[2] = z; }
         const long long ri = (s3[0] + 1) + (long long)Nx * ((s3[1] + 1) + (long long)Ny * (s3[2] + 1));
-        P::template st<q>(dst, (long long)q * g.qstride + idx, (R)ref[q * nn + ri]);
+        P::template st<q>(to_global<typename P::T>(dst) + (long long)q * g.qstride, (unsigned)idx, (R)ref[q * nn + ri]);
     });
     if (g.wall >= 0 && raw) {
         const int A = g.wall;
@@ -930,7 +930,7 @@ __global__ void fill_eq_kernel(typename Pop<ST>::T* f, Geom g) {
     if (idx >= g.nint) return;
     Unroll<0, Q>::run([&](auto qc) __attribute__((always_inline)) {
         constexpr int q = decltype(qc)::value;
-        P::template st<q>(f, (long long)q * g.qstride + idx, (R)wq(q));
+        P::template st<q>(to_global<typename P::T>(f) + (long long)q * g.qstride, (unsigned)idx, (R)wq(q));
     });
 }
 int launch_fill_equilibrium(Sim* s, void* f) {

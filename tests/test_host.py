@@ -291,6 +291,23 @@ def test_invalid_parameters_rejected_before_device_use():
     assert rc == -1 and b"rounding" in L.d3q19_last_error()
 
 
+def test_population_plane_above_4_gib_rejected_before_device_use():
+    """Kernels address a population plane by a 32-bit byte offset, so a lattice whose plane (interior
+    nodes x bytes per stored value) reaches 4 GiB is refused: 1024^3 interior nodes in fp32, 512 x
+    1024 x 1024 in fp64."""
+    L = d3q19.lib()
+    h = C.c_void_p()
+    for nx, storage in ((1024, 0), (512, 3)):
+        # a small case, its lattice sizes then raised (the host-side arrays stay small)
+        c = d3q19.Case(lines=["total_lattice_size 12 12 13", "boundary_conditions_xyz 0 0 1"])
+        for d, n in enumerate((nx, 1024, 1024)):
+            c.ip.LatticeSize[d] = n + 2 + (d == 2)
+            c.ip.SystemSize[d] = n
+        c.opt.storage = storage
+        rc = L.d3q19_create(0, C.byref(c.ip), C.byref(c.fp), C.byref(c.opt), None, C.byref(h))
+        assert rc == -4 and b"4 GiB" in L.d3q19_last_error()
+
+
 def test_parameter_checking_rejects_two_wall_axes():
     with pytest.raises(ValueError):
         d3q19.Case(lines=["boundary_conditions_xyz 1 0 1"])
