@@ -252,6 +252,33 @@ def serving(args):
             "max_rel_diff_wall_stress_vs_sequential": agree, "same_step_count": steps_equal}
 
 
+def large(args):
+    """The 256x256x128 Casson workload (500 spheres r=5 x 128 markers, 8.45 M interior nodes, five
+    times the Infinity Cache in fp32), fp32 and fp16-shifted populations: the collide alone (16
+    event-timed launches) and the whole step (fused step kernel + IBM pass, graph-replayed, timed
+    with HIP events on the simulation's stream)."""
+    import d3q19
+    out = {}
+    for st, name in ((0, "fp32"), (1, "fp16_shift")):
+        c = d3q19.Case(lines=c3_deck(storage=st))
+        nodes = (c.N[0] - 2) * (c.N[1] - 2) * (c.N[2] - 2)
+        s = d3q19.Sim(c)
+        s.write_particles(c.parKin)
+        s.run(1, args.warmup)
+        s.sync()
+        col = s.time_collide(16)
+        tot, _ = s.time_steps(args.warmup + 1, args.steps)
+        assert s.check_finite() == 0, "non-finite state"
+        s.close()
+        ms = tot / args.steps
+        out[name] = {"step_ms": round(ms, 5), "mlups": round(nodes / (ms * 1e-3) / 1e6, 1), "collide_ms": round(col, 5),
+                     "collide_achieved_GBs": round((152.0 if st == 0 else 76.0) * nodes / (col * 1e-3) / 1e9, 1)}
+        print("large: %s done" % name, flush=True)
+    out["fp16_speedup_step"] = round(out["fp32"]["step_ms"] / out["fp16_shift"]["step_ms"], 3)
+    out["interior_nodes"] = nodes
+    return out
+
+
 def precision(args):
     """The 256x256x128 Casson precision workload: fp32 / fp16-shifted / bf16-shifted populations
     against the fp64 build, errors of u, drag coefficient and wall stress."""
@@ -296,9 +323,9 @@ def main():
     ap.add_argument("--no-large", action="store_true", help="skip the 256x256x128 collide roofline")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="only the launches profiled by the HBM-traffic counter passes")
-    ap.add_argument("--mode", default="step", choices=["step", "ensemble", "serving", "precision"],
-                    help="step: the per-step benchmark line (default); ensemble / serving / precision: the batched, "
-                         "serving and storage-precision workloads, one JSON line each")
+    ap.add_argument("--mode", default="step", choices=["step", "ensemble", "serving", "precision", "large"],
+                    help="step: the per-step benchmark line (default); ensemble / serving / precision / large: the "
+                         "batched, serving, storage-precision and 256x256x128 workloads, one JSON line each")
     ap.add_argument("--ens-steps", type=int, default=1000)
     ap.add_argument("--ens-warmup", type=int, default=10)
     ap.add_argument("--ens-seq-steps", type=int, default=100)
@@ -325,6 +352,8 @@ def main():
         print(json.dumps({"workload": "serving", **serving(args)}), flush=True)
     elif args.mode == "precision":
         print(json.dumps({"workload": "precision study", **precision(args)}), flush=True)
+    elif args.mode == "large":
+        print(json.dumps({"workload": "256x256x128 Casson shear cell, 500 spheres", **large(args)}), flush=True)
     else:
         run(args)
 
