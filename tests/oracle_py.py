@@ -24,11 +24,15 @@ def ensure_built():
             break
 
 
-def load(sfx):
-    if sfx in _libs:
-        return _libs[sfx]
-    ensure_built()
-    L = C.CDLL(os.path.join(BUILD, "liboracle_%s.so" % sfx))
+def load(sfx, path=None):
+    """The oracle library of precision sfx ("f32" / "f64"); `path` loads another build of the same
+    source (the pin tests load deliberately broken builds to show that their checks fail)."""
+    key = (sfx, path)
+    if key in _libs:
+        return _libs[key]
+    if path is None:
+        ensure_built()
+    L = C.CDLL(path or os.path.join(BUILD, "liboracle_%s.so" % sfx))
     V = C.c_void_p
     getattr(L, "orc_create_" + sfx).restype = V
     getattr(L, "orc_create_" + sfx).argtypes = [V, V, V]
@@ -54,16 +58,16 @@ def load(sfx):
     getattr(L, "orc_steps_done_" + sfx).argtypes = [V]
     getattr(L, "orc_steps_done_" + sfx).restype = C.c_int
     getattr(L, "orc_pair_force_of_" + sfx).argtypes = [V, C.c_int, C.c_int]
-    _libs[sfx] = L
+    _libs[key] = L
     return L
 
 
 class Oracle:
     """One oracle simulation; arrays are numpy views on the oracle's own memory."""
 
-    def __init__(self, case, precision="f32", opt=None):
+    def __init__(self, case, precision="f32", opt=None, lib_path=None):
         self.sfx = precision
-        self.L = load(precision)
+        self.L = load(precision, lib_path)
         self.dtype = np.float32 if precision == "f32" else np.float64
         o = opt if opt is not None else case.opt
         self.h = getattr(self.L, "orc_create_" + precision)(C.byref(case.ip), C.byref(case.fp), C.byref(o))
