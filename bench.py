@@ -1,14 +1,19 @@
 """Benchmark: full D3Q19 IBM shear-cell steps on one MI355X.
 
-Workload (default): the 128^3 power-law configuration -- box 128x128x128 between walls moving at
-+-0.02 (LatticeSize 130x130x131), power-law fluid n = 0.7 with K = nu*g0^(1-n), nu from tau 0.9 and
-g0 = 2U/H, local tau from the strain rate clamped to [0.51, 5], BGK collision, 32 spheres of radius
-6 with 192 markers each at seeded uniform-random centres (surface gap >= 2), Peskin 4-point IBM,
-fp32 populations.  A step is the whole time step (dynamics, collide+stream, IBM, contacts).
+Workload (default line): the 256x256x128 Casson shear cell -- box 256x256x128 between walls moving
+at +-0.02 (LatticeSize 258x258x131), Casson fluid with yield stress 1e-5 and plastic viscosity of
+tau 0.7, Papanastasiou m = 1e4, local tau from the strain rate clamped to [0.505, 5], BGK collision,
+500 spheres of diameter 10 with 128 markers each at seeded centres, Peskin 4-point IBM, fp32
+populations.  A step is the whole time step: the step kernel (collide+stream with the spheres'
+dynamics, force sums and contacts in extra blocks of the same launch) and the IBM pass.  The
+lattice (1.3 GB of fp32 populations) is five times the 256 MB Infinity Cache, so the bandwidth is
+HBM bandwidth.
 
-Prints one JSON line: MLUPS of the fp32 step (value), the fp16-shifted-storage step, the
-collide+stream kernel's HBM roofline (per-launch event timing on the simulation's stream), and a
-CPU baseline (the oracle, OpenMP, on a bounded sample of the same workload).
+Prints one JSON line: MLUPS of the fp32 step (value) over the timed steps, the step kernel's HBM
+roofline (its own start/end HIP events on the simulation's stream, inside the timed region), the
+attainable bandwidth of the same access pattern without arithmetic, the fp16-shifted-storage step
+on the same workload, the 128^3 power-law workload, a bounded serving stream against sequential
+solves, and a CPU baseline (the oracle, OpenMP, on a bounded sample of the same workload).
 """
 import argparse
 import json
@@ -21,12 +26,12 @@ sys.path.insert(0, os.path.join(ROOT, "dispersion-shear_d3q19-opencl_amd"))
 sys.path.insert(0, os.path.join(ROOT, "tests"))
 
 HBM_PEAK_GBS = 8000.0
-TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "hbm_traffic_collide_c2_powerlaw.md")
+TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "hbm_traffic_step_c3_casson.md")
 
 
 def measured_traffic(key):
-    """HBM bytes per collide launch from the committed counter-pass summary (FETCH_SIZE and
-    WRITE_SIZE, calibrated with a copy kernel of known size), or None when absent."""
+    """HBM bytes per launch from the committed counter-pass summary (FETCH_SIZE and WRITE_SIZE,
+    calibrated with a copy kernel of known size), or None when absent."""
     try:
         for line in open(TRAFFIC_SUMMARY):
             if line.startswith(key + ":"):
@@ -37,113 +42,132 @@ def measured_traffic(key):
 
 
 METRIC = "MLUPS & %HBM roofline, FP32 vs FP16-store; shear-cell solves/s + p99 latency"
+C3_NAME = ("256x256x128 Casson shear cell (LatticeSize 258x258x131, walls +-0.02, yield stress 1e-5, tau_p 0.7, "
+           "Papanastasiou m=1e4, tau clamp [0.505,5]), 500 spheres d=10 x 128 markers, BGK, Peskin-4 IBM")
 
 
 from workloads import c2_deck, c3_deck, c4_decks, c5_requests, poisson_arrivals, precision_study  # noqa: E402
 
 
+def host_cpu():
+    """nproc, OMP_NUM_THREADS and the CPU model of the host this runs on."""
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"nproc": len(os.sched_getaffinity(0)), "omp_num_threads": os.environ.get("OMP_NUM_THREADS"), "cpu_model": model}
+
+
 def cpu_baseline(sample_steps):
-    """The oracle (float32, OpenMP) on `sample_steps` steps of the same workload."""
+    """The oracle (float32, OpenMP) on `sample_steps` steps of the same 256x256x128 workload."""
     import d3q19
     import oracle_py
-    c = d3q19.Case(lines=c2_deck())
+    c = d3q19.Case(lines=c3_deck())
     o = oracle_py.Oracle(c, "f32").load_case()
     o.step(1)  # warm-up (page-in)
     t0 = time.perf_counter()
     o.step(sample_steps)
     dt = time.perf_counter() - t0
+    o.close()
     nodes = (c.N[0] - 2) * (c.N[1] - 2) * (c.N[2] - 2)
-    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return {"value": round(nodes * sample_steps / dt / 1e6, 3), "unit": "MLUPS", "cores": cores, "kind": "port",
-            "sample": "%d steps of the same 128^3 power-law, 32-sphere workload (oracle/d3q19_oracle.c, float32, "
-                      "OpenMP), %.1f s" % (sample_steps, dt)}
+    h = host_cpu()
+    threads = int(h["omp_num_threads"]) if h["omp_num_threads"] else h["nproc"]
+    return {"value": round(nodes * sample_steps / dt / 1e6, 3), "unit": "MLUPS", "cores": threads, "kind": "port",
+            "sample": "%d steps of the same 256x256x128 Casson 500-sphere workload (oracle/d3q19_oracle.c, float32, "
+                      "OpenMP), %.1f s" % (sample_steps, dt), **h}
+
+
+def step_run(lines, steps, warmup, graph_steps=0):
+    """Warm-up steps, then `steps` timed real steps whose kernels carry their own start/end events;
+    optionally as many more steps replayed from the HIP graph (d3q19_run), timed on the host
+    around synchronised calls."""
+    import d3q19
+    c = d3q19.Case(lines=lines)
+    nodes = (c.N[0] - 2) * (c.N[1] - 2) * (c.N[2] - 2)
+    s = d3q19.Sim(c)
+    if c.np:
+        s.write_particles(c.parKin)
+    s.run(1, warmup)
+    s.sync()
+    total, kstep, kibm = s.time_steps(1 + warmup, steps)
+    out = {"nodes": nodes, "ms_per_step": total / steps, "mlups": nodes * steps / (total * 1e-3) / 1e6,
+           "step_kernel_ms": kstep, "ibm_ms": kibm}
+    if graph_steps:
+        t0 = time.perf_counter()
+        s.run(1 + warmup + steps, graph_steps)
+        s.sync()
+        g = (time.perf_counter() - t0) * 1e3 / graph_steps
+        out["graph_ms_per_step"] = g
+        out["graph_mlups"] = nodes / (g * 1e-3) / 1e6
+    assert s.check_finite() == 0, "non-finite state or a sphere moved a lattice unit in one step"
+    s.close()
+    return out
 
 
 def run(args):
     import d3q19
-    c = d3q19.Case(lines=c2_deck(storage=0))
-    nodes = (c.N[0] - 2) * (c.N[1] - 2) * (c.N[2] - 2)
-    sim = d3q19.Sim(c)
-    sim.upload_case()
-    sim.run(1, args.warmup)
-    sim.sync()
-    total_ms, collide_ms = sim.time_steps(1 + args.warmup, args.steps)
-    sim.sync()
-    assert sim.check_finite() == 0, "non-finite state"
-    mlups = nodes * args.steps / (total_ms * 1e-3) / 1e6
-    # collide+stream roofline: algorithmic bytes 19 loads + 19 stores of fp32 per node
+    r32 = step_run(c3_deck(storage=0), args.steps, args.warmup, graph_steps=args.graph_steps)
+    nodes = r32["nodes"]
+    # step-kernel roofline: algorithmic bytes 19 loads + 19 stores of fp32 per interior node
     bytes_per_launch = 152.0 * nodes
-    achieved = bytes_per_launch / (collide_ms * 1e-3) / 1e9
-    sim.close()
-
-    c16 = d3q19.Case(lines=c2_deck(storage=1))
-    s16 = d3q19.Sim(c16)
-    s16.upload_case()
-    s16.run(1, args.warmup)
-    s16.sync()
-    t16, col16 = s16.time_steps(1 + args.warmup, args.steps)
-    s16.close()
-    mlups16 = nodes * args.steps / (t16 * 1e-3) / 1e6
-    ach16 = 76.0 * nodes / (col16 * 1e-3) / 1e9
-
+    achieved = bytes_per_launch / (r32["step_kernel_ms"] * 1e-3) / 1e9
+    attain, _ = d3q19.stream_probe(256, 256, 128, streams=19, width=1, pull=True, launches=20)
+    r16 = step_run(c3_deck(storage=1), args.steps, args.warmup, graph_steps=args.graph_steps)
+    ach16 = 76.0 * nodes / (r16["step_kernel_ms"] * 1e-3) / 1e9
     out = {
         "metric": METRIC,
-        "value": round(mlups, 1),
+        "value": round(r32["mlups"], 1),
         "unit": "MLUPS",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(total_ms / args.steps, 5),
+        "ms_per_step": round(r32["ms_per_step"], 5),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded sphere placement, fluid at rest)",
-        "config": {"workload": "128^3 power-law shear cell (LatticeSize 130x130x131, walls +-0.02, n=0.7, "
-                               "tau clamp [0.51,5]), 32 spheres r=6 x 192 markers, BGK, Peskin-4 IBM, fp32 populations",
-                   "interior_nodes": nodes},
+        "config": {"workload": C3_NAME + ", fp32 populations", "interior_nodes": nodes,
+                   "step": "step kernel (collide+stream + spheres' turn) + IBM pass, timed real steps"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": measured_traffic("traffic_bytes_per_launch_fp32"),
-                     "traffic_source": "profiles/hbm_traffic_collide_c2_powerlaw.md (rocprofv3 FETCH_SIZE x2.0, WRITE_SIZE x1.0, "
-                                       "scaled by a 1 GiB copy)",
+                     "traffic_source": "profiles/hbm_traffic_step_c3_casson.md (rocprofv3 FETCH_SIZE and WRITE_SIZE of the "
+                                       "step kernel, each scaled by the same counter's reading for a copy of known size)",
                      "algorithmic_bytes": int(bytes_per_launch),
-                     "kernel": "collide_kernel (fused collide+stream), 152 B/node algorithmic",
-                     "kernel_ms": round(collide_ms, 5)},
-        "fp16_store": {"mlups": round(mlups16, 1), "ms_per_step": round(t16 / args.steps, 5),
-                       "collide_ms": round(col16, 5), "collide_achieved_GBs": round(ach16, 1),
-                       "collide_traffic": measured_traffic("traffic_bytes_per_launch_fp16"),
-                       "speedup_vs_fp32": round(mlups16 / mlups, 3),
-                       "rounding": "stochastic (drag within 2e-4 of the fp64 build after 5000 steps)",
-                       "note": "the two 80 MB fp16 lattices fit the 256 MB Infinity Cache, so this collide reads partly "
-                               "from cache; see roofline_256x256x128 for a cache-independent lattice"},
+                     "kernel": "step_kernel_w6 (fused collide+stream with the spheres' turn), 152 B/node algorithmic",
+                     "kernel_ms": round(r32["step_kernel_ms"], 5),
+                     "attainable_GBs": round(attain, 1), "frac_of_attainable": round(achieved / attain, 4),
+                     "attainable_note": "d3q19_stream_probe: the same 19+19 fp32 plane pattern at the pull offsets with no "
+                                        "arithmetic, 256x256x128 box"},
+        "ibm_ms": round(r32["ibm_ms"], 5),
+        "graph_replay": {"ms_per_step": round(r32.get("graph_ms_per_step", 0.0), 5),
+                         "mlups": round(r32.get("graph_mlups", 0.0), 1)},
+        "fp16_store": {"mlups": round(r16["mlups"], 1), "ms_per_step": round(r16["ms_per_step"], 5),
+                       "step_kernel_ms": round(r16["step_kernel_ms"], 5), "ibm_ms": round(r16["ibm_ms"], 5),
+                       "step_kernel_achieved_GBs": round(ach16, 1), "frac": round(ach16 / HBM_PEAK_GBS, 4),
+                       "traffic": measured_traffic("traffic_bytes_per_launch_fp16"),
+                       "speedup_vs_fp32": round(r16["mlups"] / r32["mlups"], 3),
+                       "rounding": "stochastic; drag within 2.2e-4 of the fp64 build after 3000 steps of this workload "
+                                   "(bench.py --mode precision)"},
     }
-    if not args.no_large:
-        out["roofline_256x256x128"] = large_lattice_collide()
+    if not args.no_c2:
+        c2 = {}
+        for st, name in ((0, "fp32"), (1, "fp16_shift")):
+            r = step_run(c2_deck(storage=st), args.steps, args.warmup)
+            c2[name] = {"mlups": round(r["mlups"], 1), "ms_per_step": round(r["ms_per_step"], 5),
+                        "step_kernel_ms": round(r["step_kernel_ms"], 5)}
+        c2["workload"] = "128^3 power-law shear cell, 32 spheres d=12 x 192 markers (lattices partly Infinity-Cache resident)"
+        out["c2_128cubed"] = c2
+    if not args.no_serving:
+        out["serving"] = serving_summary(args)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cpu_steps)
     print(json.dumps(out), flush=True)
-
-
-def large_lattice_collide():
-    """The collide on the 256x256x128 Casson lattice with 500 spheres (8.45 M nodes, 1.28 GB of
-    fp32 populations moved per launch, five times the 256 MB Infinity Cache), so that the achieved
-    bandwidth cannot come from cache residency; 16 launches timed with HIP events on the
-    simulation's stream."""
-    import d3q19
-    c = d3q19.Case(lines=c3_deck(storage=0))
-    nodes = (c.N[0] - 2) * (c.N[1] - 2) * (c.N[2] - 2)
-    s = d3q19.Sim(c)
-    s.write_particles(c.parKin)
-    s.run(1, 10)
-    s.sync()
-    ms = s.time_collide(16)
-    assert s.check_finite() == 0, "non-finite state"
-    s.close()
-    ach = 152.0 * nodes / (ms * 1e-3) / 1e9
-    return {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
-            "kernel_ms": round(ms, 5), "interior_nodes": nodes,
-            "workload": "256x256x128 Casson shear cell, 500 spheres r=5 x 128 markers, fp32 populations, 152 B/node"}
 
 
 def pct(a, q):
@@ -178,7 +202,7 @@ def ensemble(args):
             sm.write_particles(c.parKin)
         sm.run(1, 4)
         sm.sync()
-        ms, _ = sm.time_steps(5, seq_steps)
+        ms, _, _ = sm.time_steps(5, seq_steps)
         seq_ms += ms
         sm.close()
     mlups_seq = nodes * seq_steps / (seq_ms * 1e-3) / 1e6
@@ -191,6 +215,24 @@ def ensemble(args):
 
 
 def serving(args):
+    return serving_run(args.requests, args.max_edge, args.check_interval, args.max_steps, args.load, args.rate,
+                       args.serve_storage, args.arena_gb)
+
+
+def serving_summary(args):
+    """A bounded sample of the serving stream for the default line: the first 12 requests of the
+    stream (edges 32-160), each solve capped at 2000 steps, at 4x the sequential capacity."""
+    r = serving_run(12, 160, 100, 2000, 4.0, 0.0, 0, args.arena_gb, quiet=True)
+    return {"solves_per_s": r["batched"]["solves_per_s"], "p50_latency_s": r["batched"]["p50_latency_s"],
+            "p99_latency_s": r["batched"]["p99_latency_s"], "vs_sequential": r["speedup_solves_per_s"],
+            "sequential": r["sequential"], "requests": r["requests"], "max_steps": r["max_steps"],
+            "converged": r["converged"], "same_step_count": r["same_step_count"],
+            "max_rel_diff_wall_stress_vs_sequential": r["max_rel_diff_wall_stress_vs_sequential"],
+            "sample": "bounded: 12 requests of the 32-160 edge stream, solves capped at 2000 steps (bench.py --mode "
+                      "serving runs the stream to convergence)"}
+
+
+def serving_run(requests, max_edge, check_interval, max_steps, load, rate_arg, serve_storage, arena_gb, quiet=False):
     """Shear-cell solve requests as sequential plain fp32 solves (single simulations, one after another),
     then the same requests arriving as a Poisson stream into the continuous-batching server.  The
     arrival rate is `load` times the sequential capacity measured first (offered load above what the
@@ -198,21 +240,22 @@ def serving(args):
     the sequential latencies are those of a FIFO single server with the measured service times under
     the same arrivals."""
     import d3q19
-    decks = c5_requests(count=args.requests, edge_range=(32, args.max_edge), storage=0)
+    decks = c5_requests(count=requests, edge_range=(32, max_edge), storage=0)
     cases = [d3q19.Case(lines=d) for d in decks]
-    sp = d3q19.solve_params(tolerance=1e-4, check_interval=args.check_interval, min_steps=10 * args.check_interval,
-                            max_steps=args.max_steps)
+    sp = d3q19.solve_params(tolerance=1e-4, check_interval=check_interval, min_steps=min(10 * check_interval, max_steps),
+                            max_steps=max_steps)
     service, seq = [], {}
     for i, c in enumerate(cases):
         a = time.perf_counter()
         seq[i] = d3q19.solve(c, sp)
         service.append(time.perf_counter() - a)
-        print("serving: sequential %d/%d %s %d steps %.3f s" % (i + 1, len(cases), c.N, seq[i].steps, service[-1]), flush=True)
+        if not quiet:
+            print("serving: sequential %d/%d %s %d steps %.3f s" % (i + 1, len(cases), c.N, seq[i].steps, service[-1]),
+                  flush=True)
     cap_seq = len(cases) / sum(service)
-    rate = args.rate if args.rate > 0 else args.load * cap_seq
+    rate = rate_arg if rate_arg > 0 else load * cap_seq
     arr = poisson_arrivals(rate, len(cases))
-    srv = d3q19.Server(storage=args.serve_storage, collision=d3q19.COLL_BGK, ibm=d3q19.IBM_PESKIN4,
-                       arena_bytes=args.arena_gb << 30)
+    srv = d3q19.Server(storage=serve_storage, collision=d3q19.COLL_BGK, ibm=d3q19.IBM_PESKIN4, arena_bytes=arena_gb << 30)
     t0 = d3q19.clock()
     nxt = 0
     rid_of = {}
@@ -224,8 +267,9 @@ def serving(args):
             nxt += 1
         for r in srv.progress():
             done[rid_of[r.id]] = r
-            print("serving: batched %d/%d done (request %d, %d steps)" % (len(done), len(cases), rid_of[r.id], r.steps),
-                  flush=True)
+            if not quiet:
+                print("serving: batched %d/%d done (request %d, %d steps)" % (len(done), len(cases), rid_of[r.id], r.steps),
+                      flush=True)
         q, res, _ = srv.status()
         if q == 0 and res == 0 and nxt < len(cases):
             time.sleep(max(0.0, arr[nxt] - (d3q19.clock() - t0)))
@@ -241,8 +285,8 @@ def serving(args):
     steps_equal = sum(1 for i in range(len(cases)) if done[i].steps == seq[i].steps)
     names = {0: "fp32", 1: "fp16_shift", 2: "bf16_shift", 3: "fp64"}
     return {"requests": len(cases), "arrival_rate_per_s": round(rate, 4), "offered_load_vs_sequential": round(rate / cap_seq, 3),
-            "edges": [c.N[0] - 2 for c in cases], "spheres": [c.np for c in cases], "check_interval": args.check_interval,
-            "tolerance": 1e-4, "max_steps": args.max_steps, "server_storage": names[args.serve_storage],
+            "edges": [c.N[0] - 2 for c in cases], "spheres": [c.np for c in cases], "check_interval": check_interval,
+            "tolerance": 1e-4, "max_steps": max_steps, "server_storage": names[serve_storage],
             "converged": sum(1 for r in done.values() if r.status == 0),
             "batched": {"solves_per_s": round(len(cases) / span, 4), "p50_latency_s": round(pct(lat, 50), 4),
                         "p99_latency_s": round(pct(lat, 99), 4), "makespan_s": round(span, 3)},
@@ -255,8 +299,8 @@ def serving(args):
 def large(args):
     """The 256x256x128 Casson workload (500 spheres r=5 x 128 markers, 8.45 M interior nodes, five
     times the Infinity Cache in fp32), fp32 and fp16-shifted populations: the collide alone (16
-    event-timed launches) and the whole step (fused step kernel + IBM pass, graph-replayed, timed
-    with HIP events on the simulation's stream)."""
+    event-timed launches, after the steps: they advance the lattice without the spheres) and the whole
+    step (step kernel + IBM pass, real steps timed with HIP events on the simulation's stream)."""
     import d3q19
     out = {}
     for st, name in ((0, "fp32"), (1, "fp16_shift")):
@@ -266,9 +310,9 @@ def large(args):
         s.write_particles(c.parKin)
         s.run(1, args.warmup)
         s.sync()
-        col = s.time_collide(16)
-        tot, _ = s.time_steps(args.warmup + 1, args.steps)
+        tot, _, _ = s.time_steps(args.warmup + 1, args.steps)
         assert s.check_finite() == 0, "non-finite state"
+        col = s.time_collide(16)
         s.close()
         ms = tot / args.steps
         out[name] = {"step_ms": round(ms, 5), "mlups": round(nodes / (ms * 1e-3) / 1e6, 1), "collide_ms": round(col, 5),
@@ -285,26 +329,58 @@ def precision(args):
     return precision_study(c3_deck, args.prec_steps)
 
 
+def stream_sweep(args):
+    """Attainable bandwidth of the collide's memory pattern on this GPU (d3q19_stream_probe): a
+    streaming kernel with 19 fp32 planes read at the D3Q19 pull offsets and 19 written, no
+    arithmetic, beside a plain copy, for 1 / 2 / 4 consecutive nodes per lane, on the
+    256x256x128 box and on a 512x256x256 box (2.5 GB per direction); then the 19-plane pull
+    pattern with 1 node per lane at 2-8 resident 256-thread blocks per CU (dynamic LDS caps them)."""
+    import d3q19
+    rows = []
+    for box in ((256, 256, 128), (512, 256, 256)):
+        for streams, width, pull in ((1, 4, False), (19, 1, False), (19, 2, False), (19, 4, False),
+                                     (19, 1, True), (19, 2, True), (19, 4, True), (19, -64, False), (19, -64, True),
+                                     (19, -256, True)):
+            # the copy moves the bytes of the 19 planes, so that neither fits the Infinity Cache
+            cbox = (box[0] * 19, box[1], box[2]) if streams == 1 else box
+            gbs, ms = d3q19.stream_probe(*cbox, streams=streams, width=width, pull=pull, launches=args.probe_launches)
+            rows.append({"box": list(cbox), "streams": streams, "width": width, "pull": pull, "lds_bytes": 0,
+                         "layout": "chunks of %d nodes" % -width if width < 0 else "planes",
+                         "GBs": round(gbs, 1), "ms": round(ms, 5), "frac_of_8TBs": round(gbs / HBM_PEAK_GBS, 4)})
+            print("stream: %s" % json.dumps(rows[-1]), flush=True)
+    for blocks in (2, 3, 4, 6, 8):
+        lds = min(65536, (160 * 1024) // blocks - 256)
+        gbs, ms = d3q19.stream_probe(512, 256, 256, streams=19, width=1, pull=True, lds_bytes=lds,
+                                     launches=args.probe_launches)
+        rows.append({"box": [512, 256, 256], "streams": 19, "width": 1, "pull": True, "lds_bytes": lds,
+                     "blocks_per_cu": blocks, "GBs": round(gbs, 1), "ms": round(ms, 5),
+                     "frac_of_8TBs": round(gbs / HBM_PEAK_GBS, 4)})
+        print("stream: %s" % json.dumps(rows[-1]), flush=True)
+    best = max(r["GBs"] for r in rows if r["streams"] == 19)
+    return {"rows": rows, "best_19_plane_GBs": best}
+
+
 COPY_ELEMS = 1 << 28  # float32 elements of the calibration copy (1 GiB read + 1 GiB written)
 
 
 def traffic_probe():
     """Launches for the HBM-traffic counter passes (rocprofv3 --pmc FETCH_SIZE, then WRITE_SIZE):
-    5 direct collide+stream launches of the workload in fp32 and in fp16-shifted storage, then a
-    plain elementwise copy of known size (torch.neg over 1 GiB of float32) whose counter readings
-    calibrate the counters (expected 1 GiB fetched and 1 GiB written per launch)."""
+    5 timed real steps of the 256x256x128 workload in fp32 and in fp16-shifted storage (after 10
+    warm-up steps), then a plain elementwise copy of known size (torch.neg over 1 GiB of float32)
+    whose counter readings calibrate the counters (expected 1 GiB fetched and 1 GiB written per
+    launch)."""
     import d3q19
     import torch
     for storage in (0, 1):
-        c = d3q19.Case(lines=c2_deck(storage=storage))
+        c = d3q19.Case(lines=c3_deck(storage=storage))
         s = d3q19.Sim(c)
-        s.upload_case()
-        s.run(1, 4)
+        s.write_particles(c.parKin)
+        s.run(1, 10)
         s.sync()
-        s.time_collide(5)
+        s.time_steps(11, 5)
         s.sync()
         s.close()
-        print("probe: collide storage=%d done" % storage, flush=True)
+        print("probe: 5 steps storage=%d done" % storage, flush=True)
     src = torch.ones(COPY_ELEMS, dtype=torch.float32, device="cuda")
     dst = torch.empty_like(src)
     for _ in range(3):
@@ -316,14 +392,17 @@ def traffic_probe():
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--cpu-steps", type=int, default=60)
+    ap.add_argument("--graph-steps", type=int, default=200, help="graph-replayed steps timed after the timed region")
+    ap.add_argument("--cpu-steps", type=int, default=40)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-large", action="store_true", help="skip the 256x256x128 collide roofline")
+    ap.add_argument("--no-c2", action="store_true", help="skip the 128^3 power-law workload")
+    ap.add_argument("--no-serving", action="store_true", help="skip the bounded serving stream")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="only the launches profiled by the HBM-traffic counter passes")
-    ap.add_argument("--mode", default="step", choices=["step", "ensemble", "serving", "precision", "large"],
+    ap.add_argument("--probe-launches", type=int, default=20)
+    ap.add_argument("--mode", default="step", choices=["step", "ensemble", "serving", "precision", "large", "stream"],
                     help="step: the per-step benchmark line (default); ensemble / serving / precision / large: the "
                          "batched, serving, storage-precision and 256x256x128 workloads, one JSON line each")
     ap.add_argument("--ens-steps", type=int, default=1000)
@@ -352,6 +431,8 @@ def main():
         print(json.dumps({"workload": "serving", **serving(args)}), flush=True)
     elif args.mode == "precision":
         print(json.dumps({"workload": "precision study", **precision(args)}), flush=True)
+    elif args.mode == "stream":
+        print(json.dumps({"workload": "attainable bandwidth of the collide pattern", **stream_sweep(args)}), flush=True)
     elif args.mode == "large":
         print(json.dumps({"workload": "256x256x128 Casson shear cell, 500 spheres", **large(args)}), flush=True)
     else:

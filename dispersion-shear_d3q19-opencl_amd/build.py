@@ -28,7 +28,7 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-I" + INC, "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
 ]
-HIP_SOURCES = ["abi.hip", "lbm.hip", "ibm.hip", "particles.hip", "batch.hip", "checkpoint.hip"]
+HIP_SOURCES = ["abi.hip", "lbm.hip", "ibm.hip", "particles.hip", "batch.hip", "checkpoint.hip", "probe.hip"]
 
 
 def run(cmd):

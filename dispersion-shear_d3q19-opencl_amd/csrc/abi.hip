@@ -140,6 +140,33 @@ static void default_block(Sim* s) {
 static void* cur_f(Sim* s) { return s->parity == 0 ? s->fA : s->fB; }
 static void* nxt_f(Sim* s) { return s->parity == 0 ? s->fB : s->fA; }
 
+// Watchdog of a simulation or batch member on its current lattice: a density pass over the
+// populations (non-finite or non-positive density anywhere) and the spheres' displacement flag
+// (set when a sphere moves a lattice unit or more in one step, which its dilated brick marks do not
+// cover).  The pass writes its flag into a spare word of the simulation's reduction scratch, so
+// nothing is allocated.  The enqueue form copies both words to host memory asynchronously; they are
+// valid after the stream is synchronised.
+int sim_watchdog_enqueue(Sim* s, int* h_nonfinite, int* h_moved) {
+    int* flag = (int*)(s->red + 4 * 512 + 4);
+    HIP_OK(hipMemsetAsync(flag, 0, sizeof(int), s->stream));
+    const int rc = launch_check_finite(s, cur_f(s), flag);
+    if (rc) return rc;
+    HIP_OK(hipMemcpyAsync(h_nonfinite, flag, sizeof(int), hipMemcpyDeviceToHost, s->stream));
+    *h_moved = 0;
+    if (s->np > 0 && s->d_flag) HIP_OK(hipMemcpyAsync(h_moved, s->d_flag, sizeof(int), hipMemcpyDeviceToHost, s->stream));
+    return 0;
+}
+
+int sim_watchdog(Sim* s, int* nonfinite, int* moved) {
+    int h[2] = {0, 0};
+    const int rc = sim_watchdog_enqueue(s, &h[0], &h[1]);
+    if (rc) return rc;
+    HIP_OK(hipStreamSynchronize(s->stream));
+    *nonfinite = h[0];
+    *moved = h[1];
+    return 0;
+}
+
 void sim_release(Sim* s) {
     void* ptrs[] = {s->fA, s->fB, s->u, s->tau, s->facc, s->mask[0], s->mask[1], s->mask[2], s->bcraw, s->pos, s->vel,
                     s->pos_alt, s->vel_alt, s->quat, s->omega, s->ffluid, s->tfluid, s->fcont, s->tcont, s->pts, s->mforce,
@@ -271,15 +298,18 @@ static int do_collide_standalone(Sim* s, int flags) {
 // queue beside the collide -- runs in extra blocks of the collide launch, then the IBM pass.  The
 // collide writes u in the bricks marked for this epoch, made by the previous fused step (or by the
 // prime kernel) from the positions before this step's dynamics, dilated by one lattice unit.
-static int enqueue_fused(Sim* s, int flags, int do_post, hipStream_t st) {
+// ev (optional): four events, recorded at the start and end of the step kernel and of the IBM pass
+static int enqueue_fused(Sim* s, int flags, int do_post, hipStream_t st, const hipEvent_t* ev = nullptr) {
     int rc;
-    if ((rc = launch_step_fused(s, cur_f(s), nxt_f(s), (flags & D3Q19_STEP_WRITE_U) ? 1 : 0, do_post, st))) return rc;
+    if ((rc = launch_step_fused(s, cur_f(s), nxt_f(s), (flags & D3Q19_STEP_WRITE_U) ? 1 : 0, do_post, st, ev ? ev[0] : nullptr,
+                                ev ? ev[1] : nullptr)))
+        return rc;
     s->parity ^= 1;
     if (s->np > 0) {
         std::swap(s->pos, s->pos_alt);
         std::swap(s->vel, s->vel_alt);
     }
-    return launch_ibm(s, st);
+    return launch_ibm(s, st, ev ? ev[2] : nullptr, ev ? ev[3] : nullptr);
 }
 
 static int build_graph(Sim* s) {
@@ -306,8 +336,9 @@ static int build_graph(Sim* s) {
 
 // n steps: the prime marks if needed, a first fused step without a pending post, graph-replayed
 // pairs, the remaining steps (the last one carries flags), then the last step's force sums and
-// contacts, so that the state after the call is the reference's state after n steps.
-static int run_steps(Sim* s, int t_first, int n, int flags) {
+// contacts, so that the state after the call is the reference's state after n steps.  With ev
+// (4 events per step) every step is launched directly, its kernels timed by the events.
+static int run_steps(Sim* s, int t_first, int n, int flags, const hipEvent_t* ev = nullptr) {
     if (n <= 0) return 0;
     int rc;
     // the contacts of step t read the zones of generation (t - 1) / rebuild_freq
@@ -317,10 +348,10 @@ static int run_steps(Sim* s, int t_first, int n, int flags) {
         if ((rc = launch_sphere_prime(s, s->stream))) return rc;
         s->marks_next = true;
     }
-    if ((rc = enqueue_fused(s, n == 1 ? flags : 0, 0, s->stream))) return rc;
+    if ((rc = enqueue_fused(s, n == 1 ? flags : 0, 0, s->stream, ev))) return rc;
     int done = 1;
     const int tail = (flags && n > 1) ? 1 : 0;
-    const int pairs = (n - 1 - tail) / 2;
+    const int pairs = ev ? 0 : (n - 1 - tail) / 2;
     if (pairs > 0) {
         if (!s->graph2 || s->graph_f != cur_f(s) || s->graph_pos != (const void*)s->pos || s->graph_tofs != s->tofs) {
             if ((rc = build_graph(s))) return rc;
@@ -329,7 +360,7 @@ static int run_steps(Sim* s, int t_first, int n, int flags) {
         done += 2 * pairs;
     }
     while (done < n) {
-        if ((rc = enqueue_fused(s, done == n - 1 ? flags : 0, 1, s->stream))) return rc;
+        if ((rc = enqueue_fused(s, done == n - 1 ? flags : 0, 1, s->stream, ev ? ev + 4 * done : nullptr))) return rc;
         done++;
     }
     return launch_sphere_post(s, 1, 1, s->stream);
@@ -419,6 +450,22 @@ size_t sim_state_bytes(const int_param_struct* ip, const flp_param_struct* fp, c
 
 int sim_validate(const int_param_struct* ip, const flp_param_struct* fp, const d3q19_options* opt) { return validate(ip, fp, opt); }
 
+// The brick marking (mark_sphere) lists at most MARK_BRICKS bricks per axis.  Along an axis a
+// sphere's markers have floor coordinates over at most floor(2R) + 2 values, whose stencils (padded
+// by up to 3 nodes for a step's motion) cover E = floor(2R) + 9 consecutive nodes, so at most
+// ceil(E / 4) + 1 bricks; R is the marking radius (the longest marker offset, at least 0.505 d).
+// A larger sphere is refused rather than marked incompletely.
+static int check_marking_span(const int_param_struct* ip, const flp_param_struct* fp, const d3q19_float4* pts) {
+    double rmax = 0.505 * (double)fp->ParticleDiam + 1e-3;
+    for (int m = 0; m < ip->PointsPerParticle; m++)
+        rmax = std::max(rmax, std::sqrt((double)pts[m].x * pts[m].x + (double)pts[m].y * pts[m].y + (double)pts[m].z * pts[m].z));
+    const double E = std::floor(2.0 * rmax) + 9.0;
+    if (!(rmax < 1e6) || (long long)std::ceil(E / 4.0) + 1 > MARK_BRICKS)
+        return fail(D3Q19_E_UNSUPPORTED, "sphere too large for the brick marking (at most " + std::to_string(MARK_BRICKS) +
+                                             " bricks of 4 nodes per axis: diameter up to about 240 lattice units)");
+    return 0;
+}
+
 // Allocation and initial state of a simulation (d3q19_create) or of a batch member, whose memory
 // comes from the batch arena and which uses the batch's stream and epoch counter.
 int sim_init(Sim* s, int device, const int_param_struct* ip, const flp_param_struct* fp, const d3q19_options* opt,
@@ -426,6 +473,7 @@ int sim_init(Sim* s, int device, const int_param_struct* ip, const flp_param_str
     int rc = validate(ip, fp, opt);
     if (rc) return rc;
     if (ip->NumParticles > 0 && !sphere_points) return fail(D3Q19_E_INVALID, "sphere_points required when NumParticles > 0");
+    if (ip->NumParticles > 0 && (rc = check_marking_span(ip, fp, sphere_points))) return rc;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev)
         return fail(D3Q19_E_DEVICE, "no such HIP device");
@@ -981,21 +1029,11 @@ int d3q19_observables_get(d3q19_sim* s, d3q19_observables* out) {
 
 int d3q19_check_finite(d3q19_sim* s) {
     CHECK_SIM(s);
-    int* flag = nullptr;
-    int rc = dalloc(&flag, sizeof(int));
+    int nonfinite = 0, moved = 0;
+    const int rc = sim_watchdog(s, &nonfinite, &moved);
     if (rc) return rc;
-    hipError_t e = hipMemsetAsync(flag, 0, sizeof(int), s->stream);
-    if (e == hipSuccess) rc = launch_check_finite(s, cur_f(s), flag);
-    int h = 0;
-    if (e == hipSuccess && !rc) e = hipMemcpyAsync(&h, flag, sizeof(int), hipMemcpyDeviceToHost, s->stream);
-    hipError_t e2 = hipStreamSynchronize(s->stream);
-    (void)hipFree(flag);
-    if (rc) return rc;
-    if (e != hipSuccess || e2 != hipSuccess) return fail(D3Q19_E_DEVICE, "check_finite failed");
-    if (h) return fail(D3Q19_E_NONFINITE, "non-finite or non-positive density in the population field");
-    int moved = 0;
-    HIP_OK(hipMemcpy(&moved, s->d_flag, sizeof(int), hipMemcpyDeviceToHost));
     if (moved) return fail(D3Q19_E_STATE, "a sphere moved one lattice unit or more in one step");
+    if (nonfinite) return fail(D3Q19_E_NONFINITE, "non-finite or non-positive density in the population field");
     return 0;
 }
 
@@ -1028,37 +1066,42 @@ int d3q19_time_collide(d3q19_sim* s, int n, int flags, double* ms) {
     return 0;
 }
 
-int d3q19_time_steps(d3q19_sim* s, int t_first, int n, double* total_ms, double* collide_ms) {
+int d3q19_time_steps(d3q19_sim* s, int t_first, int n, double* total_ms, double* kernel_ms) {
     CHECK_SIM(s);
-    if (n <= 0 || !total_ms) return fail(D3Q19_E_INVALID, "bad timing request");
-    HIP_OK(hipStreamSynchronize(s->stream));
-    HIP_OK(hipEventRecord(s->ev0, s->stream));
-    int rc = d3q19_run(s, t_first, n, 0);
+    if (n <= 0 || n > 4096 || !total_ms) return fail(D3Q19_E_INVALID, "bad timing request");
+    // 4 events per step (step kernel, IBM pass: each kernel's own start and end)
+    std::vector<hipEvent_t> evs(4 * (size_t)n, nullptr);
+    int rc = 0;
+    for (auto& ev : evs)
+        if (hipEventCreate(&ev) != hipSuccess) {
+            rc = fail(D3Q19_E_DEVICE, "hipEventCreate failed");
+            break;
+        }
+    float t = 0.f;
+    if (!rc && hipStreamSynchronize(s->stream) != hipSuccess) rc = fail(D3Q19_E_DEVICE, "hipStreamSynchronize failed");
+    if (!rc && hipEventRecord(s->ev0, s->stream) != hipSuccess) rc = fail(D3Q19_E_DEVICE, "hipEventRecord failed");
+    if (!rc) rc = run_steps(s, t_first, n, 0, evs.data());
+    if (!rc && (hipEventRecord(s->ev1, s->stream) != hipSuccess || hipEventSynchronize(s->ev1) != hipSuccess ||
+                hipEventElapsedTime(&t, s->ev0, s->ev1) != hipSuccess))
+        rc = fail(D3Q19_E_DEVICE, "step timing failed");
+    double acc[2] = {0.0, 0.0};
+    for (int i = 0; i < n && !rc && kernel_ms; i++)
+        for (int k = 0; k < 2; k++) {
+            if (k == 1 && s->np == 0) continue;
+            float dt = 0.f;
+            if (hipEventElapsedTime(&dt, evs[4 * i + 2 * k], evs[4 * i + 2 * k + 1]) != hipSuccess) {
+                rc = fail(D3Q19_E_DEVICE, "hipEventElapsedTime failed");
+                break;
+            }
+            acc[k] += dt;
+        }
+    for (auto& ev : evs)
+        if (ev) (void)hipEventDestroy(ev);
     if (rc) return rc;
-    HIP_OK(hipEventRecord(s->ev1, s->stream));
-    HIP_OK(hipEventSynchronize(s->ev1));
-    float t = 0;
-    HIP_OK(hipEventElapsedTime(&t, s->ev0, s->ev1));
     *total_ms = (double)t;
-    if (collide_ms) {
-        // per-launch duration of the collide kernel alone, same stream, same state
-        std::vector<hipEvent_t> evs(2 * 16);
-        for (auto& ev : evs) HIP_OK(hipEventCreate(&ev));
-        double acc = 0;
-        for (int i = 0; i < 16; i++) {
-            rc = launch_collide(s, cur_f(s), nxt_f(s), 0, 1, s->stream, evs[2 * i], evs[2 * i + 1]);
-            if (rc) break;
-            s->parity ^= 1;
-        }
-        HIP_OK(hipStreamSynchronize(s->stream));
-        for (int i = 0; i < 16 && !rc; i++) {
-            float dt = 0;
-            HIP_OK(hipEventElapsedTime(&dt, evs[2 * i], evs[2 * i + 1]));
-            acc += dt;
-        }
-        for (auto& ev : evs) (void)hipEventDestroy(ev);
-        if (rc) return rc;
-        *collide_ms = acc / 16;
+    if (kernel_ms) {
+        kernel_ms[0] = acc[0] / n;
+        kernel_ms[1] = acc[1] / n;
     }
     return 0;
 }

@@ -566,11 +566,16 @@ struct Server {
     int interval = 0;  // steps per scheduling round = the requests' check interval
 };
 
-// convergence rule shared by the server and the sequential solve; returns 1 when finished
-static int check_request(Request* r, const d3q19_observables& o, d3q19_result* res) {
+// convergence rule shared by the server and the sequential solve; returns 1 when finished.
+// nonfinite / moved are the cell's watchdog words at this check: a non-finite or non-positive
+// density anywhere in its lattice, or a sphere that moved a lattice unit or more in one step (its
+// dilated brick marks no longer covered its stencils, so the run is not trusted).  Either ends the
+// request with an error status (D3Q19_E_STATE for the displacement, D3Q19_E_NONFINITE otherwise).
+static int check_request(Request* r, const d3q19_observables& o, int nonfinite, int moved, d3q19_result* res) {
     const double sw = o.wall_stress;
     int status = -99;
-    if (!std::isfinite(sw)) status = D3Q19_E_NONFINITE;
+    if (moved) status = D3Q19_E_STATE;
+    else if (nonfinite || !std::isfinite(sw)) status = D3Q19_E_NONFINITE;
     else if (r->have_last && r->steps >= r->sp.min_steps && fabs(sw - r->last) <= r->sp.tolerance * fabs(sw)) status = 0;
     else if (r->steps >= r->sp.max_steps) status = 1;
     r->last = sw;
@@ -626,12 +631,20 @@ static int server_progress(Server* v, d3q19_result* done, int cap, int* n_done) 
     for (Request* r : v->resident) ids.push_back(r->cell);
     std::vector<d3q19_observables> obs(ids.size());
     if ((rc = batch_observables(b, ids.data(), (int)ids.size(), obs.data()))) return rc;
+    // watchdog of every resident cell on its current lattice: passes enqueued, one synchronisation
+    std::vector<int> bad(ids.size(), 0), moved(ids.size(), 0);
+    for (size_t i = 0; i < ids.size(); i++) {
+        Member& m = b->cells[ids[i]];
+        sync_view(m, b->phase);
+        if ((rc = sim_watchdog_enqueue(m.sim, &bad[i], &moved[i]))) return rc;
+    }
+    HIP_OK(hipStreamSynchronize(b->stream));
     std::vector<Request*> keep;
     for (size_t i = 0; i < v->resident.size(); i++) {
         Request* r = v->resident[i];
         r->steps += v->interval;
         d3q19_result res;
-        if (check_request(r, obs[i], &res)) {
+        if (check_request(r, obs[i], bad[i], moved[i], &res)) {
             v->outbox.push_back(res);
             rc = batch_remove(b, r->cell);
             delete r;
@@ -882,7 +895,9 @@ int d3q19_solve(int device, const int_param_struct* ip, const flp_param_struct* 
         r.steps += sp->check_interval;
         d3q19_observables o;
         if ((rc = d3q19_observables_get(s, &o))) break;
-        if (check_request(&r, o, out)) break;
+        int bad = 0, moved = 0;
+        if ((rc = sim_watchdog(s, &bad, &moved))) break;
+        if (check_request(&r, o, bad, moved, out)) break;
     }
     d3q19_destroy(s);
     return rc;

@@ -18,7 +18,7 @@ using namespace d3q19;
 namespace {
 
 constexpr char MAGIC[8] = {'D', '3', 'Q', '1', '9', 'C', 'K', '1'};
-constexpr uint32_t VERSION = 2;  // 2: u stored with its components interleaved per node
+constexpr uint32_t VERSION = 3;  // 2: u stored with its components interleaved per node; 3: displacement watchdog
 
 struct Header {
     char magic[8];
@@ -44,7 +44,7 @@ struct Section {
 
 enum : uint32_t {
     S_F = 1, S_U, S_TAU, S_FACC, S_MASK0, S_MASK1, S_MASK2, S_EPOCHS, S_POS, S_VEL, S_QUAT, S_OMEGA, S_FFLUID,
-    S_TFLUID, S_FCONT, S_TCONT, S_MFORCE, S_ZONE, S_BCRAW, S_PTS
+    S_TFLUID, S_FCONT, S_TCONT, S_MFORCE, S_ZONE, S_BCRAW, S_PTS, S_DFLAG
 };
 
 struct Array {
@@ -85,6 +85,9 @@ std::vector<Array> state_arrays(Sim* s) {
         v.push_back({S_TCONT, s->tcont, 3 * np * 8, false});
         v.push_back({S_MFORCE, s->mforce, 6 * np * (size_t)s->ppp * 8, false});
         v.push_back({S_ZONE, s->zone, 2 * np * sizeof(int), false});
+        // the watchdog travels with the state: a restored run reports a displacement that the run
+        // it continues had raised
+        v.push_back({S_DFLAG, s->d_flag, sizeof(int), false});
         v.push_back({S_PTS, s->pts, 4 * (size_t)s->ppp * sizeof(float), true});
     }
     return v;

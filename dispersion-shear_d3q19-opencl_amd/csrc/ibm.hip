@@ -12,6 +12,7 @@
 // the next collide (GPU_program.cl:356-377, buffer shell contributing zero and no periodic wrap)
 // is folded into the spread: each marker spreads directly with the convolution of its trilinear
 // weights and the (1/4,1/2,1/4)^3 kernel, restricted to interior target nodes.
+#include <hip/hip_ext.h>
 #include "internal.h"
 
 namespace d3q19 {
@@ -327,7 +328,14 @@ IbmArgs make_ibm_args(Sim* s) {
     return a;
 }
 
-int launch_ibm(Sim* s, hipStream_t st) {
+// a kernel launch, timed by events at the kernel's own start and end when ev0 is set
+template <class K, class... Args>
+static void launch_timed(K kernel, dim3 grid, dim3 block, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1, Args... args) {
+    if (ev0) hipExtLaunchKernelGGL(kernel, grid, block, 0, st, ev0, ev1, 0, args...);
+    else hipLaunchKernelGGL(kernel, grid, block, 0, st, args...);
+}
+
+int launch_ibm(Sim* s, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1) {
     if (s->np == 0) return 0;
     const IbmArgs a = make_ibm_args(s);
     const bool dbl = s->storage == D3Q19_STORE_FP64;
@@ -335,11 +343,11 @@ int launch_ibm(Sim* s, hipStream_t st) {
         const int blocks = s->np * ((s->ppp + TILE_MARKERS - 1) / TILE_MARKERS);
         const dim3 th(64 * TILE_MARKERS);
         if (a.mode == D3Q19_IBM_PESKIN4) {
-            if (dbl) hipLaunchKernelGGL((ibm_tile_kernel<double, D3Q19_IBM_PESKIN4>), dim3(blocks), th, 0, st, a);
-            else hipLaunchKernelGGL((ibm_tile_kernel<float, D3Q19_IBM_PESKIN4>), dim3(blocks), th, 0, st, a);
+            if (dbl) launch_timed(ibm_tile_kernel<double, D3Q19_IBM_PESKIN4>, dim3(blocks), dim3(th), st, ev0, ev1, a);
+            else launch_timed(ibm_tile_kernel<float, D3Q19_IBM_PESKIN4>, dim3(blocks), dim3(th), st, ev0, ev1, a);
         } else {
-            if (dbl) hipLaunchKernelGGL((ibm_tile_kernel<double, D3Q19_IBM_LINEAR_SMOOTHED>), dim3(blocks), th, 0, st, a);
-            else hipLaunchKernelGGL((ibm_tile_kernel<float, D3Q19_IBM_LINEAR_SMOOTHED>), dim3(blocks), th, 0, st, a);
+            if (dbl) launch_timed(ibm_tile_kernel<double, D3Q19_IBM_LINEAR_SMOOTHED>, dim3(blocks), dim3(th), st, ev0, ev1, a);
+            else launch_timed(ibm_tile_kernel<float, D3Q19_IBM_LINEAR_SMOOTHED>, dim3(blocks), dim3(th), st, ev0, ev1, a);
         }
         HIP_OK(hipGetLastError());
         return 0;
@@ -347,11 +355,11 @@ int launch_ibm(Sim* s, hipStream_t st) {
     const int total = s->np * s->ppp;
     const int th = 64 * IBM_WAVES, blocks = (total + IBM_WAVES - 1) / IBM_WAVES;  // one wavefront per marker
     if (a.mode == D3Q19_IBM_PESKIN4) {
-        if (dbl) hipLaunchKernelGGL((ibm_marker_kernel<double, D3Q19_IBM_PESKIN4>), dim3(blocks), dim3(th), 0, st, a);
-        else hipLaunchKernelGGL((ibm_marker_kernel<float, D3Q19_IBM_PESKIN4>), dim3(blocks), dim3(th), 0, st, a);
+        if (dbl) launch_timed(ibm_marker_kernel<double, D3Q19_IBM_PESKIN4>, dim3(blocks), dim3(th), st, ev0, ev1, a);
+        else launch_timed(ibm_marker_kernel<float, D3Q19_IBM_PESKIN4>, dim3(blocks), dim3(th), st, ev0, ev1, a);
     } else {
-        if (dbl) hipLaunchKernelGGL((ibm_marker_kernel<double, D3Q19_IBM_LINEAR_SMOOTHED>), dim3(blocks), dim3(th), 0, st, a);
-        else hipLaunchKernelGGL((ibm_marker_kernel<float, D3Q19_IBM_LINEAR_SMOOTHED>), dim3(blocks), dim3(th), 0, st, a);
+        if (dbl) launch_timed(ibm_marker_kernel<double, D3Q19_IBM_LINEAR_SMOOTHED>, dim3(blocks), dim3(th), st, ev0, ev1, a);
+        else launch_timed(ibm_marker_kernel<float, D3Q19_IBM_LINEAR_SMOOTHED>, dim3(blocks), dim3(th), st, ev0, ev1, a);
     }
     HIP_OK(hipGetLastError());
     return 0;

@@ -36,6 +36,7 @@ constexpr int IBM_WAVES = 4;                          // markers (wavefronts) pe
 constexpr int TILE_MARKERS = 8;                       // markers per block of the LDS-staged spreading
 constexpr int TILE_NODES = 1024;                      // LDS box of that spreading (nodes)
 constexpr float FP16_SCALE = 16384.0f;                // 2^14, relative-deviation scale of fp16 storage
+constexpr int MARK_BRICKS = 64;                       // bricks per axis a sphere's marking can list
 
 // ---- velocity set (reference order, D3Q19-OpenCL.c:9-13) ---------------------------------
 __host__ __device__ constexpr int cx(int q) {
@@ -57,6 +58,17 @@ __host__ __device__ constexpr int opposite(int q) {
         if (cx(p) == -cx(q) && cy(p) == -cy(q) && cz(p) == -cz(q)) r = p;
     return r;
 }
+
+// compile-time loop
+template <int I, int N> struct Unroll {
+    template <class F> __device__ __forceinline__ static void run(F&& f) {
+        f(std::integral_constant<int, I>{});
+        Unroll<I + 1, N>::run(f);
+    }
+};
+template <int N> struct Unroll<N, N> {
+    template <class F> __device__ __forceinline__ static void run(F&&) {}
+};
 
 // ---- d'Humieres moment basis, row order of GPU_program.cl:410-449 -----------------------------
 __host__ __device__ constexpr int mrow(int k, int q) {
@@ -359,6 +371,9 @@ int sim_init(Sim* s, int device, const int_param_struct* ip, const flp_param_str
              const d3q19_float4* sphere_points, Arena* arena, hipStream_t shared_stream, uint32_t* shared_epochs);
 void sim_release(Sim* s);
 int sim_validate(const int_param_struct* ip, const flp_param_struct* fp, const d3q19_options* opt);
+// watchdog: non-finite density on the current lattice, sphere displacement flag (see abi.hip)
+int sim_watchdog_enqueue(Sim* s, int* h_nonfinite, int* h_moved);
+int sim_watchdog(Sim* s, int* nonfinite, int* moved);
 size_t sim_state_bytes(const int_param_struct* ip, const flp_param_struct* fp, const d3q19_options* opt);
 // observables of one cell from the 8 raw sums of the observables kernel
 void derive_observables(const Sim* s, const double* raw, d3q19_observables* out);
@@ -380,14 +395,15 @@ int launch_export_force(Sim* s, float* ref_out);
 int launch_plane_mean(Sim* s, double* dev_out);
 int launch_moments(Sim* s, const void* fsrc, double* dev_out);
 int launch_check_finite(Sim* s, const void* fsrc, int* dev_flag);
-int launch_ibm(Sim* s, hipStream_t st);
+int launch_ibm(Sim* s, hipStream_t st, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
 int launch_sphere_pre(Sim* s, int do_dyn, int do_mark, hipStream_t st);       // dynamics, brick marks
 int launch_sphere_post(Sim* s, int do_reduce, int do_contacts, hipStream_t st); // force sums, contacts
 int launch_zone_init(Sim* s, hipStream_t st);  // zones of the current positions, both generations
 int launch_sphere_prime(Sim* s, hipStream_t st);                                // dilated marks, next epoch
 // fused step: the spheres' turn (post of the last step if pending, dynamics into the other
 // position buffer, dilated marks of the epoch after this one) in np extra blocks of the collide
-int launch_step_fused(Sim* s, const void* fin, void* fout, int write_u_all, int do_post, hipStream_t st);
+int launch_step_fused(Sim* s, const void* fin, void* fout, int write_u_all, int do_post, hipStream_t st, hipEvent_t ev0 = nullptr,
+                      hipEvent_t ev1 = nullptr);
 int launch_fill_equilibrium(Sim* s, void* f);
 int launch_fill_tau(Sim* s);
 

@@ -123,17 +123,6 @@ template <int q> __device__ __forceinline__ uint32_t sr_word(uint32_t h) {
     else return (x << s) | (x >> (32 - s));
 }
 
-// compile-time loop
-template <int I, int N> struct Unroll {
-    template <class F> __device__ __forceinline__ static void run(F&& f) {
-        f(std::integral_constant<int, I>{});
-        Unroll<I + 1, N>::run(f);
-    }
-};
-template <int N> struct Unroll<N, N> {
-    template <class F> __device__ __forceinline__ static void run(F&&) {}
-};
-
 // Arithmetic helpers: the float build uses the hardware reciprocal, square root, exp2 and log2
 // (about 1 ulp); the double build (the fp64 oracle build) keeps IEEE operations.
 template <class R> __device__ __forceinline__ R rcp_(R x) { return (R)1 / x; }
@@ -356,8 +345,8 @@ __device__ __forceinline__ R strain_rate(const R* ss, R ux, R uy, R uz, R gx, R 
 
 // One lattice update at interior node (x, y, z).  VT: 0 constant tau, 1 per-node tau, 2 per-node tau
 // when a.vt is set (batched cells of mixed rheology).  probe = 1 (autotuning trials) leaves the
-// pending spread force and the tau field untouched so a trial launch does not change the state
-// a later step reads.
+// pending spread force, the tau field and u untouched so a trial launch does not change the state
+// a later step reads (its populations go to the lattice the next step overwrites).
 template <int ST, int COLL, int VT, bool WZ, bool SBASE_STORES = true>
 __device__ __forceinline__ void collide_node(const CollideArgs& a, const void* fin, void* fout, int write_u_all, int x, int y,
                                              int z, int probe) {
@@ -408,7 +397,7 @@ __device__ __forceinline__ void collide_node(const CollideArgs& a, const void* f
     const R ux = (f[1] - f[2] + f[7] + f[8] + f[9] + f[10] - f[11] - f[12] - f[13] - f[14] + (R)0.5 * gx) * inv_rho;
     const R uy = (f[3] - f[4] + f[7] - f[8] + f[11] - f[12] + f[15] + f[16] - f[17] - f[18] + (R)0.5 * gy) * inv_rho;
     const R uz = (f[5] - f[6] + f[9] - f[10] + f[13] - f[14] + f[15] - f[16] + f[17] - f[18] + (R)0.5 * gz) * inv_rho;
-    if (write_u_all || (a.use_umask && umark == ep)) {
+    if (!probe && (write_u_all || (a.use_umask && umark == ep))) {
         // interleaved (x, y, z) per node: one write stream instead of three (the three separate
         // component arrays cost 7.5% of the fp32 collide on the 256x256x128 workload, whose marked
         // bricks hold a third of the nodes)
@@ -695,31 +684,41 @@ int launch_collide(Sim* s, const void* fin, void* fout, int write_u_all, int use
     return 0;
 }
 
+// A kernel launch, with start/end events recorded at the kernel's own start and end when ev0 is set
+template <class K, class... Args>
+static void launch_maybe_timed(K kernel, dim3 grid, dim3 block, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1, Args... args) {
+    if (ev0) hipExtLaunchKernelGGL(kernel, grid, block, 0, st, ev0, ev1, 0, args...);
+    else hipLaunchKernelGGL(kernel, grid, block, 0, st, args...);
+}
+
 template <int ST, int COLL, int VT>
-static void launch_fused_one(const CollideArgs& a, const PartArgs& p, int nsph, int do_post, int threads, hipStream_t st) {
+static void launch_fused_one(const CollideArgs& a, const PartArgs& p, int nsph, int do_post, int threads, hipStream_t st,
+                             hipEvent_t ev0, hipEvent_t ev1) {
     const bool wz = a.g.wall == 2 && a.g.per[0] && a.g.per[1] && a.fl.tan_mask == 0;
     const dim3 grid(a.nblk + nsph);
     if constexpr (COLL == D3Q19_COLL_BGK && ST != D3Q19_STORE_FP64) {
-        if (wz) hipLaunchKernelGGL((step_kernel_w6<ST, COLL, VT, true>), grid, dim3(threads), 0, st, a, p, nsph, do_post);
-        else hipLaunchKernelGGL((step_kernel_w6<ST, COLL, VT, false>), grid, dim3(threads), 0, st, a, p, nsph, do_post);
+        if (wz) launch_maybe_timed(step_kernel_w6<ST, COLL, VT, true>, grid, dim3(threads), st, ev0, ev1, a, p, nsph, do_post);
+        else launch_maybe_timed(step_kernel_w6<ST, COLL, VT, false>, grid, dim3(threads), st, ev0, ev1, a, p, nsph, do_post);
     } else {
-        if (wz) hipLaunchKernelGGL((step_kernel<ST, COLL, VT, true>), grid, dim3(threads), 0, st, a, p, nsph, do_post);
-        else hipLaunchKernelGGL((step_kernel<ST, COLL, VT, false>), grid, dim3(threads), 0, st, a, p, nsph, do_post);
+        if (wz) launch_maybe_timed(step_kernel<ST, COLL, VT, true>, grid, dim3(threads), st, ev0, ev1, a, p, nsph, do_post);
+        else launch_maybe_timed(step_kernel<ST, COLL, VT, false>, grid, dim3(threads), st, ev0, ev1, a, p, nsph, do_post);
     }
 }
 
 template <int ST>
-static void fused_st(const CollideArgs& a, const PartArgs& p, int nsph, int do_post, int coll, int vt, int threads, hipStream_t st) {
+static void fused_st(const CollideArgs& a, const PartArgs& p, int nsph, int do_post, int coll, int vt, int threads, hipStream_t st,
+                     hipEvent_t ev0, hipEvent_t ev1) {
     if (coll == D3Q19_COLL_BGK) {
-        if (vt) launch_fused_one<ST, D3Q19_COLL_BGK, 1>(a, p, nsph, do_post, threads, st);
-        else launch_fused_one<ST, D3Q19_COLL_BGK, 0>(a, p, nsph, do_post, threads, st);
+        if (vt) launch_fused_one<ST, D3Q19_COLL_BGK, 1>(a, p, nsph, do_post, threads, st, ev0, ev1);
+        else launch_fused_one<ST, D3Q19_COLL_BGK, 0>(a, p, nsph, do_post, threads, st, ev0, ev1);
     } else {
-        if (vt) launch_fused_one<ST, D3Q19_COLL_MRT, 1>(a, p, nsph, do_post, threads, st);
-        else launch_fused_one<ST, D3Q19_COLL_MRT, 0>(a, p, nsph, do_post, threads, st);
+        if (vt) launch_fused_one<ST, D3Q19_COLL_MRT, 1>(a, p, nsph, do_post, threads, st, ev0, ev1);
+        else launch_fused_one<ST, D3Q19_COLL_MRT, 0>(a, p, nsph, do_post, threads, st, ev0, ev1);
     }
 }
 
-int launch_step_fused(Sim* s, const void* fin, void* fout, int write_u_all, int do_post, hipStream_t st) {
+int launch_step_fused(Sim* s, const void* fin, void* fout, int write_u_all, int do_post, hipStream_t st, hipEvent_t ev0,
+                      hipEvent_t ev1) {
     const CollideArgs a = make_collide_args(s, fin, fout, write_u_all, 1);
     PartArgs p = make_part_args(s);
     p.pos_out = gp(s->pos_alt);
@@ -728,10 +727,10 @@ int launch_step_fused(Sim* s, const void* fin, void* fout, int write_u_all, int 
     const int threads = a.bx * a.by;
     const int coll = s->opt.collision, vt = s->opt.variable_viscosity;
     switch (s->storage) {
-    case D3Q19_STORE_FP16_SHIFT: fused_st<D3Q19_STORE_FP16_SHIFT>(a, p, nsph, do_post, coll, vt, threads, st); break;
-    case D3Q19_STORE_BF16_SHIFT: fused_st<D3Q19_STORE_BF16_SHIFT>(a, p, nsph, do_post, coll, vt, threads, st); break;
-    case D3Q19_STORE_FP64: fused_st<D3Q19_STORE_FP64>(a, p, nsph, do_post, coll, vt, threads, st); break;
-    default: fused_st<D3Q19_STORE_FP32>(a, p, nsph, do_post, coll, vt, threads, st); break;
+    case D3Q19_STORE_FP16_SHIFT: fused_st<D3Q19_STORE_FP16_SHIFT>(a, p, nsph, do_post, coll, vt, threads, st, ev0, ev1); break;
+    case D3Q19_STORE_BF16_SHIFT: fused_st<D3Q19_STORE_BF16_SHIFT>(a, p, nsph, do_post, coll, vt, threads, st, ev0, ev1); break;
+    case D3Q19_STORE_FP64: fused_st<D3Q19_STORE_FP64>(a, p, nsph, do_post, coll, vt, threads, st, ev0, ev1); break;
+    default: fused_st<D3Q19_STORE_FP32>(a, p, nsph, do_post, coll, vt, threads, st, ev0, ev1); break;
     }
     HIP_OK(hipGetLastError());
     s->bcraw_valid = 0;

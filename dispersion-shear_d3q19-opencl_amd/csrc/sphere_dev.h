@@ -41,8 +41,8 @@ __host__ __device__ inline P mask_of(P m0, P m1, P m2, uint32_t e) {
 template <class MP>
 __device__ void mark_sphere(const double* c3, const Geom& g, double radius, double rin, double rout, MP mk, uint32_t e, int pad) {
     __shared__ uint8_t fl[3][1024];
-    __shared__ int lists[3][64];
-    __shared__ float dn[3][64], df[3][64];
+    __shared__ int lists[3][MARK_BRICKS];  // sim_init refuses spheres that need more bricks per axis
+    __shared__ float dn[3][MARK_BRICKS], df[3][MARK_BRICKS];
     __shared__ int counts[3];
     const int nd[3] = {g.nx, g.ny, g.nz};
     const int bn[3] = {g.bnx, g.bny, g.bnz};
@@ -79,7 +79,7 @@ __device__ void mark_sphere(const double* c3, const Geom& g, double radius, doub
     if (threadIdx.x < 3) {
         const int d = threadIdx.x;
         int cnt = 0;
-        for (int b = 0; b < bn[d] && cnt < 64; b++)
+        for (int b = 0; b < bn[d] && cnt < MARK_BRICKS; b++)
             if (fl[d][b]) {
                 double lo_r = 4.0 * b - pad - c3[d], hi_r = 4.0 * b + 3 + pad - c3[d];
                 if (g.per[d]) {

@@ -150,7 +150,7 @@ STORE_FP32, STORE_FP16, STORE_BF16, STORE_FP64 = 0, 1, 2, 3
 CONTACT_ZONES, CONTACT_ALL_PAIRS = 0, 1
 ROUND_STOCHASTIC, ROUND_NEAREST = 0, 1
 STEP_WRITE_U = 1
-E_NONFINITE = -6
+E_INVALID, E_DEVICE, E_NOMEM, E_UNSUPPORTED, E_STATE, E_NONFINITE, E_CAPACITY = -1, -2, -3, -4, -5, -6, -7
 
 assert C.sizeof(IntParams) == 332 and C.sizeof(FlpParams) == 176
 
@@ -243,6 +243,7 @@ def lib():
         L.d3q19_server_get_batch.argtypes = [V, P(V)]
         L.d3q19_solve.argtypes = [C.c_int, P(IntParams), P(FlpParams), P(Options), V, V, P(SolveParams), P(Result)]
         L.d3q19_clock.restype = C.c_double
+        L.d3q19_stream_probe.argtypes = [C.c_int] * 9 + [P(C.c_double), P(C.c_double)]
         L.d3q19_strerror.argtypes = [C.c_int]
         L.d3q19_strerror.restype = C.c_char_p
         L.d3q19_last_error.restype = C.c_char_p
@@ -455,9 +456,10 @@ class Sim:
         return ms.value
 
     def time_steps(self, t_first, n):
-        tot, col = C.c_double(), C.c_double()
-        check(self.L.d3q19_time_steps(self.h, t_first, n, C.byref(tot), C.byref(col)), "time_steps")
-        return tot.value, col.value
+        """n real steps, each kernel timed: (total ms, mean step-kernel ms, mean IBM-pass ms)."""
+        tot, k = C.c_double(), (C.c_double * 2)()
+        check(self.L.d3q19_time_steps(self.h, t_first, n, C.byref(tot), k), "time_steps")
+        return tot.value, k[0], k[1]
 
     def observables(self):
         o = Observables()
@@ -661,3 +663,12 @@ def solve(case, sp, opt=None, device=0):
 
 def clock():
     return lib().d3q19_clock()
+
+
+def stream_probe(nx, ny, nz, streams=19, width=1, pull=True, lds_bytes=0, launches=20, device=0):
+    """Attainable bandwidth of the collide's access pattern without its arithmetic
+    (d3q19_stream_probe): returns (GB/s counting every byte read and written, ms per launch)."""
+    g, ms = C.c_double(), C.c_double()
+    check(lib().d3q19_stream_probe(device, nx, ny, nz, streams, width, int(pull), lds_bytes, launches, C.byref(g), C.byref(ms)),
+          "stream_probe")
+    return g.value, ms.value

@@ -16,6 +16,9 @@ extern "C" {
 #endif
 
 #define HOST_WORD 128
+/* largest marker set (the reference's automatic choice tops out at 1024, D3Q19-OpenCL.c:564-571);
+ * callers size the marker buffer of host_sphere_discretization as 4 * HOST_MAX_MARKERS floats */
+#define HOST_MAX_MARKERS 1024
 
 /* host_param_struct, D3Q19-OpenCL_header.h:75-94 (fields not read by kernels), plus the
  * configuration-mode options that the reference fixes at compile time. */
@@ -65,7 +68,8 @@ int host_parameter_checking(const int_param_struct* ip, const flp_param_struct* 
 /* SystemSize rule, D3Q19-OpenCL.c:119-128 */
 void host_system_size(int_param_struct* ip);
 /* sphere_discretization, D3Q19-OpenCL.c:558-610: marker count, PointArea, scaled points.
- * `points` must hold 4*1024 floats.  Returns 0, or -1 on a missing / malformed file. */
+ * `points` must hold 4 * HOST_MAX_MARKERS floats.  Returns 0, or -1 on a missing / malformed file
+ * or a marker_count above HOST_MAX_MARKERS. */
 int host_sphere_discretization(int_param_struct* ip, flp_param_struct* fp, const host_param_struct* hp, float* points);
 /* Work-group split of the marker kernel (D3Q19-OpenCL.c:536-550) for a device whose marker
  * kernel work-group limit is `max_wg`. */

@@ -183,8 +183,13 @@ int host_initialize_data(const char* path, int_param_struct* ip, flp_param_struc
 
 int host_parameter_checking(const int_param_struct* ip, const flp_param_struct* fp, const host_param_struct* hp) {
     (void)fp;
-    (void)hp;
     int errors = 0;
+    /* host_sphere_discretization fills 4 floats per marker into a buffer sized for the reference's
+     * largest automatic marker set (1024, D3Q19-OpenCL.c:564-571) */
+    if (hp->MarkerCount < 0 || hp->MarkerCount > HOST_MAX_MARKERS) {
+        fprintf(stderr, "Error: marker_count must be between 0 (automatic) and %d.\n", HOST_MAX_MARKERS);
+        errors++;
+    }
     if (ip->BoundaryConds[0] + ip->BoundaryConds[1] + ip->BoundaryConds[2] > 1) {
         fprintf(stderr, "Error: More than 1 pair of faces with velocity boundaries not yet supported.\n");
         errors++;
@@ -215,6 +220,10 @@ int host_sphere_discretization(int_param_struct* ip, flp_param_struct* fp, const
     float area = (float)(M_PI * d * d);
     int n = 1;
     if (hp->MarkerCount > 0) {
+        if (hp->MarkerCount > HOST_MAX_MARKERS) {
+            fprintf(stderr, "Error: marker_count above %d\n", HOST_MAX_MARKERS);
+            return -1;
+        }
         n = hp->MarkerCount;
     } else {
         int p2 = (int)ceil(log2(area));

@@ -59,7 +59,7 @@ int main(int argc, char* argv[]) {
     if (host_parameter_checking(&ip, &fp, &hp) > 0) return EXIT_FAILURE;
     host_system_size(&ip);
 
-    float* points = (float*)calloc(4 * 1024, sizeof(float));
+    float* points = (float*)calloc(4 * HOST_MAX_MARKERS, sizeof(float));
     if (host_sphere_discretization(&ip, &fp, &hp, points) != 0) return EXIT_FAILURE;
     host_workgroup_split(&ip, 256);
 

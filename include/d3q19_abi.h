@@ -241,8 +241,10 @@ int d3q19_sync(d3q19_sim* sim);
 int d3q19_plane_mean_u(d3q19_sim* sim, double* means);
 /* Total fluid mass and momentum over interior nodes (4 doubles) from the pre-collision f. */
 int d3q19_moments(d3q19_sim* sim, double* out4);
-/* Watchdog: returns D3Q19_E_NONFINITE if any interior node has a non-finite or non-positive
- * density. */
+/* Watchdog: returns D3Q19_E_STATE if a sphere has moved a lattice unit or more in one step since
+ * the simulation was created (its brick marks then missed part of its stencils), else
+ * D3Q19_E_NONFINITE if any interior node has a non-finite or non-positive density.  Allocates
+ * nothing. */
 int d3q19_check_finite(d3q19_sim* sim);
 
 /* Shear-cell observables, reduced on the device in FP64 from the stored populations.
@@ -352,7 +354,10 @@ typedef struct {
 } d3q19_solve_params;
 typedef struct {
     int64_t id;
-    int32_t status;             /* 0 converged, 1 max_steps reached, <0 error code */
+    int32_t status;             /* 0 converged, 1 max_steps reached, <0 error code: D3Q19_E_NONFINITE
+                                   (non-finite or non-positive density, or non-finite wall stress,
+                                   at a check), D3Q19_E_STATE (a sphere moved a lattice unit or
+                                   more in one step; the run is not trusted) */
     int32_t steps;
     d3q19_observables obs;
     double t_submit, t_admit, t_finish;   /* d3q19_clock() seconds */
@@ -379,12 +384,30 @@ int d3q19_solve(int device, const int_param_struct* ip, const flp_param_struct* 
 double d3q19_clock(void);
 
 /* Timing on the simulation's own stream (HIP events recorded at each kernel's own start and end,
- * so dispatch gaps are not counted). Runs n (1..4096) collide_stream launches and returns the
- * mean kernel duration in milliseconds. */
+ * so dispatch gaps are not counted).
+ * d3q19_time_collide runs n (1..4096) collide_stream launches -- the collide+stream kernel alone,
+ * without the particle dynamics and the IBM pass -- and returns their mean duration in
+ * milliseconds.  It advances the lattice state by those n collides (the spread force pending
+ * before it is consumed by the first): a kernel benchmark, whose state a simulation should not
+ * continue from.
+ * d3q19_time_steps runs n (1..4096) real full steps t_first .. t_first+n-1 (the state advances
+ * exactly as d3q19_run's would), each kernel launched directly with its own events: total_ms is
+ * the time from before the first step to after the last; kernel_ms (2 doubles, may be NULL) receives
+ * the mean duration of the step kernel (collide+stream with the spheres' turn) and of the IBM pass
+ * (0 without spheres). */
 int d3q19_time_collide(d3q19_sim* sim, int n, int flags, double* ms_per_launch);
-/* Runs n full steps from t_first and returns total milliseconds plus the mean collide launch
- * duration (events recorded around every collide launch on the same stream). */
-int d3q19_time_steps(d3q19_sim* sim, int t_first, int n, double* total_ms, double* collide_ms);
+int d3q19_time_steps(d3q19_sim* sim, int t_first, int n, double* total_ms, double* kernel_ms);
+
+/* Attainable-bandwidth probe (no counterpart in the reference; a measurement utility).  A
+ * streaming kernel with the collide+stream kernel's memory traffic and none of its arithmetic
+ * on an nx*ny*nz box of fp32 planes: `streams` (19, or 1 for a plain copy) planes read and as many
+ * written, `width` (1, 2, 4) consecutive nodes per lane, read at the D3Q19 pull offsets when
+ * `pull` is set; width -64 / -256 keeps one node per lane with the 19 populations interleaved in
+ * chunks of 64 / 256 nodes instead of planes; `lds_bytes` (0..65536) of dynamic LDS per
+ * 256-thread block caps the resident blocks.  Allocates and frees its own buffers.  *gbs = bytes read + written per launch / mean
+ * launch time (HIP events around `launches` launches), *ms = that mean (ms may be NULL). */
+int d3q19_stream_probe(int device, int nx, int ny, int nz, int streams, int width, int pull, int lds_bytes, int launches,
+                       double* gbs, double* ms);
 
 /* Sizes of the state (bytes of device memory held by the simulation). */
 int d3q19_footprint(const int_param_struct* intDat, const d3q19_options* opt, size_t* bytes);

@@ -105,6 +105,9 @@ def test_autotune_choice_and_state_unchanged():
     print("autotuned:", t.as_dict(), flush=True)
     assert t.tile in (64, 128, 256) and t.span in (1, 2, 4, 8) and t.grid_collide >= 0 and t.collide_ms > 0
     assert b1.tuning().tile == t.tile
+    # the trial launches write neither u (the velocity the next IBM pass reads) nor tau nor force
+    for a, c in zip(i1, i2):
+        assert np.array_equal(b1.read_u(a), b2.read_u(c))
     b1.run(10)
     b2.run(10)
     for a, c in zip(i1, i2):
@@ -219,3 +222,49 @@ def test_server_result_alone_equals_result_in_mix():
         assert a.status == m.status and a.steps == m.steps
         assert a.obs.wall_stress == m.obs.wall_stress
         assert a.obs.drag_coefficient == m.obs.drag_coefficient
+
+
+def kicked_cell():
+    """A shear cell whose one sphere starts at 1.5 lattice units per step: it crosses more than one
+    lattice unit in its first step, beyond what its dilated brick marks cover."""
+    c = shear_cell(24, np_=1, diam=6.0, markers=128, uw=0.02, seed=2)
+    k = c.parKin[: 16 * c.np].reshape(4, c.np, 4)
+    k[1, 0, :3] = (1.5, 0.0, 0.0)
+    c.parKin[: 16 * c.np] = k.ravel()
+    return c
+
+
+def test_displacement_watchdog_ends_requests_with_an_error():
+    """A sphere that moves a lattice unit or more in one step makes d3q19_check_finite, the
+    sequential solve and the server report D3Q19_E_STATE instead of a converged result; the flag
+    travels with a checkpoint."""
+    c = kicked_cell()
+    s = d3q19.Sim(c)
+    s.write_particles(c.parKin)
+    s.run(1, 4)
+    s.sync()
+    assert s.check_finite() == d3q19.E_STATE
+    img = s.checkpoint()
+    s.close()
+    s2 = d3q19.Sim(c)
+    assert s2.check_finite() == 0
+    s2.restore(img)
+    assert s2.check_finite() == d3q19.E_STATE
+    s2.close()
+    sp = d3q19.solve_params(tolerance=1e-4, check_interval=20, min_steps=0, max_steps=200)
+    r = d3q19.solve(c, sp)
+    print("solve: status %d after %d steps" % (r.status, r.steps), flush=True)
+    assert r.status == d3q19.E_STATE and r.steps == 20
+    srv = d3q19.Server(storage=0, arena_bytes=1 << 28)
+    calm = shear_cell(16, np_=1, diam=6.0, seed=3)
+    ids = [srv.submit(calm, sp), srv.submit(c, sp)]
+    done = {}
+    for _ in range(50):
+        for res in srv.progress():
+            done[res.id] = res
+        if len(done) == 2:
+            break
+    srv.close()
+    print("server: statuses %s" % {k: (v.status, v.steps) for k, v in done.items()}, flush=True)
+    assert done[ids[1]].status == d3q19.E_STATE and done[ids[1]].steps == 20
+    assert done[ids[0]].status in (0, 1)

@@ -329,3 +329,34 @@ def test_host_program_reports_missing_gpu(gpu_available):
         shutil.copy(os.path.join(GOLD, "inputs", "input_file.txt"), tmp)
         r = subprocess.run([exe], cwd=tmp, capture_output=True, text=True, timeout=120)
     assert r.returncode != 0
+
+
+def test_marker_count_above_buffer_capacity_rejected():
+    """The marker buffer holds 4 x 1024 floats (the reference's largest automatic set); a larger
+    marker_count is an input error, not a buffer overrun."""
+    with pytest.raises(ValueError):
+        d3q19.Case(lines=["particle_diameter 4.0", "num_particles 1", "marker_count 2048"])
+    with pytest.raises(ValueError):
+        d3q19.Case(lines=["particle_diameter 4.0", "num_particles 1", "marker_count -3"])
+
+
+def test_sphere_beyond_brick_marking_capacity_rejected_before_device_use():
+    """The brick marking lists at most 64 bricks of 4 nodes per axis for one sphere; d3q19_create
+    refuses a sphere whose stencils could span more (diameter above about 240 lattice units) instead
+    of marking it incompletely: at d = 250 the markers' floor coordinates span 253 nodes, their
+    padded stencils 261 nodes, which can touch 67 bricks.  d = 200 is accepted up to the device
+    check."""
+    L = d3q19.lib()
+    c = d3q19.Case(lines=["total_lattice_size 40 40 41", "boundary_conditions_xyz 0 0 1", "num_particles 1",
+                          "particle_diameter 8.0", "marker_count 128", "initial_particle_distribution 3"])
+    h = C.c_void_p()
+    for diam, want_reject in ((250.0, True), (200.0, False)):
+        c.fp.ParticleDiam = diam
+        pts = (c.points.reshape(-1, 4) * (diam / 8.0)).astype(np.float32).ravel()
+        rc = L.d3q19_create(0, C.byref(c.ip), C.byref(c.fp), C.byref(c.opt), pts.ctypes.data_as(C.c_void_p), C.byref(h))
+        if want_reject:
+            assert rc == d3q19.E_UNSUPPORTED and b"brick marking" in L.d3q19_last_error()
+        else:
+            assert rc != d3q19.E_UNSUPPORTED
+            if rc == 0:
+                L.d3q19_destroy(h)
