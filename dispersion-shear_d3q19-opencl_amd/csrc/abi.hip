@@ -11,6 +11,7 @@
 #include <cmath>
 #include <utility>
 #include <string>
+#include <mutex>
 #include <vector>
 #include "internal.h"
 
@@ -999,6 +1000,68 @@ int d3q19_plane_mean_u(d3q19_sim* s, double* means) {
     (void)hipFree(d);
     if (rc) return rc;
     if (e != hipSuccess || e2 != hipSuccess) return fail(D3Q19_E_DEVICE, "plane_mean_u copy failed");
+    return 0;
+}
+
+// d3q19_plane_mean_u without allocation.  The plane means are staged in `scratch`, a file-scope
+// device array of PLANE_MEAN_MAX_NZ planes x 3 components.  Host code must not pass the array's name
+// to kernels or copies (in host code it is a stand-in, not a device address): its address on each
+// device comes from hipGetSymbolAddress, taken once per device with that device current, at the
+// entry point's first call there.  Every call on a device shares that device's array, so calls
+// that overlap on one device (different simulations on different streams) race; the two shell
+// planes it reports as zero are cleared by a small kernel on every call.
+}  // extern "C"
+
+namespace {
+constexpr int PLANE_MEAN_MAX_NZ = 2048;
+static __device__ __align__(256) double scratch[3 * PLANE_MEAN_MAX_NZ];
+constexpr int MAX_DEVICES = 64;
+double* g_scratch[MAX_DEVICES] = {};
+std::once_flag g_scratch_once[MAX_DEVICES];
+int g_scratch_rc[MAX_DEVICES] = {};
+
+__global__ void zero_shell_planes_kernel(double* out, int Nz) {
+    const int c = threadIdx.x;
+    if (c < 3) {
+        out[c * Nz] = 0.0;
+        out[c * Nz + Nz - 1] = 0.0;
+    }
+}
+
+// the device address of `scratch` on the current device (set up once per device)
+int scratch_on(int device, double** ptr) {
+    if (device < 0 || device >= MAX_DEVICES) return fail(D3Q19_E_UNSUPPORTED, "device index above the scratch table");
+    std::call_once(g_scratch_once[device], [device]() {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            g_scratch_rc[device] = fail(D3Q19_E_DEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+            return;
+        }
+        g_scratch[device] = (double*)p;
+    });
+    if (g_scratch_rc[device]) return g_scratch_rc[device];
+    *ptr = g_scratch[device];
+    return 0;
+}
+}  // namespace
+
+extern "C" {
+
+int d3q19_plane_mean_u_noalloc(d3q19_sim* s, double* means) {
+    CHECK_SIM(s);
+    if (!means) return fail(D3Q19_E_INVALID, "null output");
+    if (s->Nz > PLANE_MEAN_MAX_NZ)
+        return fail(D3Q19_E_CAPACITY, "d3q19_plane_mean_u_noalloc: LatticeSize[2] above " + std::to_string(PLANE_MEAN_MAX_NZ) +
+                                          " (its device scratch holds that many planes); use d3q19_plane_mean_u");
+    double* d = nullptr;
+    int rc = scratch_on(s->device, &d);
+    if (rc) return rc;
+    hipLaunchKernelGGL(zero_shell_planes_kernel, dim3(1), dim3(64), 0, s->stream, d, s->Nz);
+    HIP_OK(hipGetLastError());
+    if ((rc = launch_plane_mean(s, d))) return rc;
+    HIP_OK(hipMemcpyAsync(means, d, 3 * (size_t)s->Nz * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    HIP_OK(hipStreamSynchronize(s->stream));
     return 0;
 }
 

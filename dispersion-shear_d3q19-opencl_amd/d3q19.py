@@ -209,6 +209,7 @@ def lib():
         L.d3q19_step.argtypes = [V, C.c_int, C.c_int]
         L.d3q19_run.argtypes = [V, C.c_int, C.c_int, C.c_int]
         L.d3q19_plane_mean_u.argtypes = [V, V]
+        L.d3q19_plane_mean_u_noalloc.argtypes = [V, V]
         L.d3q19_moments.argtypes = [V, V]
         L.d3q19_time_collide.argtypes = [V, C.c_int, C.c_int, P(C.c_double)]
         L.d3q19_time_steps.argtypes = [V, C.c_int, C.c_int, P(C.c_double), P(C.c_double)]
@@ -443,6 +444,12 @@ class Sim:
     def plane_mean_u(self):
         out = np.zeros(3 * self.case.N[2])
         check(self.L.d3q19_plane_mean_u(self.h, _ptr(out)), "plane_mean_u")
+        return out.reshape(3, -1)
+
+    def plane_mean_u_noalloc(self):
+        """d3q19_plane_mean_u_noalloc: the same plane means, staged in the library's device scratch."""
+        out = np.zeros(3 * self.case.N[2])
+        check(self.L.d3q19_plane_mean_u_noalloc(self.h, _ptr(out)), "plane_mean_u_noalloc")
         return out.reshape(3, -1)
 
     def moments(self):

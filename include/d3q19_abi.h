@@ -239,6 +239,14 @@ int d3q19_sync(d3q19_sim* sim);
  * planes 0 and Nz-1 zero), the quantity of compute_shear_stress (D3Q19-OpenCL.c:694-725);
  * needs u written on every node (D3Q19_STEP_WRITE_U on the last step). */
 int d3q19_plane_mean_u(d3q19_sim* sim, double* means);
+/* d3q19_plane_mean_u without any allocation.  No counterpart in the reference: it is for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  The plane means are staged in a device array of the library, 3 x 2048
+ * doubles (48 KiB) per GPU, held until the process ends; LatticeSize[2] above 2048 is refused with
+ * D3Q19_E_CAPACITY.  All calls on one GPU share that array: calls that overlap on one GPU (different
+ * simulations on different streams or threads) race and must be serialised by the caller.  Output
+ * and semantics are those of d3q19_plane_mean_u (bitwise). */
+int d3q19_plane_mean_u_noalloc(d3q19_sim* sim, double* means);
 /* Total fluid mass and momentum over interior nodes (4 doubles) from the pre-collision f. */
 int d3q19_moments(d3q19_sim* sim, double* out4);
 /* Watchdog: returns D3Q19_E_STATE if a sphere has moved a lattice unit or more in one step since
