@@ -47,6 +47,37 @@ def build_host():
          os.path.join(LIB, "libd3q19host.so")] + srcs + ["-lm"])
 
 
+LLVM = "/opt/rocm/llvm/bin"
+
+
+def check_kernel_registration(obj):
+    """Every kernel (and device variable) that the host half of `obj` registers by name must have a
+    definition in its gfx950 code object; a launch of a registered kernel without one aborts in the
+    HIP runtime ("Cannot find Symbol"), so such an object fails the build instead.  The host half
+    holds the registered names as string literals; the code object is the gfx950 entry of the
+    object's .hip_fatbin bundle, whose kernels carry <name>.kd descriptors."""
+    import tempfile
+    r = subprocess.run([os.path.join(LLVM, "llvm-readelf"), "-p", ".rodata.str1.1", obj], capture_output=True, text=True)
+    host = set()
+    for line in r.stdout.splitlines():
+        if "]" in line:
+            name = line.split("]", 1)[1].strip()
+            if name.startswith("_Z"):
+                host.add(name)
+    if not host:
+        return
+    with tempfile.TemporaryDirectory() as tmp:
+        fb, co = os.path.join(tmp, "fatbin"), os.path.join(tmp, "gfx950.co")
+        run([os.path.join(LLVM, "llvm-objcopy"), "--dump-section", ".hip_fatbin=" + fb, obj, os.path.join(tmp, "discard.o")])
+        run([os.path.join(LLVM, "clang-offload-bundler"), "--type=o", "--targets=hipv4-amdgcn-amd-amdhsa--gfx950", "--input=" + fb,
+             "--output=" + co, "--unbundle"])
+        syms = run([os.path.join(LLVM, "llvm-readelf"), "--syms", co]).stdout.split()
+    device = {t[:-3] if t.endswith(".kd") else t for t in syms}
+    missing = sorted(n for n in host if n not in device)
+    if missing:
+        raise RuntimeError("%s registers kernels without gfx950 code: %s" % (os.path.basename(obj), ", ".join(missing[:5])))
+
+
 def build_hip(jobs=4):
     os.makedirs(OBJ, exist_ok=True)
     os.makedirs(LIB, exist_ok=True)
@@ -58,6 +89,8 @@ def build_hip(jobs=4):
 
     with ThreadPoolExecutor(max_workers=jobs) as ex:
         objs = list(ex.map(one, HIP_SOURCES))
+    for o in objs:
+        check_kernel_registration(o)
     run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", os.path.join(LIB, "libd3q19.so")] + objs)
 
 

@@ -144,12 +144,28 @@ __device__ __forceinline__ void ibm_marker(const IbmArgs& a, const D3Q19_GLOBAL 
     }
 }
 
+// Sphere p belongs to this pass: its stencils reach at most up to a z plane in [zlo, zhi).  The top
+// plane of a marker's stencil is floor(z_marker) + 2 <= floor(z_sphere + zreach) + 2 (planes beyond a
+// wall are dropped); a stencil that wraps around a periodic z axis and a non-finite position belong
+// to the last slab, whose pass runs after every slab has collided.
+__device__ __forceinline__ bool pass_owns(const IbmArgs& a, const D3Q19_GLOBAL double* pos, int p) {
+    if (a.zlo <= 0 && a.zhi >= a.g.nz) return true;
+    const double zc = pos[3 * p + 2];
+    int top = a.g.nz - 1;
+    if (zc > -1e7 && zc < 1e7) {
+        const int hi = (int)floor(zc + a.zreach) + 2, lo = (int)floor(zc - a.zreach) - 1;
+        if (!a.g.per[2]) top = hi < a.g.nz - 1 ? hi : a.g.nz - 1;
+        else if (hi < a.g.nz && lo >= 0) top = hi;
+    }
+    return top >= a.zlo && top < a.zhi;
+}
+
 template <class R, int MODE>
 __global__ void __launch_bounds__(64 * IBM_WAVES) ibm_marker_kernel(IbmArgs a) {
     const int lane = threadIdx.x & 63;
     const int gid = blockIdx.x * IBM_WAVES + (int)(threadIdx.x >> 6);
     if (gid >= a.np * a.ppp) return;  // uniform per wavefront
-    if (gid == 0 && lane == 0) {
+    if (gid == 0 && lane == 0 && a.commit) {
         // the step's epoch (read as epochs[0] + 1 by the kernels before this one) is committed,
         // and its spread force is now the pending one
         const uint32_t e = a.epochs[0] + 1u;
@@ -159,6 +175,7 @@ __global__ void __launch_bounds__(64 * IBM_WAVES) ibm_marker_kernel(IbmArgs a) {
     // wavefronts that run together take markers of different spheres, and a sphere's markers are
     // visited with a stride of about 0.618 ppp, so concurrent atomics rarely meet on one node
     const int p = gid % a.np, k = gid / a.np;
+    if (!pass_owns(a, a.pos, p)) return;  // uniform per wavefront
     const int m = (int)(((long long)k * a.mstride) % a.ppp);
     const long long nint = a.g.nint;
     ibm_marker<R, MODE>(a, a.pos, a.vel, p, m, lane, [&](const int* sn, double w, const double* F) {
@@ -249,12 +266,13 @@ __global__ void __launch_bounds__(64 * TILE_MARKERS) ibm_tile_kernel(IbmArgs a) 
     __shared__ long long tile[3][TILE_NODES];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int b = blockIdx.x;
-    if (b == 0 && threadIdx.x == 0) {
+    if (b == 0 && threadIdx.x == 0 && a.commit) {
         const uint32_t e = a.epochs[0] + 1u;  // commits the step's epoch, as ibm_marker_kernel does
         a.epochs[0] = e;
         a.epochs[1] = e;
     }
     const int p = b % a.np, grp = b / a.np;
+    if (!pass_owns(a, a.pos, p)) return;  // uniform over the block
     const int j = grp * TILE_MARKERS + w;
     const bool has = j < a.ppp;
     ibm_tile_group<R, MODE>(a, a.pos, a.vel, p, grp, has ? a.perm[j] : 0, has, lane, tile);
@@ -325,19 +343,26 @@ IbmArgs make_ibm_args(Sim* s) {
     a.facc = gp(s->facc);
     a.mforce = gp(s->mforce);
     a.epochs = gp(s->epochs);
+    a.zreach = s->mark_rout;
+    a.zlo = 0;
+    a.zhi = 1 << 30;
+    a.commit = 1;
     return a;
 }
 
 // a kernel launch, timed by events at the kernel's own start and end when ev0 is set
 template <class K, class... Args>
 static void launch_timed(K kernel, dim3 grid, dim3 block, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1, Args... args) {
-    if (ev0) hipExtLaunchKernelGGL(kernel, grid, block, 0, st, ev0, ev1, 0, args...);
+    if (ev0 && ev1) hipExtLaunchKernelGGL(kernel, grid, block, 0, st, ev0, ev1, 0, args...);
     else hipLaunchKernelGGL(kernel, grid, block, 0, st, args...);
 }
 
-int launch_ibm(Sim* s, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1) {
+int launch_ibm(Sim* s, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1, int zlo, int zhi, int commit) {
     if (s->np == 0) return 0;
-    const IbmArgs a = make_ibm_args(s);
+    IbmArgs a = make_ibm_args(s);
+    a.zlo = zlo;
+    a.zhi = zhi;
+    a.commit = commit;
     const bool dbl = s->storage == D3Q19_STORE_FP64;
     if (s->tile_ok) {
         const int blocks = s->np * ((s->ppp + TILE_MARKERS - 1) / TILE_MARKERS);

@@ -59,6 +59,34 @@ __host__ __device__ constexpr int opposite(int q) {
     return r;
 }
 
+// the 9 direction pairs (q, opposite(q)) of the velocity set, one member each: 1, 3, 5, 7, 8, 9, 10, 15, 16
+__host__ __device__ constexpr int pair_dir(int k) {
+    return k == 0 ? 1 : k == 1 ? 3 : k == 2 ? 5 : k == 3 ? 7 : k == 4 ? 8 : k == 5 ? 9 : k == 6 ? 10 : k == 7 ? 15 : 16;
+}
+
+// fused multiply-add written out (the collide compiles with implicit contraction off, so the
+// rounding of every node's arithmetic is fixed by its source, whatever kernel it is inlined into)
+template <class R> __device__ __forceinline__ R fma_(R a, R b, R c);
+template <> __device__ __forceinline__ float fma_<float>(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+template <> __device__ __forceinline__ double fma_<double>(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
+// c_q . v for a velocity-set direction q, with only its nonzero components (a product by a zero
+// component cannot be dropped by the compiler under IEEE semantics)
+template <int s, class R> __device__ __forceinline__ R signed_(R v) {
+    if constexpr (s > 0) return v;
+    else return -v;
+}
+template <int q, class R> __device__ __forceinline__ R cdot(R x, R y, R z) {
+    constexpr int X = cx(q), Y = cy(q), Z = cz(q);
+    if constexpr (X != 0 && Y != 0) return signed_<X>(x) + signed_<Y>(y);
+    else if constexpr (X != 0 && Z != 0) return signed_<X>(x) + signed_<Z>(z);
+    else if constexpr (Y != 0 && Z != 0) return signed_<Y>(y) + signed_<Z>(z);
+    else if constexpr (X != 0) return signed_<X>(x);
+    else if constexpr (Y != 0) return signed_<Y>(y);
+    else if constexpr (Z != 0) return signed_<Z>(z);
+    else return (R)0;
+}
+
 // compile-time loop
 template <int I, int N> struct Unroll {
     template <class F> __device__ __forceinline__ static void run(F&& f) {
@@ -178,7 +206,8 @@ struct CollideArgs {
     int vt;                // per-node tau (runtime form of the kernels' VT template argument)
     int bx, by;            // block tile
     int nbx, nby;          // tiles per plane
-    int nblk;
+    int nblk;              // tiles of this launch
+    int z0;                // first z plane of this launch's tiles (a z-slab of the lattice)
 };
 
 struct IbmArgs {
@@ -197,6 +226,12 @@ struct IbmArgs {
     D3Q19_GLOBAL long long* facc;
     D3Q19_GLOBAL double* mforce;
     D3Q19_GLOBAL uint32_t* epochs;
+    // slab-owned passes: this launch runs the spheres whose stencils reach at most up to a z plane
+    // in [zlo, zhi) (the top of a stencil is floor(z + zreach) + 2; a stencil that leaves the lattice
+    // in z belongs to the last slab); commit: this launch commits the step's epoch
+    double zreach;
+    int zlo, zhi;
+    int commit;
 };
 
 struct PartArgs {
@@ -353,6 +388,12 @@ struct Sim {
     const void* graph_pos = nullptr;
     int graph_tofs = 0;
     int parity = 0;            // 0: current post-collision field in fA
+    // collide in z-slabs overlapped with the IBM pass (single simulations with spheres)
+    static constexpr int MAX_SLABS = 16;
+    int slabs = 1;
+    int zb[MAX_SLABS + 1] = {};        // slab k holds planes [zb[k], zb[k+1])
+    hipStream_t aux = nullptr;         // the IBM pass's stream
+    hipEvent_t sev[MAX_SLABS + 1] = {};  // slab k collided; aux done
     int block[3] = {64, 4, 1};
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     // batch membership: device memory comes from the batch's arena, the batch's stream and epoch
@@ -395,15 +436,21 @@ int launch_export_force(Sim* s, float* ref_out);
 int launch_plane_mean(Sim* s, double* dev_out);
 int launch_moments(Sim* s, const void* fsrc, double* dev_out);
 int launch_check_finite(Sim* s, const void* fsrc, int* dev_flag);
-int launch_ibm(Sim* s, hipStream_t st, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
+// zlo / zhi / commit: the slab-owned pass of the overlapped step (see IbmArgs); the defaults run
+// every sphere and commit the epoch
+int launch_ibm(Sim* s, hipStream_t st, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr, int zlo = 0, int zhi = 1 << 30,
+               int commit = 1);
 int launch_sphere_pre(Sim* s, int do_dyn, int do_mark, hipStream_t st);       // dynamics, brick marks
 int launch_sphere_post(Sim* s, int do_reduce, int do_contacts, hipStream_t st); // force sums, contacts
 int launch_zone_init(Sim* s, hipStream_t st);  // zones of the current positions, both generations
 int launch_sphere_prime(Sim* s, hipStream_t st);                                // dilated marks, next epoch
 // fused step: the spheres' turn (post of the last step if pending, dynamics into the other
 // position buffer, dilated marks of the epoch after this one) in np extra blocks of the collide
+// nslab > 1: the collide tiles go out as s->slabs launches of the planes [zb[k], zb[k+1]) (the
+// spheres' turn with the first), each followed by a record of slab_ev[k] on st; ev0 / ev1 time the
+// first launch's start and the last one's end
 int launch_step_fused(Sim* s, const void* fin, void* fout, int write_u_all, int do_post, hipStream_t st, hipEvent_t ev0 = nullptr,
-                      hipEvent_t ev1 = nullptr);
+                      hipEvent_t ev1 = nullptr, int nslab = 1, hipEvent_t* slab_ev = nullptr);
 int launch_fill_equilibrium(Sim* s, void* f);
 int launch_fill_tau(Sim* s);
 

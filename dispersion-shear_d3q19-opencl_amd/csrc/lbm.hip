@@ -15,6 +15,12 @@
 #include <hip/hip_ext.h>
 #include "sphere_dev.h"
 
+// No implicit contraction of multiplies and adds into FMAs in this file: the per-node arithmetic
+// is inlined into several kernels (single simulation, batched cells, timing trials), and the same
+// source must round the same way in all of them, so that a cell's results are bitwise those of
+// the same cell alone.  The FMAs that pay are written out (fma_).
+#pragma clang fp contract(off)
+
 namespace d3q19 {
 
 // ---- population storage formats --------------------------------------------------------------
@@ -170,8 +176,10 @@ __device__ __forceinline__ void wall_layer(R* f, const float* uw, int calc_rho, 
         if constexpr (cc(q, A) == inward) {
             constexpr int o = opposite(q);
             constexpr bool axial = (cc(q, a1) == 0 && cc(q, a2) == 0);
-            const R cu = un + (R)cc(q, a1) * ua1 + (R)cc(q, a2) * ua2;
-            R v = f[o] + rho * cu * (axial ? (R)(1.0 / 3.0) : (R)(1.0 / 6.0));
+            R cu = un;
+            if constexpr (cc(q, a1) != 0) cu = cu + signed_<cc(q, a1)>(ua1);
+            if constexpr (cc(q, a2) != 0) cu = cu + signed_<cc(q, a2)>(ua2);
+            R v = fma_(rho * cu, axial ? (R)(1.0 / 3.0) : (R)(1.0 / 6.0), f[o]);
             if (use_raw) v = (R)raw[((size_t)(S * 5 + k)) * psz + pidx];
             f[q] = v;
             k++;
@@ -264,17 +272,17 @@ __device__ __forceinline__ R tau_law(const FluidParams& fl, R srt) {
     } else if (model == D3Q19_VISC_CASSON) {
         const R ty = (R)fl.vp[0], eta = (R)fl.vp[1], m = (R)fl.vp[2];
         R a = sqrt_(ty * isrt);
-        if (m > (R)0) a = a * ((R)1 - exp_(-sqrt_(m * srt)));
+        if (m > (R)0) a = fma_(-a, exp_(-sqrt_(m * srt)), a);
         const R b = a + sqrt_(eta);
         nu = b * b;
     } else {
         const R ty = (R)fl.vp[0], k = (R)fl.vp[1], n = (R)fl.vp[2];
-        if (n == (R)0.5) nu = ty * isrt + k * rcp_(sqrt_(srt));
-        else if (n == (R)1.0) nu = ty * isrt + k;
-        else if (n == (R)2.0) nu = ty * isrt + k * srt;
-        else nu = ty * isrt + k * pow_(srt, n - (R)1);
+        if (n == (R)0.5) nu = fma_(ty, isrt, k * rcp_(sqrt_(srt)));
+        else if (n == (R)1.0) nu = fma_(ty, isrt, k);
+        else if (n == (R)2.0) nu = fma_(ty, isrt, k * srt);
+        else nu = fma_(ty, isrt, k * pow_(srt, n - (R)1));
     }
-    tau = (R)3 * nu + (R)0.5;
+    tau = fma_((R)3, nu, (R)0.5);
     if (tau < (R)fl.tau_min) tau = (R)fl.tau_min;
     if (fl.tau_max > 0.f && tau > (R)fl.tau_max) tau = (R)fl.tau_max;
     return tau;
@@ -293,7 +301,7 @@ __device__ __forceinline__ void mrt_relax(const R* v, R itau, R* out) {
             constexpr int c = mrow(k, q);
             if constexpr (c == 1) m += v[q];
             else if constexpr (c == -1) m -= v[q];
-            else if constexpr (c != 0) m += (R)c * v[q];
+            else if constexpr (c != 0) m = fma_((R)c, v[q], m);
         });
         constexpr double rate = (k == 0) ? 0.0 : (k == 1) ? 1.19 : (k == 2 || k == 12 || k == 13) ? 1.40
                               : (k == 4 || k == 6 || k == 11) ? 1.20 : (k >= 16) ? 1.98 : 1.0;
@@ -311,7 +319,7 @@ __device__ __forceinline__ void mrt_relax(const R* v, R itau, R* out) {
             constexpr int c = mrow(k, q);
             if constexpr (c == 1) o += a[k];
             else if constexpr (c == -1) o -= a[k];
-            else if constexpr (c != 0) o += (R)c * a[k];
+            else if constexpr (c != 0) o = fma_((R)c, a[k], o);
         });
         out[q] = o;
     });
@@ -391,12 +399,17 @@ __device__ __forceinline__ void collide_node(const CollideArgs& a, const void* f
         }
     }
 
+    // density and momentum summed in the reference's order (GPU_program.cl:343-350, 387-394); the
+    // strain rate takes the pair sums f_q + f_qb
     R rho = (R)0;
     Unroll<0, Q>::run([&](auto qc) __attribute__((always_inline)) { rho += f[decltype(qc)::value]; });
+    const R jx = f[1] - f[2] + f[7] + f[8] + f[9] + f[10] - f[11] - f[12] - f[13] - f[14];
+    const R jy = f[3] - f[4] + f[7] - f[8] + f[11] - f[12] + f[15] + f[16] - f[17] - f[18];
+    const R jz = f[5] - f[6] + f[9] - f[10] + f[13] - f[14] + f[15] - f[16] + f[17] - f[18];
     const R inv_rho = rcp_(rho);
-    const R ux = (f[1] - f[2] + f[7] + f[8] + f[9] + f[10] - f[11] - f[12] - f[13] - f[14] + (R)0.5 * gx) * inv_rho;
-    const R uy = (f[3] - f[4] + f[7] - f[8] + f[11] - f[12] + f[15] + f[16] - f[17] - f[18] + (R)0.5 * gy) * inv_rho;
-    const R uz = (f[5] - f[6] + f[9] - f[10] + f[13] - f[14] + f[15] - f[16] + f[17] - f[18] + (R)0.5 * gz) * inv_rho;
+    const R ux = fma_((R)0.5, gx, jx) * inv_rho;
+    const R uy = fma_((R)0.5, gy, jy) * inv_rho;
+    const R uz = fma_((R)0.5, gz, jz) * inv_rho;
     if (!probe && (write_u_all || (a.use_umask && umark == ep))) {
         // interleaved (x, y, z) per node: one write stream instead of three (the three separate
         // component arrays cost 7.5% of the fp32 collide on the 256x256x128 workload, whose marked
@@ -410,79 +423,103 @@ __device__ __forceinline__ void collide_node(const CollideArgs& a, const void* f
     // equilibrium (GPU_program.cl:978-1007), non-equilibrium n = feq - f, Guo term (1010-1038)
     const R usq = ux * ux + uy * uy + uz * uz;
     const R udg = ux * gx + uy * gy + uz * gz;
-    R ne[Q], fg[Q];
-    Unroll<0, Q>::run([&](auto qc) __attribute__((always_inline)) {
-        constexpr int q = decltype(qc)::value;
-        const R cu = (R)cx(q) * ux + (R)cy(q) * uy + (R)cz(q) * uz;
-        const R cg = (R)cx(q) * gx + (R)cy(q) * gy + (R)cz(q) * gz;
-        R feq;
-        if constexpr (q == 0) {
-            feq = rho * (R)(1.0 / 3.0) * ((R)1 - (R)1.5 * usq);
-            fg[q] = -udg;
-        } else {
-            feq = rho * (R)wq(q) * ((R)1 + (R)3 * cu + (R)4.5 * cu * cu - (R)1.5 * usq);
-            fg[q] = (cg - udg + (R)3 * cu * cg) * (R)(q < 7 ? 1.0 / 6.0 : 1.0 / 12.0);
-        }
-        ne[q] = feq - f[q];
-    });
-
     R tau = (R)a.fl.tau0;
     const auto taup = to_global<R>(a.tau);
-    if (VT == 1 || (VT == 2 && a.vt)) {
-        const R told = taup[n];
-        const R itold = rcp_(told);
-        R srt;
-        if constexpr (COLL == D3Q19_COLL_BGK) {
-            // collision change of BGK: ne/tau + (1 - 1/(2 tau)) F.  Its second moment minus (uF + Fu)
-            // is (Pi(ne) - (uF + Fu)/2)/tau because the Guo term's second moment is uF + Fu.
-            R pxx = 0, pyy = 0, pzz = 0, pxy = 0, pxz = 0, pyz = 0;
-            Unroll<1, Q>::run([&](auto qc) __attribute__((always_inline)) {
-                constexpr int q = decltype(qc)::value;
-                if constexpr (cx(q) != 0) pxx += ne[q];
-                if constexpr (cy(q) != 0) pyy += ne[q];
-                if constexpr (cz(q) != 0) pzz += ne[q];
-                if constexpr (cx(q) * cy(q) == 1) pxy += ne[q];
-                if constexpr (cx(q) * cy(q) == -1) pxy -= ne[q];
-                if constexpr (cx(q) * cz(q) == 1) pxz += ne[q];
-                if constexpr (cx(q) * cz(q) == -1) pxz -= ne[q];
-                if constexpr (cy(q) * cz(q) == 1) pyz += ne[q];
-                if constexpr (cy(q) * cz(q) == -1) pyz -= ne[q];
-            });
-            const R sxx = pxx - ux * gx, syy = pyy - uy * gy, szz = pzz - uz * gz;
-            const R sxy = pxy - (R)0.5 * (ux * gy + uy * gx);
-            const R sxz = pxz - (R)0.5 * (ux * gz + uz * gx);
-            const R syz = pyz - (R)0.5 * (uy * gz + uz * gy);
-            const R n2 = sxx * sxx + syy * syy + szz * szz + (R)2 * (sxy * sxy + sxz * sxz + syz * syz);
-            srt = sqrt_(n2) * itold * (R)2.1213203435596424 * inv_rho;
-        } else {
-            R ss[Q];
-            R v[Q];
-            Unroll<0, Q>::run([&](auto qc) __attribute__((always_inline)) {
-                constexpr int q = decltype(qc)::value;
-                v[q] = ne[q] - (R)0.5 * fg[q];
-            });
-            mrt_relax<R>(v, itold, ss);
-            Unroll<0, Q>::run([&](auto qc) __attribute__((always_inline)) { ss[decltype(qc)::value] += fg[decltype(qc)::value]; });
-            srt = strain_rate<R>(ss, ux, uy, uz, gx, gy, gz, inv_rho);
-        }
-        tau = tau_law<R>(a.fl, srt);
-        if (!probe) taup[n] = tau;
-    }
-    const R itau = rcp_(tau);
+    const bool local_tau = VT == 1 || (VT == 2 && a.vt);
     R fo[Q];
     if constexpr (COLL == D3Q19_COLL_BGK) {
-        const R pf = (R)1 - (R)0.5 * itau;
-        Unroll<0, Q>::run([&](auto qc) __attribute__((always_inline)) {
-            constexpr int q = decltype(qc)::value;
-            fo[q] = f[q] + ne[q] * itau + pf * fg[q];
+        // BGK in pair form.  A direction q and its opposite qb have c_qb = -c_q, so
+        //   f_eq(q), f_eq(qb) = S +- A,   S = w rho (1 - 1.5 u.u + 4.5 (c.u)^2),   A = 3 w rho (c.u)
+        //   F(q), F(qb)       = G +- H,   G = w' (3 (c.u)(c.g) - u.g),             H = w' (c.g)
+        // (w' = 1/6 on the axes, 1/12 on the diagonals; the Guo term of GPU_program.cl:1010-1038),
+        // each evaluated once per pair; the update is f + (f_eq - f)/tau + (1 - 1/(2 tau)) F as in
+        // GPU_program.cl:943-944, which keeps the small non-equilibrium part f_eq - f exact to a
+        // rounding of f_eq.
+        const R c0 = fma_((R)-1.5, usq, (R)1);
+        const R S0 = rho * (R)(1.0 / 3.0) * c0;
+        R S[9], A[9], G[9], H[9];
+        Unroll<0, 9>::run([&](auto kc) __attribute__((always_inline)) {
+            constexpr int k = decltype(kc)::value;
+            constexpr int q = pair_dir(k);
+            const R cu = cdot<q>(ux, uy, uz);
+            const R cg = cdot<q>(gx, gy, gz);
+            const R wr = rho * (R)wq(q);
+            constexpr double wg = q < 7 ? 1.0 / 6.0 : 1.0 / 12.0;
+            S[k] = wr * fma_((R)4.5 * cu, cu, c0);
+            A[k] = ((R)3 * wr) * cu;
+            G[k] = (R)wg * fma_((R)3 * cu, cg, -udg);
+            H[k] = (R)wg * cg;
+        });
+        if (local_tau) {
+            // collision change of BGK: f_neq/tau + (1 - 1/(2 tau)) F.  Its second moment minus
+            // (uF + Fu) is (Pi(f_neq) - (uF + Fu)/2)/tau because the Guo term's second moment is
+            // uF + Fu; Pi(f_neq) takes only the pair sums f_neq(q) + f_neq(qb) = 2 S - f_q - f_qb.
+            const R told = taup[n];
+            const R itold = rcp_(told);
+            R pxx = 0, pyy = 0, pzz = 0, pxy = 0, pxz = 0, pyz = 0;
+            Unroll<0, 9>::run([&](auto kc) __attribute__((always_inline)) {
+                constexpr int k = decltype(kc)::value;
+                constexpr int q = pair_dir(k), qb = opposite(q);
+                const R sp = fma_((R)2, S[k], -(f[q] + f[qb]));
+                if constexpr (cx(q) != 0) pxx += sp;
+                if constexpr (cy(q) != 0) pyy += sp;
+                if constexpr (cz(q) != 0) pzz += sp;
+                if constexpr (cx(q) * cy(q) == 1) pxy += sp;
+                if constexpr (cx(q) * cy(q) == -1) pxy -= sp;
+                if constexpr (cx(q) * cz(q) == 1) pxz += sp;
+                if constexpr (cx(q) * cz(q) == -1) pxz -= sp;
+                if constexpr (cy(q) * cz(q) == 1) pyz += sp;
+                if constexpr (cy(q) * cz(q) == -1) pyz -= sp;
+            });
+            const R sxx = fma_(-ux, gx, pxx), syy = fma_(-uy, gy, pyy), szz = fma_(-uz, gz, pzz);
+            const R sxy = fma_((R)-0.5, fma_(ux, gy, uy * gx), pxy);
+            const R sxz = fma_((R)-0.5, fma_(ux, gz, uz * gx), pxz);
+            const R syz = fma_((R)-0.5, fma_(uy, gz, uz * gy), pyz);
+            const R n2 = fma_(sxx, sxx, fma_(syy, syy, fma_(szz, szz, (R)2 * fma_(sxy, sxy, fma_(sxz, sxz, syz * syz)))));
+            tau = tau_law<R>(a.fl, sqrt_(n2) * itold * (R)2.1213203435596424 * inv_rho);
+            if (!probe) taup[n] = tau;
+        }
+        const R itau = rcp_(tau);
+        const R pf = fma_((R)-0.5, itau, (R)1);
+        fo[0] = fma_(pf, -udg, fma_(S0 - f[0], itau, f[0]));
+        Unroll<0, 9>::run([&](auto kc) __attribute__((always_inline)) {
+            constexpr int k = decltype(kc)::value;
+            constexpr int q = pair_dir(k), qb = opposite(q);
+            fo[q] = fma_(pf, G[k] + H[k], fma_((S[k] + A[k]) - f[q], itau, f[q]));
+            fo[qb] = fma_(pf, G[k] - H[k], fma_((S[k] - A[k]) - f[qb], itau, f[qb]));
         });
     } else {
-        R v[Q], o[Q];
+        R ne[Q], fg[Q];
+        const R c0 = fma_((R)-1.5, usq, (R)1);
         Unroll<0, Q>::run([&](auto qc) __attribute__((always_inline)) {
             constexpr int q = decltype(qc)::value;
-            v[q] = ne[q] - (R)0.5 * fg[q];
+            const R cu = cdot<q>(ux, uy, uz);
+            const R cg = cdot<q>(gx, gy, gz);
+            R feq;
+            if constexpr (q == 0) {
+                feq = rho * (R)(1.0 / 3.0) * c0;
+                fg[q] = -udg;
+            } else {
+                feq = rho * (R)wq(q) * fma_((R)4.5 * cu, cu, fma_((R)3, cu, c0));
+                fg[q] = fma_((R)3 * cu, cg, cg - udg) * (R)(q < 7 ? 1.0 / 6.0 : 1.0 / 12.0);
+            }
+            ne[q] = feq - f[q];
         });
-        mrt_relax<R>(v, itau, o);
+        R v[Q];
+        Unroll<0, Q>::run([&](auto qc) __attribute__((always_inline)) {
+            constexpr int q = decltype(qc)::value;
+            v[q] = fma_((R)-0.5, fg[q], ne[q]);
+        });
+        if (local_tau) {
+            const R told = taup[n];
+            R ss[Q];
+            mrt_relax<R>(v, rcp_(told), ss);
+            Unroll<0, Q>::run([&](auto qc) __attribute__((always_inline)) { ss[decltype(qc)::value] += fg[decltype(qc)::value]; });
+            tau = tau_law<R>(a.fl, strain_rate<R>(ss, ux, uy, uz, gx, gy, gz, inv_rho));
+            if (!probe) taup[n] = tau;
+        }
+        R o[Q];
+        mrt_relax<R>(v, rcp_(tau), o);
         Unroll<0, Q>::run([&](auto qc) __attribute__((always_inline)) {
             constexpr int q = decltype(qc)::value;
             fo[q] = f[q] + o[q] + fg[q];
@@ -523,13 +560,14 @@ __global__ void __launch_bounds__(256) collide_kernel(CollideArgs a) {
     // the y/z halo rows.
     const int L = blockIdx.x;
     const int per_plane = a.nbx * a.nby;
-    const int z = L / per_plane;
-    const int rem = L - z * per_plane;
+    const int zl = L / per_plane;  // plane within this launch
+    const int z = a.z0 + zl;
+    const int rem = L - zl * per_plane;
     const int tyb = rem / a.nbx;
     const int txb = rem - tyb * a.nbx;
     const int x = txb * a.bx + (int)(threadIdx.x % a.bx);
     const int y = tyb * a.by + (int)(threadIdx.x / a.bx);
-    if (x >= a.g.nx || y >= a.g.ny || z >= a.g.nz) return;
+    if (x >= a.g.nx || y >= a.g.ny || z >= a.g.nz || z < 0) return;
     collide_node<ST, COLL, VT, WZ>(a, a.fin, a.fout, a.write_u_all, x, y, z, 0);
 }
 
@@ -546,13 +584,14 @@ __device__ __forceinline__ void step_body(const CollideArgs& a, const PartArgs& 
     }
     const int L = (int)blockIdx.x - nsph;
     const int per_plane = a.nbx * a.nby;
-    const int z = L / per_plane;
-    const int rem = L - z * per_plane;
+    const int zl = L / per_plane;  // plane within this launch (a z-slab from plane a.z0)
+    const int z = a.z0 + zl;
+    const int rem = L - zl * per_plane;
     const int tyb = rem / a.nbx;
     const int txb = rem - tyb * a.nbx;
     const int x = txb * a.bx + (int)(threadIdx.x % a.bx);
     const int y = tyb * a.by + (int)(threadIdx.x / a.bx);
-    if (x >= a.g.nx || y >= a.g.ny || z >= a.g.nz) return;
+    if (x >= a.g.nx || y >= a.g.ny || z >= a.g.nz || z < 0) return;
     collide_node<ST, COLL, VT, WZ>(a, a.fin, a.fout, a.write_u_all, x, y, z, 0);
 }
 
@@ -665,6 +704,7 @@ CollideArgs make_collide_args(Sim* s, const void* fin, void* fout, int write_u_a
     a.nbx = (s->g.nx + a.bx - 1) / a.bx;
     a.nby = (s->g.ny + a.by - 1) / a.by;
     a.nblk = a.nbx * a.nby * s->g.nz;
+    a.z0 = 0;
     return a;
 }
 
@@ -687,7 +727,7 @@ int launch_collide(Sim* s, const void* fin, void* fout, int write_u_all, int use
 // A kernel launch, with start/end events recorded at the kernel's own start and end when ev0 is set
 template <class K, class... Args>
 static void launch_maybe_timed(K kernel, dim3 grid, dim3 block, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1, Args... args) {
-    if (ev0) hipExtLaunchKernelGGL(kernel, grid, block, 0, st, ev0, ev1, 0, args...);
+    if (ev0 && ev1) hipExtLaunchKernelGGL(kernel, grid, block, 0, st, ev0, ev1, 0, args...);
     else hipLaunchKernelGGL(kernel, grid, block, 0, st, args...);
 }
 
@@ -718,21 +758,34 @@ static void fused_st(const CollideArgs& a, const PartArgs& p, int nsph, int do_p
 }
 
 int launch_step_fused(Sim* s, const void* fin, void* fout, int write_u_all, int do_post, hipStream_t st, hipEvent_t ev0,
-                      hipEvent_t ev1) {
-    const CollideArgs a = make_collide_args(s, fin, fout, write_u_all, 1);
+                      hipEvent_t ev1, int nslab, hipEvent_t* slab_ev) {
+    const CollideArgs all = make_collide_args(s, fin, fout, write_u_all, 1);
     PartArgs p = make_part_args(s);
     p.pos_out = gp(s->pos_alt);
     p.vel_out = gp(s->vel_alt);
-    const int nsph = s->np;
-    const int threads = a.bx * a.by;
+    const int threads = all.bx * all.by;
     const int coll = s->opt.collision, vt = s->opt.variable_viscosity;
-    switch (s->storage) {
-    case D3Q19_STORE_FP16_SHIFT: fused_st<D3Q19_STORE_FP16_SHIFT>(a, p, nsph, do_post, coll, vt, threads, st, ev0, ev1); break;
-    case D3Q19_STORE_BF16_SHIFT: fused_st<D3Q19_STORE_BF16_SHIFT>(a, p, nsph, do_post, coll, vt, threads, st, ev0, ev1); break;
-    case D3Q19_STORE_FP64: fused_st<D3Q19_STORE_FP64>(a, p, nsph, do_post, coll, vt, threads, st, ev0, ev1); break;
-    default: fused_st<D3Q19_STORE_FP32>(a, p, nsph, do_post, coll, vt, threads, st, ev0, ev1); break;
+    if (nslab < 1) nslab = 1;
+    // one launch: its own start and end events; slabs: stream events around the launches
+    if (nslab > 1 && ev0) HIP_OK(hipEventRecord(ev0, st));
+    for (int k = 0; k < nslab; k++) {
+        CollideArgs a = all;
+        if (nslab > 1) {
+            a.z0 = s->zb[k];
+            a.nblk = a.nbx * a.nby * (s->zb[k + 1] - s->zb[k]);
+        }
+        const int nsph = k == 0 ? s->np : 0;  // the spheres' turn goes with the first launch
+        hipEvent_t e0 = nslab == 1 ? ev0 : nullptr, e1 = nslab == 1 ? ev1 : nullptr;
+        switch (s->storage) {
+        case D3Q19_STORE_FP16_SHIFT: fused_st<D3Q19_STORE_FP16_SHIFT>(a, p, nsph, do_post, coll, vt, threads, st, e0, e1); break;
+        case D3Q19_STORE_BF16_SHIFT: fused_st<D3Q19_STORE_BF16_SHIFT>(a, p, nsph, do_post, coll, vt, threads, st, e0, e1); break;
+        case D3Q19_STORE_FP64: fused_st<D3Q19_STORE_FP64>(a, p, nsph, do_post, coll, vt, threads, st, e0, e1); break;
+        default: fused_st<D3Q19_STORE_FP32>(a, p, nsph, do_post, coll, vt, threads, st, e0, e1); break;
+        }
+        HIP_OK(hipGetLastError());
+        if (slab_ev) HIP_OK(hipEventRecord(slab_ev[k], st));
     }
-    HIP_OK(hipGetLastError());
+    if (nslab > 1 && ev1) HIP_OK(hipEventRecord(ev1, st));
     s->bcraw_valid = 0;
     return 0;
 }

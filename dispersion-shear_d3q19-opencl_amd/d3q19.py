@@ -71,7 +71,8 @@ class Options(C.Structure):
         ("contact_detection", C.c_int32),
         ("rebuild_freq", C.c_int32),
         ("rounding", C.c_int32),
-        ("reserved", C.c_int32 * 3),
+        ("overlap_slabs", C.c_int32),
+        ("reserved", C.c_int32 * 2),
     ]
 
 

@@ -116,7 +116,12 @@ typedef struct {
     int32_t rebuild_freq;       /* hostDat.RebuildFreq: zone lists rebuilt after steps t with
                                    t % rebuild_freq == 0 (sim_main.c:429-444); <= 0 selects 10 */
     int32_t rounding;           /* D3Q19_ROUND_*: how the collide rounds fp16 / bf16 shifted stores */
-    int32_t reserved[3];
+    int32_t overlap_slabs;      /* single simulations with spheres: the collide runs as this many
+                                   z-slabs and each slab's spheres go through the IBM pass on a second
+                                   stream while the later slabs collide; 0 or 1 = one launch each (the
+                                   faster on MI355X), 2..16 = that many slabs.  Results do not depend
+                                   on it (bitwise). */
+    int32_t reserved[2];
 } d3q19_options;
 
 /* Rounding of the collide's 16-bit stores (no counterpart in the reference, which stores fp32).

@@ -51,7 +51,8 @@ def run_triplet(c, steps, kernel_level=False):
 def envelope_check(name, g, o32, o64):
     eg, er = rel_linf(g, o64), rel_linf(o32, o64)
     bar = max(TOL, ENVELOPE * er)
-    print("%s: gpu-vs-f64 %.3e  f32oracle-vs-f64 %.3e  bar %.3e" % (name, eg, er, bar), flush=True)
+    print("%s: gpu-vs-f64 %.3e  f32oracle-vs-f64 %.3e  gpu-vs-f32oracle %.3e  bar %.3e" % (name, eg, er, rel_linf(g, o32), bar),
+          flush=True)
     assert eg <= bar, "%s outside the float32 envelope: %.3e > %.3e" % (name, eg, bar)
 
 

@@ -360,3 +360,26 @@ def test_sphere_beyond_brick_marking_capacity_rejected_before_device_use():
             assert rc != d3q19.E_UNSUPPORTED
             if rc == 0:
                 L.d3q19_destroy(h)
+
+
+def test_build_refuses_an_object_whose_kernels_lack_device_code(tmp_path):
+    """build.py checks every compiled object: each kernel that its host half registers by name must
+    be defined in its gfx950 code object, so a library that would abort at launch with "Cannot find
+    Symbol" (a host half paired with device code of another source) fails the build instead.  An
+    object whose .hip_fatbin is replaced by another object's device code is refused."""
+    import build as pkg_build
+    srcs = {"a": "k_alpha", "b": "k_beta"}
+    for name, kern in srcs.items():
+        with open(os.path.join(tmp_path, name + ".hip"), "w") as f:
+            f.write("#include <hip/hip_runtime.h>\n__global__ void %s(int* p) { *p = 1; }\n"
+                    "void launch_%s(int* p) { hipLaunchKernelGGL(%s, dim3(1), dim3(1), 0, 0, p); }\n" % (kern, name, kern))
+        subprocess.run([pkg_build.HIPCC, "--offload-arch=gfx950", "-c", "-fPIC", os.path.join(tmp_path, name + ".hip"), "-o",
+                        os.path.join(tmp_path, name + ".o")], check=True, capture_output=True)
+    pkg_build.check_kernel_registration(os.path.join(tmp_path, "a.o"))
+    fb, bad = os.path.join(tmp_path, "b.fatbin"), os.path.join(tmp_path, "bad.o")
+    subprocess.run([os.path.join(pkg_build.LLVM, "llvm-objcopy"), "--dump-section", ".hip_fatbin=" + fb, os.path.join(tmp_path, "b.o"),
+                    os.path.join(tmp_path, "discard.o")], check=True)
+    subprocess.run([os.path.join(pkg_build.LLVM, "llvm-objcopy"), "--update-section", ".hip_fatbin=" + fb, os.path.join(tmp_path, "a.o"),
+                    bad], check=True)
+    with pytest.raises(RuntimeError, match="k_alpha"):
+        pkg_build.check_kernel_registration(bad)
