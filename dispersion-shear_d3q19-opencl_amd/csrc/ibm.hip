@@ -17,15 +17,41 @@
 
 namespace d3q19 {
 
+// Peskin's 4-point kernel; one square root per call (the lanes of a wavefront take either branch)
 __device__ __forceinline__ double peskin4(double r) {
     r = fabs(r);
-    if (r < 1.0) return (3.0 - 2.0 * r + sqrt(1.0 + 4.0 * r - 4.0 * r * r)) * 0.125;
-    if (r < 2.0) return (5.0 - 2.0 * r - sqrt(-7.0 + 12.0 * r - 4.0 * r * r)) * 0.125;
-    return 0.0;
+    const bool inner = r < 1.0;
+    const double s = sqrt(inner ? 1.0 + 4.0 * r - 4.0 * r * r : -7.0 + 12.0 * r - 4.0 * r * r);
+    const double w = (inner ? 3.0 - 2.0 * r + s : 5.0 - 2.0 * r - s) * 0.125;
+    return r < 2.0 ? w : 0.0;
+}
+
+// fmod(x + L, L) (GPU_program.cl:141-144), exactly: for y = x + L in [0, 2L) fmod(y, L) is y or y - L,
+// and y - L is exact there (Sterbenz); other values take the library fmod
+__device__ __forceinline__ double wrap_coord(double x, double L) {
+    const double y = x + L;
+    if (y >= 0.0 && y < 2.0 * L) return y >= L ? y - L : y;
+    return fmod(y, L);
+}
+
+// n mod nd in [0, nd); one conditional add or subtract covers the offsets a stencil reaches
+__device__ __forceinline__ int wrap_index(int n, int nd) {
+    n += n < 0 ? nd : 0;
+    n -= n >= nd ? nd : 0;
+    if ((unsigned)n >= (unsigned)nd) n = ((n % nd) + nd) % nd;
+    return n;
+}
+
+// round to the nearest integer, ties to even (as __double2ll_rn): below 2^51 in magnitude, adding
+// 1.5 * 2^52 leaves the integer in the low mantissa bits
+__device__ __forceinline__ long long round_to_ll(double x) {
+#pragma clang fp contract(off)
+    if (!(fabs(x) < 2251799813685248.0)) return __double2ll_rn(x);
+    return __double_as_longlong(x + 6755399441055744.0) - 0x4338000000000000ll;
 }
 
 // contribution -w F[c] of one stencil node, rounded to fixed point as a whole
-__device__ __forceinline__ long long spread_value(double w, double Fc) { return __double2ll_rn((-w * Fc) * FIX_SCALE); }
+__device__ __forceinline__ long long spread_value(double w, double Fc) { return round_to_ll((-w * Fc) * FIX_SCALE); }
 
 // ... added with 64-bit integer atomics (order independent)
 __device__ __forceinline__ void spread_add(D3Q19_GLOBAL long long* facc, long long nint, int node, double w, const double* F) {
@@ -37,10 +63,29 @@ __device__ __forceinline__ void spread_add(D3Q19_GLOBAL long long* facc, long lo
     }
 }
 
+// v of the lane that the DPP control CTRL names, within a row of 16 lanes
+template <int CTRL> __device__ __forceinline__ double dpp_f64(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_mov_dpp((int)b, CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xF, 0xF, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+__device__ __forceinline__ double read_lane(double v, int l) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, l), hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// Sum over the 64 lanes in a fixed order, the same bits in every lane: pairs within each row of 16
+// (lane ^ 1, lane ^ 2, then the mirrored half-row and row: each pair adds the same two values, so
+// both lanes of a pair hold the same sum), then the four row sums as (r0 + r1) + (r2 + r3).
 __device__ __forceinline__ double wave_sum(double v) {
-    // fixed butterfly order over the 64 lanes: deterministic
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
+    v += dpp_f64<0xB1>(v);   // quad_perm [1, 0, 3, 2]
+    v += dpp_f64<0x4E>(v);   // quad_perm [2, 3, 0, 1]
+    v += dpp_f64<0x141>(v);  // row_half_mirror
+    v += dpp_f64<0x140>(v);  // row_mirror
+    return (read_lane(v, 0) + read_lane(v, 16)) + (read_lane(v, 32) + read_lane(v, 48));
 }
 
 // One 64-lane wavefront per marker; lane k = (kx, ky, kz) in [0,4)^3 owns one node of the
@@ -59,7 +104,7 @@ __device__ __forceinline__ void ibm_marker(const IbmArgs& a, const D3Q19_GLOBAL 
     bool finite = true;
     for (int d = 0; d < 3; d++) {
         const double L = (double)g.sys[d];
-        rpp[d] = fmod((pos[3 * p + d] + r0[d]) + L, L);  // GPU_program.cl:141-144
+        rpp[d] = wrap_coord(pos[3 * p + d] + r0[d], L);  // GPU_program.cl:141-144
         if (!(rpp[d] >= 0.0 && rpp[d] < L)) finite = false;  // NaN or out of range
     }
     if (!finite) {
@@ -109,7 +154,7 @@ __device__ __forceinline__ void ibm_marker(const IbmArgs& a, const D3Q19_GLOBAL 
             const int base = (int)floor(rpp[d]);
             int node = base - 1 + kk[d];
             const double w = peskin4(rpp[d] - (double)node);
-            if (g.per[d]) node = ((node % nd[d]) + nd[d]) % nd[d];
+            if (g.per[d]) node = wrap_index(node, nd[d]);
             else if (node < 0 || node >= nd[d]) sok = false;
             sn[d] = node;
             sw[d] = w;
@@ -240,14 +285,19 @@ __device__ __forceinline__ void ibm_tile_group(const IbmArgs& a, const D3Q19_GLO
             }
         });
     __syncthreads();
+    // box position of node i by multiply and shift: for i < 1024 and extents up to 1024,
+    // (i * ceil(2^20 / e)) >> 20 is i / e exactly (the excess stays below i / 2^20 < 1 / e)
+    static_assert(TILE_NODES <= 1024, "the box index split below is exact for boxes up to 1024 nodes");
+    const int mx = vol ? ((1 << 20) + ex - 1) / ex : 0, my = vol ? ((1 << 20) + ey - 1) / ey : 0;
     for (int i = threadIdx.x; i < vol; i += blockDim.x) {
         const long long v0 = tile[0][i], v1 = tile[1][i], v2 = tile[2][i];
         if ((v0 | v1 | v2) == 0) continue;
-        const int t3[3] = {i % ex, (i / ex) % ey, i / (ex * ey)};
+        const int ixy = (i * mx) >> 20, iz = (ixy * my) >> 20;
+        const int t3[3] = {i - ixy * ex, ixy - iz * ey, iz};
         int n3[3];
         for (int d = 0; d < 3; d++) {
             int n = ctr[d] + lo[d] + t3[d];
-            if (g.per[d]) n = ((n % nd[d]) + nd[d]) % nd[d];
+            if (g.per[d]) n = wrap_index(n, nd[d]);
             n3[d] = n;
         }
         const long long node = n3[0] + (long long)g.nx * (n3[1] + (long long)g.ny * n3[2]);
