@@ -380,8 +380,27 @@ __device__ __forceinline__ void collide_node(const CollideArgs& a, const void* f
                                                    : to_global<uint32_t>(fin);
     const uint32_t fmark = *fmp, umark = *ump;
 
+    // The previous relaxation time (float) is copied into LDS by the memory unit while the
+    // populations are in flight (global_load_lds: no register is held for it) and read from there
+    // where the strain rate needs it.  Loaded there from memory, it cost each wavefront one more
+    // memory round trip after the populations had arrived.  The wait for it is the one the first
+    // use of the populations needs anyway.  Blocks have at most 256 threads; a lane's slot is
+    // written and read by that lane's own wavefront only.
+    const bool local_tau = VT == 1 || (VT == 2 && a.vt);
+    const auto taup = to_global<R>(a.tau);
+    constexpr bool stage_tau = sizeof(R) == 4;
+    __shared__ float tau_stage[256];
+    if constexpr (stage_tau) {
+        if (local_tau)
+            __builtin_amdgcn_global_load_lds((const void*)(taup + n), (__attribute__((address_space(3))) void*)&tau_stage[threadIdx.x & ~63u], 4,
+                                             0, 0);
+    }
+
     R f[Q];
     gather_pre<ST, WZ>(to_global<T>(fin), g, a.fl, a.bcraw, a.use_bcraw, x, y, z, f);
+    if constexpr (stage_tau) {
+        if (local_tau) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    }
 
     R gx = (R)a.fl.gconst[0], gy = (R)a.fl.gconst[1], gz = (R)a.fl.gconst[2];
     if (force_on) {
@@ -424,8 +443,6 @@ __device__ __forceinline__ void collide_node(const CollideArgs& a, const void* f
     const R usq = ux * ux + uy * uy + uz * uz;
     const R udg = ux * gx + uy * gy + uz * gz;
     R tau = (R)a.fl.tau0;
-    const auto taup = to_global<R>(a.tau);
-    const bool local_tau = VT == 1 || (VT == 2 && a.vt);
     R fo[Q];
     if constexpr (COLL == D3Q19_COLL_BGK) {
         // BGK in pair form.  A direction q and its opposite qb have c_qb = -c_q, so
@@ -454,7 +471,7 @@ __device__ __forceinline__ void collide_node(const CollideArgs& a, const void* f
             // collision change of BGK: f_neq/tau + (1 - 1/(2 tau)) F.  Its second moment minus
             // (uF + Fu) is (Pi(f_neq) - (uF + Fu)/2)/tau because the Guo term's second moment is
             // uF + Fu; Pi(f_neq) takes only the pair sums f_neq(q) + f_neq(qb) = 2 S - f_q - f_qb.
-            const R told = taup[n];
+            const R told = stage_tau ? (R)tau_stage[threadIdx.x] : taup[n];
             const R itold = rcp_(told);
             R pxx = 0, pyy = 0, pzz = 0, pxy = 0, pxz = 0, pyz = 0;
             Unroll<0, 9>::run([&](auto kc) __attribute__((always_inline)) {
@@ -511,7 +528,7 @@ __device__ __forceinline__ void collide_node(const CollideArgs& a, const void* f
             v[q] = fma_((R)-0.5, fg[q], ne[q]);
         });
         if (local_tau) {
-            const R told = taup[n];
+            const R told = stage_tau ? (R)tau_stage[threadIdx.x] : taup[n];
             R ss[Q];
             mrt_relax<R>(v, rcp_(told), ss);
             Unroll<0, Q>::run([&](auto qc) __attribute__((always_inline)) { ss[decltype(qc)::value] += fg[decltype(qc)::value]; });
