@@ -70,6 +70,8 @@ static int validate(const int_param_struct* ip, const flp_param_struct* fp, cons
             return fail(D3Q19_E_INVALID, "unknown rounding mode");
         if (opt->overlap_slabs < 0 || opt->overlap_slabs > Sim::MAX_SLABS)
             return fail(D3Q19_E_INVALID, "overlap_slabs must be 0 (automatic) to 16");
+        if (opt->ibm_layout < D3Q19_IBM_LAYOUT_AUTO || opt->ibm_layout > D3Q19_IBM_LAYOUT_ROWS_G16)
+            return fail(D3Q19_E_INVALID, "ibm_layout must be 0 (automatic) to 3");
         if (opt->contact_detection != D3Q19_CONTACT_ZONES && opt->contact_detection != D3Q19_CONTACT_ALL_PAIRS)
             return fail(D3Q19_E_INVALID, "unknown contact detection");
         if (opt->contact_detection == D3Q19_CONTACT_ZONES && ip->NumParticles > 0) {
@@ -506,6 +508,8 @@ int sim_init(Sim* s, int device, const int_param_struct* ip, const flp_param_str
     const size_t nint = (size_t)s->g.nint;
     const size_t rs = real_size(s->storage);
     s->nbricks = s->g.bnx * s->g.bny * s->g.bnz;
+    // non-temporal population stores beyond the Infinity Cache (a batch decides for its cells)
+    s->nt_stores = !shared_stream && (long long)(2 * Q * pop_size(s->storage) * nint) > INFINITY_CACHE_BYTES;
     hipError_t e = hipSuccess;
     if (shared_stream) {
         s->stream = shared_stream;
@@ -539,13 +543,17 @@ int sim_init(Sim* s, int device, const int_param_struct* ip, const flp_param_str
         free_all(s);
         return rc;
     }
-    // marker groups of the LDS-staged spreading (TILE_MARKERS consecutive entries of perm), made
-    // compact so that their stencils overlap and their LDS boxes stay small: seeds are taken from one
-    // pole to the other (by z, then azimuth), and each takes its TILE_MARKERS - 1 nearest unassigned
-    // markers by the largest per-axis distance (ties by index, so the grouping is deterministic).
-    // Against chunks of a Morton order this shrinks the largest box on the 192-marker, d = 12 set
-    // from 1890 to 832 nodes.  Staging is enabled where every periodic extent exceeds twice the
-    // stencil reach (|offset| + 3 nodes) plus a margin.
+    // marker groups of the LDS-staged spreading (gm consecutive entries of perm), made compact so
+    // that their stencils overlap and their LDS boxes stay small: seeds are taken from one pole to
+    // the other (by z, then azimuth), and each takes its gm - 1 nearest unassigned markers by the
+    // largest per-axis distance (ties by index, so the grouping is deterministic).  Against chunks
+    // of a Morton order this shrinks the largest box on the 192-marker, d = 12 set from 1890 to 832
+    // nodes.  A single simulation with the Peskin-4 kernel takes groups of TILE_MARKERS_MAX markers
+    // when all their boxes fit in TILE_NODES (fewer, fuller blocks and a larger overlap: the
+    // 256x256x128 workload's IBM pass 74 -> 60 us), otherwise TILE_MARKERS, the size cells of a
+    // batch always use.  The grouping decides only which block adds a contribution: integer sums make
+    // the spread force independent of it.  Staging is enabled where every periodic extent exceeds
+    // twice the stencil reach (|offset| + 3 nodes) plus a margin.
     std::vector<int> perm, gbox;
     if (s->np > 0) {
         double rmax = 0.0, rmin = 1e300;
@@ -557,19 +565,21 @@ int sim_init(Sim* s, int device, const int_param_struct* ip, const flp_param_str
         }
         s->mark_rin = std::max(0.0, rmin * (1.0 - 1e-6) - 1e-6);
         s->mark_rout = rmax * (1.0 + 1e-6) + 1e-6;
-        {
+        auto pt = [&](int m, int d) { return (double)(d == 0 ? sphere_points[m].x : d == 1 ? sphere_points[m].y : sphere_points[m].z); };
+        // groups of gm markers: perm and 8 ints per group (box lo[3], extent[3], volume, 0); returns
+        // the largest box volume
+        auto build_groups = [&](int gm, std::vector<int>& pm, std::vector<int>& gb) {
             const int n = s->ppp;
             std::vector<int> order((size_t)n);
             for (int m = 0; m < n; m++) order[(size_t)m] = m;
-            auto pt = [&](int m, int d) { return (double)(d == 0 ? sphere_points[m].x : d == 1 ? sphere_points[m].y : sphere_points[m].z); };
             std::stable_sort(order.begin(), order.end(), [&](int i, int j) {
                 if (pt(i, 2) != pt(j, 2)) return pt(i, 2) > pt(j, 2);
                 return std::atan2(pt(i, 1), pt(i, 0)) < std::atan2(pt(j, 1), pt(j, 0));
             });
             std::vector<char> used((size_t)n, 0);
             std::vector<std::pair<double, int>> cand;
-            perm.clear();
-            perm.reserve((size_t)n);
+            pm.clear();
+            pm.reserve((size_t)n);
             for (int sd : order) {
                 if (used[(size_t)sd]) continue;
                 cand.clear();
@@ -579,39 +589,66 @@ int sim_init(Sim* s, int device, const int_param_struct* ip, const flp_param_str
                         for (int d = 0; d < 3; d++) dist = std::max(dist, std::fabs(pt(j, d) - pt(sd, d)));
                         cand.push_back({dist, j});
                     }
-                const size_t take = std::min(cand.size(), (size_t)TILE_MARKERS);
+                const size_t take = std::min(cand.size(), (size_t)gm);
                 std::partial_sort(cand.begin(), cand.begin() + take, cand.end());
                 for (size_t k = 0; k < take; k++) {
-                    perm.push_back(cand[k].second);
+                    pm.push_back(cand[k].second);
                     used[(size_t)cand[k].second] = 1;
                 }
             }
-        }
-        // LDS box of each group, relative to the sphere's centre node: a marker at offset r has its
-        // stencil nodes at floor(c + r) - floor(c) - 1 ... + 2, and floor(c + r) - floor(c) is
-        // floor(r) or floor(r) + 1 whatever the centre c; a lane outside the box (rounding at a
-        // periodic wrap) spreads directly
-        const int ngrp = (s->ppp + TILE_MARKERS - 1) / TILE_MARKERS;
-        gbox.assign(8 * (size_t)ngrp, 0);
-        for (int k = 0; k < ngrp; k++) {
-            int lo[3] = {1 << 30, 1 << 30, 1 << 30}, hi[3] = {-(1 << 30), -(1 << 30), -(1 << 30)};
-            for (int j = k * TILE_MARKERS; j < std::min(s->ppp, (k + 1) * TILE_MARKERS); j++) {
-                const d3q19_float4 q = sphere_points[perm[(size_t)j]];
-                const float c[3] = {q.x, q.y, q.z};
-                for (int d = 0; d < 3; d++) {
-                    const int f = (int)std::floor(c[d]);
-                    lo[d] = std::min(lo[d], f - 1);
-                    hi[d] = std::max(hi[d], f + 3);
+            // LDS box of each group, relative to the sphere's centre node: a marker at offset r has
+            // its stencil nodes at floor(c + r) - floor(c) - 1 ... + 2, and floor(c + r) - floor(c)
+            // is floor(r) or floor(r) + 1 whatever the centre c; a lane outside the box (rounding at
+            // a periodic wrap) spreads directly
+            const int ngrp = (n + gm - 1) / gm;
+            gb.assign(8 * (size_t)ngrp, 0);
+            long long vmax = 0;
+            for (int k = 0; k < ngrp; k++) {
+                int lo[3] = {1 << 30, 1 << 30, 1 << 30}, hi[3] = {-(1 << 30), -(1 << 30), -(1 << 30)};
+                for (int j = k * gm; j < std::min(n, (k + 1) * gm); j++) {
+                    const d3q19_float4 q = sphere_points[pm[(size_t)j]];
+                    const float c[3] = {q.x, q.y, q.z};
+                    for (int d = 0; d < 3; d++) {
+                        const int f = (int)std::floor(c[d]);
+                        lo[d] = std::min(lo[d], f - 1);
+                        hi[d] = std::max(hi[d], f + 3);
+                    }
                 }
+                long long vol = 1;
+                for (int d = 0; d < 3; d++) {
+                    gb[8 * (size_t)k + d] = lo[d];
+                    gb[8 * (size_t)k + 3 + d] = hi[d] - lo[d] + 1;
+                    vol *= hi[d] - lo[d] + 1;
+                }
+                gb[8 * (size_t)k + 6] = (int)std::min(vol, (long long)(1 << 30));
+                vmax = std::max(vmax, vol);
             }
-            long long vol = 1;
-            for (int d = 0; d < 3; d++) {
-                gbox[8 * (size_t)k + d] = lo[d];
-                gbox[8 * (size_t)k + 3 + d] = hi[d] - lo[d] + 1;
-                vol *= hi[d] - lo[d] + 1;
-            }
-            gbox[8 * (size_t)k + 6] = (int)std::min(vol, (long long)(1 << 30));
+            return vmax;
+        };
+        // Peskin-4 markers four to a wavefront when there are many (a quarter of the wavefronts,
+        // each with a third of the instructions per marker: the 256x256x128 workload's 64,000
+        // markers 77 -> 57 us with groups of 16); with few markers one per wavefront keeps more
+        // wavefronts in flight (the 128^3 workload's 6,144 markers: 12.4 against 14.2 us).  Both
+        // sum a marker's stencil in the same order, so the choice changes no result bit.
+        const int layout = s->opt.ibm_layout;
+        const bool peskin = s->opt.ibm_kernel == D3Q19_IBM_PESKIN4;
+        const long long blocks16 = (long long)s->np * ((s->ppp + TILE_MARKERS_MAX - 1) / TILE_MARKERS_MAX);
+        bool try16;
+        if (layout == D3Q19_IBM_LAYOUT_AUTO) {
+            s->ibm_rows = peskin && (long long)s->np * s->ppp >= IBM_ROWS_MIN_MARKERS;
+            try16 = s->ibm_rows && blocks16 >= IBM_GROUP16_MIN_BLOCKS;
+        } else {
+            s->ibm_rows = peskin && layout != D3Q19_IBM_LAYOUT_WAVE;
+            try16 = s->ibm_rows && layout == D3Q19_IBM_LAYOUT_ROWS_G16;
         }
+        s->group_markers = TILE_MARKERS;
+        if (!shared_stream && try16 && build_groups(TILE_MARKERS_MAX, perm, gbox) <= TILE_NODES)
+            s->group_markers = TILE_MARKERS_MAX;
+        else
+            (void)build_groups(TILE_MARKERS, perm, gbox);
+        s->tile_nodes = 0;
+        for (size_t k = 0; k < gbox.size(); k += 8)
+            if (gbox[k + 6] <= TILE_NODES) s->tile_nodes = std::max(s->tile_nodes, gbox[k + 6]);
         const int need = 2 * ((int)std::ceil(rmax) + 3) + 4;
         s->tile_ok = true;
         const int nd[3] = {s->g.nx, s->g.ny, s->g.nz};

@@ -5,13 +5,14 @@
 // Spreading is order independent: every contribution is rounded to a 2^-40 fixed-point integer
 // and summed with 64-bit integer additions, whose result does not depend on the order of arrival,
 // so the force field is bitwise reproducible (the reference's slot counter + float
-// read-modify-write, GPU_program.cl:217-224, is not).  The L2 atomics bound this pass (about 1.2 M
-// of them per step for 32 spheres of 192 markers); summing a block's 8 consecutive markers in an
-// LDS table first halves their number but measured slower on MI355X, so every lane adds its own
-// contribution.  The reference's 27-point smoothing of the spread force at
-// the next collide (GPU_program.cl:356-377, buffer shell contributing zero and no periodic wrap)
-// is folded into the spread: each marker spreads directly with the convolution of its trilinear
-// weights and the (1/4,1/2,1/4)^3 kernel, restricted to interior target nodes.
+// read-modify-write, GPU_program.cl:217-224, is not).  A block takes a group of neighbouring
+// markers of one sphere and sums their contributions in an LDS box first, so that each node of the
+// group's stencils takes one global atomic per component.  Peskin-4 markers run four to a
+// wavefront (a 16-lane row each); the reference's trilinear kernel one per wavefront.  The
+// reference's 27-point smoothing of the spread force at the next collide (GPU_program.cl:356-377,
+// buffer shell contributing zero and no periodic wrap) is folded into the spread: each marker
+// spreads directly with the convolution of its trilinear weights and the (1/4,1/2,1/4)^3 kernel,
+// restricted to interior target nodes.
 #include <hip/hip_ext.h>
 #include "internal.h"
 
@@ -77,14 +78,19 @@ __device__ __forceinline__ double read_lane(double v, int l) {
     return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 
-// Sum over the 64 lanes in a fixed order, the same bits in every lane: pairs within each row of 16
+// Sums in a fixed order, the same bits in every lane.  row_sum: over each row of 16 lanes, by pairs
 // (lane ^ 1, lane ^ 2, then the mirrored half-row and row: each pair adds the same two values, so
-// both lanes of a pair hold the same sum), then the four row sums as (r0 + r1) + (r2 + r3).
-__device__ __forceinline__ double wave_sum(double v) {
+// both lanes of a pair hold the same sum).  wave_sum: the four row sums as (r0 + r1) + (r2 + r3).
+__device__ __forceinline__ double row_sum(double v) {
     v += dpp_f64<0xB1>(v);   // quad_perm [1, 0, 3, 2]
     v += dpp_f64<0x4E>(v);   // quad_perm [2, 3, 0, 1]
     v += dpp_f64<0x141>(v);  // row_half_mirror
     v += dpp_f64<0x140>(v);  // row_mirror
+    return v;
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+    v = row_sum(v);
     return (read_lane(v, 0) + read_lane(v, 16)) + (read_lane(v, 32) + read_lane(v, 48));
 }
 
@@ -189,6 +195,101 @@ __device__ __forceinline__ void ibm_marker(const IbmArgs& a, const D3Q19_GLOBAL 
     }
 }
 
+// Peskin-4 with four markers per wavefront.  Row r of the wavefront (lanes 16r .. 16r + 15, the
+// groups that DPP operations work within) runs marker m of sphere p; lane j of the row owns the
+// stencil row (ky, kz) = (j & 3, j >> 2), the four nodes kx = 0..3 along x.  The marker's twelve
+// one-dimensional weights are evaluated once, by lanes 0..11 of the row (axis j >> 2, offset
+// j & 3), and read by the others through lane shuffles; the interpolation is a sum over the lane's
+// four nodes, then over the row.  Against one marker per wavefront this takes 40% of the vector
+// instructions per marker.  Rows without a marker (has false) run along with zero weights
+// and contribute nothing; every lane of the wavefront reaches the shuffles and row sums.
+template <class R, class Sink>
+__device__ __forceinline__ void ibm_rows(const IbmArgs& a, const D3Q19_GLOBAL double* pos, const D3Q19_GLOBAL double* vel, int p, int m,
+                                         bool has, int lane, Sink&& sink) {
+    const Geom& g = a.g;
+    const int nd[3] = {g.nx, g.ny, g.nz};
+    const int j = lane & 15, rowbase = lane & 48;
+    const int ky = j & 3, kz = j >> 2;
+    const int gid = p * a.ppp + m;
+    double r0[3] = {0.0, 0.0, 0.0}, rpp[3] = {0.5, 0.5, 0.5};
+    bool live = has;
+    if (has) {
+        for (int d = 0; d < 3; d++) {
+            r0[d] = (double)a.pts[4 * m + d];
+            const double L = (double)g.sys[d];
+            rpp[d] = wrap_coord(pos[3 * p + d] + r0[d], L);  // GPU_program.cl:141-144
+            if (!(rpp[d] >= 0.0 && rpp[d] < L)) live = false;  // NaN or out of range
+        }
+        // a sphere whose state is no longer finite contributes nothing (the watchdog reports it)
+        if (!live && j < 6) a.mforce[6 * (long long)gid + j] = 0.0 / 0.0;
+    }
+    int base[3];
+    for (int d = 0; d < 3; d++) base[d] = live ? (int)floor(rpp[d]) : 0;
+    // this lane's one-dimensional weight (lanes 12..15 of a row evaluate an unused one)
+    const int ax = (j >> 2) < 3 ? (j >> 2) : 0;
+    const double rsel = ax == 0 ? rpp[0] : (ax == 1 ? rpp[1] : rpp[2]);
+    const int bsel = ax == 0 ? base[0] : (ax == 1 ? base[1] : base[2]);
+    const double wj = peskin4(rsel - (double)(bsel - 1 + (j & 3)));
+    double wx[4];
+    for (int k = 0; k < 4; k++) wx[k] = __shfl(wj, rowbase + k, 64);
+    const double wy = __shfl(wj, rowbase + 4 + ky, 64), wz = __shfl(wj, rowbase + 8 + kz, 64);
+    // stencil nodes: (base - 1 + k) per axis, wrapped on periodic axes, dropped beyond a wall
+    int yn = base[1] - 1 + ky, zn = base[2] - 1 + kz;
+    bool okyz = live;
+    if (g.per[1]) yn = wrap_index(yn, nd[1]);
+    else if (yn < 0 || yn >= nd[1]) okyz = false;
+    if (g.per[2]) zn = wrap_index(zn, nd[2]);
+    else if (zn < 0 || zn >= nd[2]) okyz = false;
+    int xn[4];
+    bool ok[4];
+    double w[4];
+    for (int k = 0; k < 4; k++) {
+        int x = base[0] - 1 + k;
+        bool okx = okyz;
+        if (g.per[0]) x = wrap_index(x, nd[0]);
+        else if (x < 0 || x >= nd[0]) okx = false;
+        xn[k] = x;
+        ok[k] = okx;
+        w[k] = (wx[k] * wy) * wz;
+    }
+    // the lane's four nodes summed as (p0 + p1) + (p2 + p3), then the row over ky pairs and kz pairs:
+    // the association of one marker per wavefront (lane = kx + 4 ky + 16 kz, wave_sum), so both
+    // layouts give the same bits
+    const D3Q19_GLOBAL R* u = (const D3Q19_GLOBAL R*)a.u;
+    double pc[3][4];
+    const long long rowoff = (long long)g.nx * (yn + (long long)g.ny * zn);
+    for (int k = 0; k < 4; k++) {
+        pc[0][k] = pc[1][k] = pc[2][k] = 0.0;
+        if (ok[k]) {
+            const D3Q19_GLOBAL R* un = u + 3 * (rowoff + xn[k]);  // components interleaved per node
+            pc[0][k] = w[k] * (double)un[0];
+            pc[1][k] = w[k] * (double)un[1];
+            pc[2][k] = w[k] * (double)un[2];
+        }
+    }
+    double up[3];
+    for (int c = 0; c < 3; c++) up[c] = row_sum((pc[c][0] + pc[c][1]) + (pc[c][2] + pc[c][3]));
+    if (!live) return;
+    const D3Q19_GLOBAL double* om = a.omega + 3 * p;
+    const double wr[3] = {om[1] * r0[2] - om[2] * r0[1], om[2] * r0[0] - om[0] * r0[2], om[0] * r0[1] - om[1] * r0[0]};
+    double F[3];
+    for (int k = 0; k < 3; k++) F[k] = (up[k] - (vel[3 * p + k] + wr[k])) * a.dA;
+    for (int k = 0; k < 4; k++)
+        if (ok[k]) {
+            const int sn[3] = {xn[k], yn, zn};
+            sink(sn, w[k], F);
+        }
+    if (j == 0) {
+        D3Q19_GLOBAL double* mf = a.mforce + 6 * (long long)gid;
+        mf[0] = F[0];
+        mf[1] = F[1];
+        mf[2] = F[2];
+        mf[3] = r0[1] * F[2] - r0[2] * F[1];
+        mf[4] = r0[2] * F[0] - r0[0] * F[2];
+        mf[5] = r0[0] * F[1] - r0[1] * F[0];
+    }
+}
+
 // Sphere p belongs to this pass: its stencils reach at most up to a z plane in [zlo, zhi).  The top
 // plane of a marker's stencil is floor(z_marker) + 2 <= floor(z_sphere + zreach) + 2 (planes beyond a
 // wall are dropped); a stencil that wraps around a periodic z axis and a non-finite position belong
@@ -205,9 +306,33 @@ __device__ __forceinline__ bool pass_owns(const IbmArgs& a, const D3Q19_GLOBAL d
     return top >= a.zlo && top < a.zhi;
 }
 
-template <class R, int MODE>
+// ROWS: four Peskin-4 markers per wavefront (ibm_rows); otherwise one marker per wavefront
+// (ibm_marker, also the reference's trilinear kernel)
+template <int MODE, bool ROWS> constexpr bool rows_layout() { return MODE == D3Q19_IBM_PESKIN4 && ROWS; }
+
+template <class R, int MODE, bool ROWS>
 __global__ void __launch_bounds__(64 * IBM_WAVES) ibm_marker_kernel(IbmArgs a) {
     const int lane = threadIdx.x & 63;
+    if constexpr (rows_layout<MODE, ROWS>()) {
+        const int wave = blockIdx.x * IBM_WAVES + (int)(threadIdx.x >> 6);
+        const int total = a.np * a.ppp;
+        if (4 * wave >= total) return;  // uniform per wavefront
+        if (wave == 0 && lane == 0 && a.commit) {
+            const uint32_t e = a.epochs[0] + 1u;  // as below
+            a.epochs[0] = e;
+            a.epochs[1] = e;
+        }
+        const int gid = 4 * wave + (lane >> 4);  // uniform per row
+        bool has = gid < total;
+        const int p = has ? gid % a.np : 0, k = gid / a.np;
+        has = has && pass_owns(a, a.pos, p);
+        const int m = (int)(((long long)k * a.mstride) % a.ppp);
+        const long long nint = a.g.nint;
+        ibm_rows<R>(a, a.pos, a.vel, p, m, has, lane, [&](const int* sn, double w, const double* F) {
+            spread_add(a.facc, nint, sn[0] + a.g.nx * (sn[1] + a.g.ny * sn[2]), w, F);
+        });
+        return;
+    }
     const int gid = blockIdx.x * IBM_WAVES + (int)(threadIdx.x >> 6);
     if (gid >= a.np * a.ppp) return;  // uniform per wavefront
     if (gid == 0 && lane == 0 && a.commit) {
@@ -229,15 +354,16 @@ __global__ void __launch_bounds__(64 * IBM_WAVES) ibm_marker_kernel(IbmArgs a) {
 }
 
 // ---- spreading staged in LDS -----------------------------------------------------------------------
-// A block of TILE_MARKERS wavefronts takes TILE_MARKERS markers of one sphere that lie next to each
-// other on its surface (a Morton-ordered permutation of the marker set): each wavefront runs its
-// marker as above, but adds its fixed-point contributions into an LDS box covering the group's
-// stencils (64-bit integer LDS atomics); the block then adds the box's nonzero nodes to the spread
-// force with one global atomic per node and component.  Stencils of neighbouring markers overlap,
-// so the global atomics, which bound the spreading, drop by the overlap factor; integer sums keep
-// the result bitwise that of per-lane atomics.  Nodes are placed in the box by their offset from
-// the sphere's centre node, taken into (-n/2, n/2] on periodic axes (used only where every periodic
-// extent exceeds twice the sphere's stencil reach).  A box above TILE_NODES spreads directly.
+// A block takes a group of markers of one sphere that lie next to each other on its surface
+// (groups of a.gm markers fixed on the host): each row (Peskin-4) or wavefront (trilinear) runs its marker as
+// above, but adds its fixed-point contributions into an LDS box covering the group's stencils
+// (64-bit integer LDS atomics); the block then adds the box's nonzero nodes to the spread force with
+// one global atomic per node and component.  Stencils of neighbouring markers overlap, so the
+// global atomics drop by the overlap factor; integer sums keep the result bitwise that of per-lane
+// atomics.  Nodes are placed in the box by their offset from the sphere's centre node, taken into
+// (-n/2, n/2] on periodic axes (used only where every periodic extent exceeds twice the sphere's
+// stencil reach).  The box lives in dynamic LDS of `cap` nodes per component (the launch's
+// largest group box); a group whose box exceeds the simulation's capacity spreads directly.
 __device__ __forceinline__ int rel_offset(int n, int c, int nd, int per) {
     int o = n - c;
     if (per) {
@@ -247,9 +373,9 @@ __device__ __forceinline__ int rel_offset(int n, int c, int nd, int per) {
     return o;
 }
 
-template <class R, int MODE>
+template <class R, int MODE, bool ROWS>
 __device__ __forceinline__ void ibm_tile_group(const IbmArgs& a, const D3Q19_GLOBAL double* pos, const D3Q19_GLOBAL double* vel, int p,
-                                               int grp, int m, bool has, int lane, long long (*tile)[TILE_NODES]) {
+                                               int grp, int m, bool has, int lane, long long* tile, int cap) {
     const Geom& g = a.g;
     const int nd[3] = {g.nx, g.ny, g.nz};
     // the group's box, relative to the sphere's centre node, was computed from the offsets on the
@@ -258,39 +384,44 @@ __device__ __forceinline__ void ibm_tile_group(const IbmArgs& a, const D3Q19_GLO
     const int lo[3] = {gb[0], gb[1], gb[2]};
     const int ex = gb[3], ey = gb[4], ez = gb[5];
     const int ctr[3] = {(int)floor(pos[3 * p]), (int)floor(pos[3 * p + 1]), (int)floor(pos[3 * p + 2])};
-    const bool use_tile = a.tile_ok && gb[6] <= TILE_NODES;  // uniform over the block
+    const bool use_tile = a.tile_ok && gb[6] <= a.tile_nodes && gb[6] <= cap;  // uniform over the block
     const int vol = use_tile ? ex * ey * ez : 0;
+    long long* const t0 = tile;
+    long long* const t1 = tile + cap;
+    long long* const t2 = tile + 2 * cap;
     for (int i = threadIdx.x; i < vol; i += blockDim.x) {
-        tile[0][i] = 0;
-        tile[1][i] = 0;
-        tile[2][i] = 0;
+        t0[i] = 0;
+        t1[i] = 0;
+        t2[i] = 0;
     }
     __syncthreads();
-    if (has)
-        ibm_marker<R, MODE>(a, pos, vel, p, m, lane, [&](const int* sn, double w, const double* F) {
-            int t = -1;
-            if (use_tile) {
-                const int tx = rel_offset(sn[0], ctr[0], nd[0], g.per[0]) - lo[0];
-                const int ty = rel_offset(sn[1], ctr[1], nd[1], g.per[1]) - lo[1];
-                const int tz = rel_offset(sn[2], ctr[2], nd[2], g.per[2]) - lo[2];
-                if ((unsigned)tx < (unsigned)ex && (unsigned)ty < (unsigned)ey && (unsigned)tz < (unsigned)ez) t = tx + ex * (ty + ey * tz);
+    auto sink = [&](const int* sn, double w, const double* F) {
+        int t = -1;
+        if (use_tile) {
+            const int tx = rel_offset(sn[0], ctr[0], nd[0], g.per[0]) - lo[0];
+            const int ty = rel_offset(sn[1], ctr[1], nd[1], g.per[1]) - lo[1];
+            const int tz = rel_offset(sn[2], ctr[2], nd[2], g.per[2]) - lo[2];
+            if ((unsigned)tx < (unsigned)ex && (unsigned)ty < (unsigned)ey && (unsigned)tz < (unsigned)ez) t = tx + ex * (ty + ey * tz);
+        }
+        if (t >= 0) {
+            long long* const tc[3] = {t0, t1, t2};
+            for (int c = 0; c < 3; c++) {
+                const long long iv = spread_value(w, F[c]);
+                if (iv != 0) atomicAdd((unsigned long long*)&tc[c][t], (unsigned long long)iv);
             }
-            if (t >= 0) {
-                for (int c = 0; c < 3; c++) {
-                    const long long iv = spread_value(w, F[c]);
-                    if (iv != 0) atomicAdd((unsigned long long*)&tile[c][t], (unsigned long long)iv);
-                }
-            } else {
-                spread_add(a.facc, g.nint, sn[0] + g.nx * (sn[1] + g.ny * sn[2]), w, F);
-            }
-        });
+        } else {
+            spread_add(a.facc, g.nint, sn[0] + g.nx * (sn[1] + g.ny * sn[2]), w, F);
+        }
+    };
+    if constexpr (rows_layout<MODE, ROWS>()) ibm_rows<R>(a, pos, vel, p, m, has, lane, sink);  // every lane: row sums
+    else if (has) ibm_marker<R, MODE>(a, pos, vel, p, m, lane, sink);
     __syncthreads();
     // box position of node i by multiply and shift: for i < 1024 and extents up to 1024,
     // (i * ceil(2^20 / e)) >> 20 is i / e exactly (the excess stays below i / 2^20 < 1 / e)
     static_assert(TILE_NODES <= 1024, "the box index split below is exact for boxes up to 1024 nodes");
     const int mx = vol ? ((1 << 20) + ex - 1) / ex : 0, my = vol ? ((1 << 20) + ey - 1) / ey : 0;
     for (int i = threadIdx.x; i < vol; i += blockDim.x) {
-        const long long v0 = tile[0][i], v1 = tile[1][i], v2 = tile[2][i];
+        const long long v0 = t0[i], v1 = t1[i], v2 = t2[i];
         if ((v0 | v1 | v2) == 0) continue;
         const int ixy = (i * mx) >> 20, iz = (ixy * my) >> 20;
         const int t3[3] = {i - ixy * ex, ixy - iz * ey, iz};
@@ -309,12 +440,18 @@ __device__ __forceinline__ void ibm_tile_group(const IbmArgs& a, const D3Q19_GLO
     }
 }
 
-// one block per (sphere, group of TILE_MARKERS markers); blocks of consecutive index take different
-// spheres
-template <class R, int MODE>
-__global__ void __launch_bounds__(64 * TILE_MARKERS) ibm_tile_kernel(IbmArgs a) {
-    __shared__ long long tile[3][TILE_NODES];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+// threads per marker of a group's block: a row or a wavefront; the most threads of such a block
+// (groups of up to TILE_MARKERS_MAX markers in the row layout, else TILE_MARKERS)
+template <int MODE, bool ROWS> constexpr int marker_threads() { return rows_layout<MODE, ROWS>() ? 16 : 64; }
+template <int MODE, bool ROWS> constexpr int group_threads() {
+    return marker_threads<MODE, ROWS>() * (rows_layout<MODE, ROWS>() ? TILE_MARKERS_MAX : TILE_MARKERS);
+}
+
+// one block per (sphere, group of a.gm markers); blocks of consecutive index take different spheres
+template <class R, int MODE, bool ROWS>
+__global__ void __launch_bounds__((group_threads<MODE, ROWS>())) ibm_tile_kernel(IbmArgs a) {
+    extern __shared__ long long tile_mem[];  // 3 x a.tile_nodes
+    const int lane = threadIdx.x & 63, w = (int)threadIdx.x / marker_threads<MODE, ROWS>();  // marker slot
     const int b = blockIdx.x;
     if (b == 0 && threadIdx.x == 0 && a.commit) {
         const uint32_t e = a.epochs[0] + 1u;  // commits the step's epoch, as ibm_marker_kernel does
@@ -323,19 +460,20 @@ __global__ void __launch_bounds__(64 * TILE_MARKERS) ibm_tile_kernel(IbmArgs a) 
     }
     const int p = b % a.np, grp = b / a.np;
     if (!pass_owns(a, a.pos, p)) return;  // uniform over the block
-    const int j = grp * TILE_MARKERS + w;
+    const int j = grp * a.gm + w;
     const bool has = j < a.ppp;
-    ibm_tile_group<R, MODE>(a, a.pos, a.vel, p, grp, has ? a.perm[j] : 0, has, lane, tile);
+    ibm_tile_group<R, MODE, ROWS>(a, a.pos, a.vel, p, grp, has ? a.perm[j] : 0, has, lane, tile_mem, a.tile_nodes);
 }
 
-// Batched, spreading staged in LDS: a persistent grid of TILE_MARKERS-wavefront blocks walks the
-// group list of all cells, (sphere << 12) | first position of the group in the sphere's group
-// order, and runs each group as ibm_tile_group does for a single simulation (integer sums: the same
-// bits as one marker per wavefront).  The batch's epoch counter is committed by the first thread.
-template <class R, int MODE>
-__global__ void __launch_bounds__(64 * TILE_MARKERS) ibm_batch_tile_kernel(BatchTables t, int sp) {
-    __shared__ long long tile[3][TILE_NODES];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+// Batched, spreading staged in LDS: a persistent grid of one-group blocks walks the group list of
+// all cells, (sphere << 12) | first position of the group in the sphere's group order, and runs each
+// group as ibm_tile_group does for a single simulation (the same arithmetic: a cell's bits are those
+// of the cell alone).  The box takes TILE_NODES per component (every cell's capacity fits).  The
+// batch's epoch counter is committed by the first thread.
+template <class R, int MODE, bool ROWS>
+__global__ void __launch_bounds__((group_threads<MODE, ROWS>())) ibm_batch_tile_kernel(BatchTables t, int sp) {
+    extern __shared__ long long tile_mem[];  // 3 x TILE_NODES
+    const int lane = threadIdx.x & 63, w = (int)threadIdx.x / marker_threads<MODE, ROWS>();
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         const uint32_t e = t.epochs[0] + 1u;
         t.epochs[0] = e;
@@ -353,8 +491,8 @@ __global__ void __launch_bounds__(64 * TILE_MARKERS) ibm_batch_tile_kernel(Batch
         const int j0 = v & 4095, j = j0 + w;
         const bool has = j < ppp;
         // x(t+1), v(t+1): written by this step's turn
-        ibm_tile_group<R, MODE>(ia, cd.posb[par ^ sp ^ 1], cd.velb[par ^ sp ^ 1], sph, j0 / TILE_MARKERS, has ? ia.perm[j] : 0, has,
-                                lane, tile);
+        ibm_tile_group<R, MODE, ROWS>(ia, cd.posb[par ^ sp ^ 1], cd.velb[par ^ sp ^ 1], sph, j0 / TILE_MARKERS, has ? ia.perm[j] : 0, has,
+                                lane, tile_mem, TILE_NODES);
         __syncthreads();  // the next group reuses the tile
     }
 }
@@ -378,6 +516,8 @@ IbmArgs make_ibm_args(Sim* s) {
     IbmArgs a;
     a.mstride = marker_stride(s->ppp > 0 ? s->ppp : 1);
     a.tile_ok = s->tile_ok ? 1 : 0;
+    a.tile_nodes = s->tile_nodes;
+    a.gm = s->group_markers;
     a.perm = gp(s->perm);
     a.gbox = gp(s->gbox);
     a.g = s->g;
@@ -402,9 +542,23 @@ IbmArgs make_ibm_args(Sim* s) {
 
 // a kernel launch, timed by events at the kernel's own start and end when ev0 is set
 template <class K, class... Args>
-static void launch_timed(K kernel, dim3 grid, dim3 block, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1, Args... args) {
-    if (ev0 && ev1) hipExtLaunchKernelGGL(kernel, grid, block, 0, st, ev0, ev1, 0, args...);
-    else hipLaunchKernelGGL(kernel, grid, block, 0, st, args...);
+static void launch_timed(K kernel, dim3 grid, dim3 block, size_t lds, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1, Args... args) {
+    if (ev0 && ev1) hipExtLaunchKernelGGL(kernel, grid, block, lds, st, ev0, ev1, 0, args...);
+    else hipLaunchKernelGGL(kernel, grid, block, lds, st, args...);
+}
+
+template <class R, int MODE, bool ROWS>
+static void launch_ibm_as(Sim* s, const IbmArgs& a, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1) {
+    if (s->tile_ok) {
+        const int gm = s->group_markers;
+        const int blocks = s->np * ((s->ppp + gm - 1) / gm);
+        const size_t lds = 3 * sizeof(long long) * (size_t)s->tile_nodes;
+        launch_timed(ibm_tile_kernel<R, MODE, ROWS>, dim3(blocks), dim3(marker_threads<MODE, ROWS>() * gm), lds, st, ev0, ev1, a);
+    } else {
+        const int total = s->np * s->ppp;
+        const int per_block = IBM_WAVES * (rows_layout<MODE, ROWS>() ? 4 : 1);
+        launch_timed(ibm_marker_kernel<R, MODE, ROWS>, dim3((total + per_block - 1) / per_block), dim3(64 * IBM_WAVES), 0, st, ev0, ev1, a);
+    }
 }
 
 int launch_ibm(Sim* s, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1, int zlo, int zhi, int commit) {
@@ -414,41 +568,42 @@ int launch_ibm(Sim* s, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1, int zlo, 
     a.zhi = zhi;
     a.commit = commit;
     const bool dbl = s->storage == D3Q19_STORE_FP64;
-    if (s->tile_ok) {
-        const int blocks = s->np * ((s->ppp + TILE_MARKERS - 1) / TILE_MARKERS);
-        const dim3 th(64 * TILE_MARKERS);
-        if (a.mode == D3Q19_IBM_PESKIN4) {
-            if (dbl) launch_timed(ibm_tile_kernel<double, D3Q19_IBM_PESKIN4>, dim3(blocks), dim3(th), st, ev0, ev1, a);
-            else launch_timed(ibm_tile_kernel<float, D3Q19_IBM_PESKIN4>, dim3(blocks), dim3(th), st, ev0, ev1, a);
-        } else {
-            if (dbl) launch_timed(ibm_tile_kernel<double, D3Q19_IBM_LINEAR_SMOOTHED>, dim3(blocks), dim3(th), st, ev0, ev1, a);
-            else launch_timed(ibm_tile_kernel<float, D3Q19_IBM_LINEAR_SMOOTHED>, dim3(blocks), dim3(th), st, ev0, ev1, a);
-        }
-        HIP_OK(hipGetLastError());
-        return 0;
-    }
-    const int total = s->np * s->ppp;
-    const int th = 64 * IBM_WAVES, blocks = (total + IBM_WAVES - 1) / IBM_WAVES;  // one wavefront per marker
     if (a.mode == D3Q19_IBM_PESKIN4) {
-        if (dbl) launch_timed(ibm_marker_kernel<double, D3Q19_IBM_PESKIN4>, dim3(blocks), dim3(th), st, ev0, ev1, a);
-        else launch_timed(ibm_marker_kernel<float, D3Q19_IBM_PESKIN4>, dim3(blocks), dim3(th), st, ev0, ev1, a);
+        if (s->ibm_rows) {
+            if (dbl) launch_ibm_as<double, D3Q19_IBM_PESKIN4, true>(s, a, st, ev0, ev1);
+            else launch_ibm_as<float, D3Q19_IBM_PESKIN4, true>(s, a, st, ev0, ev1);
+        } else {
+            if (dbl) launch_ibm_as<double, D3Q19_IBM_PESKIN4, false>(s, a, st, ev0, ev1);
+            else launch_ibm_as<float, D3Q19_IBM_PESKIN4, false>(s, a, st, ev0, ev1);
+        }
     } else {
-        if (dbl) launch_timed(ibm_marker_kernel<double, D3Q19_IBM_LINEAR_SMOOTHED>, dim3(blocks), dim3(th), st, ev0, ev1, a);
-        else launch_timed(ibm_marker_kernel<float, D3Q19_IBM_LINEAR_SMOOTHED>, dim3(blocks), dim3(th), st, ev0, ev1, a);
+        if (dbl) launch_ibm_as<double, D3Q19_IBM_LINEAR_SMOOTHED, false>(s, a, st, ev0, ev1);
+        else launch_ibm_as<float, D3Q19_IBM_LINEAR_SMOOTHED, false>(s, a, st, ev0, ev1);
     }
     HIP_OK(hipGetLastError());
     return 0;
 }
 
+template <class R, int MODE, bool ROWS>
+static void launch_ibm_batch_as(const BatchLaunch& L, int sp, hipStream_t st) {
+    const size_t lds = 3 * sizeof(long long) * (size_t)TILE_NODES;
+    hipLaunchKernelGGL((ibm_batch_tile_kernel<R, MODE, ROWS>), dim3(L.grid_ibm), dim3(marker_threads<MODE, ROWS>() * TILE_MARKERS), lds, st,
+                       L.t, sp);
+}
+
 int launch_ibm_batch(const BatchLaunch& L, int sp, hipStream_t st) {
     const bool dbl = L.storage == D3Q19_STORE_FP64;
-    const dim3 grid(L.grid_ibm), block(64 * TILE_MARKERS);
     if (L.ibm_mode == D3Q19_IBM_PESKIN4) {
-        if (dbl) hipLaunchKernelGGL((ibm_batch_tile_kernel<double, D3Q19_IBM_PESKIN4>), grid, block, 0, st, L.t, sp);
-        else hipLaunchKernelGGL((ibm_batch_tile_kernel<float, D3Q19_IBM_PESKIN4>), grid, block, 0, st, L.t, sp);
+        if (L.ibm_rows) {
+            if (dbl) launch_ibm_batch_as<double, D3Q19_IBM_PESKIN4, true>(L, sp, st);
+            else launch_ibm_batch_as<float, D3Q19_IBM_PESKIN4, true>(L, sp, st);
+        } else {
+            if (dbl) launch_ibm_batch_as<double, D3Q19_IBM_PESKIN4, false>(L, sp, st);
+            else launch_ibm_batch_as<float, D3Q19_IBM_PESKIN4, false>(L, sp, st);
+        }
     } else {
-        if (dbl) hipLaunchKernelGGL((ibm_batch_tile_kernel<double, D3Q19_IBM_LINEAR_SMOOTHED>), grid, block, 0, st, L.t, sp);
-        else hipLaunchKernelGGL((ibm_batch_tile_kernel<float, D3Q19_IBM_LINEAR_SMOOTHED>), grid, block, 0, st, L.t, sp);
+        if (dbl) launch_ibm_batch_as<double, D3Q19_IBM_LINEAR_SMOOTHED, false>(L, sp, st);
+        else launch_ibm_batch_as<float, D3Q19_IBM_LINEAR_SMOOTHED, false>(L, sp, st);
     }
     HIP_OK(hipGetLastError());
     return 0;

@@ -34,6 +34,10 @@ constexpr double FIX_INV = 1.0 / 1099511627776.0;
 constexpr int BRICK = 4;                              // brick edge (nodes)
 constexpr int IBM_WAVES = 4;                          // markers (wavefronts) per IBM block
 constexpr int TILE_MARKERS = 8;                       // markers per block of the LDS-staged spreading
+constexpr long long INFINITY_CACHE_BYTES = 256ll << 20;  // MI355X's last-level (memory-side) cache
+constexpr long long IBM_ROWS_MIN_MARKERS = 16384;        // markers from which four per wavefront pay
+constexpr int IBM_GROUP16_MIN_BLOCKS = 1024;             // blocks (groups) needed to take groups of 16
+constexpr int TILE_MARKERS_MAX = 16;                  // ... of a single simulation's Peskin-4 groups whose boxes fit
 constexpr int TILE_NODES = 1024;                      // LDS box of that spreading (nodes)
 constexpr float FP16_SCALE = 16384.0f;                // 2^14, relative-deviation scale of fp16 storage
 constexpr int MARK_BRICKS = 64;                       // bricks per axis a sphere's marking can list
@@ -208,6 +212,7 @@ struct CollideArgs {
     int nbx, nby;          // tiles per plane
     int nblk;              // tiles of this launch
     int z0;                // first z plane of this launch's tiles (a z-slab of the lattice)
+    int nt_stores;         // population stores non-temporal (lattices beyond the Infinity Cache)
 };
 
 struct IbmArgs {
@@ -215,6 +220,8 @@ struct IbmArgs {
     int np, ppp, mode;
     int mstride;  // marker visiting order: wavefront k of sphere p takes marker (k * mstride) mod ppp
     int tile_ok;  // spreading may be staged in LDS (every periodic extent exceeds twice the stencil reach)
+    int tile_nodes;  // LDS box capacity of this simulation's launches (the largest group box, at most TILE_NODES)
+    int gm;          // markers per group (TILE_MARKERS, or TILE_MARKERS_MAX for a single simulation's Peskin-4 groups)
     const D3Q19_GLOBAL int* perm;  // ppp marker indices in Morton order of their offsets
     const D3Q19_GLOBAL int* gbox;  // per group of TILE_MARKERS: LDS box relative to the centre node (lo[3], extent[3], volume, 0)
     double dA;
@@ -370,6 +377,10 @@ struct Sim {
     int* perm = nullptr;       // ppp marker indices in Morton order of their offsets (LDS-staged spreading)
     int* gbox = nullptr;       // the LDS box of each group of TILE_MARKERS markers (8 ints per group)
     bool tile_ok = false;      // every periodic extent exceeds twice the spheres' stencil reach
+    int tile_nodes = 0;        // the largest group box (nodes), capped at TILE_NODES: the LDS the IBM launches reserve
+    int group_markers = TILE_MARKERS;  // markers per group of the LDS-staged spreading
+    bool ibm_rows = false;     // Peskin-4 markers run four to a wavefront (many markers); else one per wavefront
+    bool nt_stores = false;    // population stores non-temporal (both population buffers exceed the Infinity Cache)
     double mark_rin = 0.0, mark_rout = 0.0;  // shortest / longest marker offset, with a margin
     long long h_epoch = 0;     // host count of committed epochs (one per step)
     int tofs = 0;              // step number t of the contacts = committed epoch + tofs
@@ -468,7 +479,9 @@ struct BatchLaunch {
     int grid_collide;      // blocks of the collide kernel over the constant-tau cells' tiles
     int grid_collide_vt;   // blocks of the collide kernel over the per-node-tau cells' tiles
     int grid_sphere;       // persistent blocks of the sphere kernels
-    int grid_ibm;          // persistent blocks (4 markers each per iteration) of the IBM kernel
+    int grid_ibm;          // persistent blocks (one marker group each per iteration) of the IBM kernel
+    int ibm_rows;          // Peskin-4 markers four to a wavefront (batches of many markers)
+    int nt_stores;         // non-temporal population stores (members' lattices beyond the Infinity Cache)
 };
 int launch_collide_batch(const BatchLaunch& L, int sp, int write_u_all, int probe, hipStream_t st);
 // a batched step's sphere turn (its own launch) and collide

@@ -33,7 +33,17 @@ __device__ __forceinline__ uint16_t f32_to_bf16(float f) {
 
 // Storage formats.  st rounds to nearest; st_sr takes 32 random bits r and rounds stochastically
 // (unbiased: the result's expectation over uniform r is the stored value), which only the 16-bit
-// formats need.
+// formats need.  NT: the store is non-temporal.  The collide's population stores are, on lattices
+// whose two population buffers exceed the 256 MB Infinity Cache (the kernels' NT argument, chosen by
+// CollideArgs::nt_stores; a runtime branch between the two forms was merged by the compiler): the next
+// read of a population is a whole sweep later, and the stores otherwise fill the L2 with dirty lines
+// that the IBM pass, launched right after the collide, has to write back first.  On the
+// 256x256x128 workload the fused step kernel went from 348 to 334 us (fp32) and the graph-replayed
+// step by 5%; on a cache-resident lattice (128^3 in fp16) they lost the cache residency: 10% slower.
+template <bool NT, class T, class PT> __device__ __forceinline__ void st_pop(PT p, unsigned off, T v) {
+    if constexpr (NT) __builtin_nontemporal_store(v, plane_at(p, off));
+    else *plane_at(p, off) = v;
+}
 template <int ST> struct Pop;
 template <> struct Pop<D3Q19_STORE_FP32> {
     using T = float;
@@ -41,8 +51,8 @@ template <> struct Pop<D3Q19_STORE_FP32> {
     static constexpr bool SR = false;
     template <int QI, class PT> __device__ static R ld(PT p, unsigned off) { return *plane_at(p, off); }
     template <int QI, class PT> __device__ static double ldd(PT p, unsigned off) { return (double)*plane_at(p, off); }
-    template <int QI, class PT> __device__ static void st(PT p, unsigned off, R v) { *plane_at(p, off) = v; }
-    template <int QI, class PT> __device__ static void st_sr(PT p, unsigned off, R v, uint32_t) { *plane_at(p, off) = v; }
+    template <int QI, bool NT = false, class PT> __device__ static void st(PT p, unsigned off, R v) { st_pop<NT>(p, off, v); }
+    template <int QI, bool NT = false, class PT> __device__ static void st_sr(PT p, unsigned off, R v, uint32_t) { st_pop<NT>(p, off, v); }
 };
 template <> struct Pop<D3Q19_STORE_FP64> {
     using T = double;
@@ -50,8 +60,8 @@ template <> struct Pop<D3Q19_STORE_FP64> {
     static constexpr bool SR = false;
     template <int QI, class PT> __device__ static R ld(PT p, unsigned off) { return *plane_at(p, off); }
     template <int QI, class PT> __device__ static double ldd(PT p, unsigned off) { return *plane_at(p, off); }
-    template <int QI, class PT> __device__ static void st(PT p, unsigned off, R v) { *plane_at(p, off) = v; }
-    template <int QI, class PT> __device__ static void st_sr(PT p, unsigned off, R v, uint32_t) { *plane_at(p, off) = v; }
+    template <int QI, bool NT = false, class PT> __device__ static void st(PT p, unsigned off, R v) { st_pop<NT>(p, off, v); }
+    template <int QI, bool NT = false, class PT> __device__ static void st_sr(PT p, unsigned off, R v, uint32_t) { st_pop<NT>(p, off, v); }
 };
 // binary16 of the relative deviation (f_q - w_q)/w_q scaled by 2^14: deviations down to ~1e-8
 // stay normal numbers and the headroom reaches |dev| = 4.
@@ -67,20 +77,20 @@ template <> struct Pop<D3Q19_STORE_FP16_SHIFT> {
         return wq(QI) + (double)*plane_at(p, off) * (wq(QI) / 16384.0);
     }
     static constexpr bool SR = true;
-    template <int QI, class PT> __device__ static void st(PT p, unsigned off, R v) {
+    template <int QI, bool NT = false, class PT> __device__ static void st(PT p, unsigned off, R v) {
         constexpr float w = (float)wq(QI);
         constexpr float isc = (float)(16384.0 / wq(QI));
-        *plane_at(p, off) = (_Float16)((v - w) * isc);
+        st_pop<NT>(p, off, (_Float16)((v - w) * isc));
     }
     // the 13 magnitude bits that binary16 drops, plus 13 random bits, carry into the kept ones with
     // probability equal to the dropped fraction; the truncated value then converts exactly (normal
     // binary16 range; |f - w|/w below 2^-28 also rounds to nearest, which is immaterial).  The
     // collide measured 2% faster this way than with v_cvt_sr_f16_f32.
-    template <int QI, class PT> __device__ static void st_sr(PT p, unsigned off, R v, uint32_t r) {
+    template <int QI, bool NT = false, class PT> __device__ static void st_sr(PT p, unsigned off, R v, uint32_t r) {
         constexpr float w = (float)wq(QI);
         constexpr float isc = (float)(16384.0 / wq(QI));
         const uint32_t b = (__float_as_uint((v - w) * isc) + (r >> 19)) & 0xFFFFE000u;
-        *plane_at(p, off) = (_Float16)__uint_as_float(b);
+        st_pop<NT>(p, off, (_Float16)__uint_as_float(b));
     }
 };
 // bfloat16 of f_q - w_q
@@ -95,15 +105,15 @@ template <> struct Pop<D3Q19_STORE_BF16_SHIFT> {
         return wq(QI) + (double)bf16_to_f32(*plane_at(p, off));
     }
     static constexpr bool SR = true;
-    template <int QI, class PT> __device__ static void st(PT p, unsigned off, R v) {
+    template <int QI, bool NT = false, class PT> __device__ static void st(PT p, unsigned off, R v) {
         constexpr float w = (float)wq(QI);
-        *plane_at(p, off) = f32_to_bf16(v - w);
+        st_pop<NT>(p, off, f32_to_bf16(v - w));
     }
     // the 16 discarded magnitude bits plus 16 random bits carry into the kept ones with
     // probability equal to the discarded fraction (finite inputs)
-    template <int QI, class PT> __device__ static void st_sr(PT p, unsigned off, R v, uint32_t r) {
+    template <int QI, bool NT = false, class PT> __device__ static void st_sr(PT p, unsigned off, R v, uint32_t r) {
         constexpr float w = (float)wq(QI);
-        *plane_at(p, off) = (uint16_t)((__float_as_uint(v - w) + (r >> 16)) >> 16);
+        st_pop<NT>(p, off, (uint16_t)((__float_as_uint(v - w) + (r >> 16)) >> 16));
     }
 };
 
@@ -194,11 +204,8 @@ __host__ __device__ inline int plane_size(const Geom& g, int A) {
 // Gather the pre-collision populations of node (x,y,z): pull from the post-collision field with
 // periodic wrap, then the wall unknowns (rebuilt, or the raw uploaded values once after an upload),
 // then the optional tangential-wall passes (sim_main.c:388-404).
-// WALLZ = true: specialisation for the configuration of every shipped input and benchmark
-// (periodic x and y, moving walls normal to z, no tangential passes), which keeps f[] in
-// registers; WALLZ = false handles any wall axis, none, and the tangential passes.
 // RO = double decodes the stored populations in double precision (observables).
-template <int ST, bool WALLZ, class RO = typename Pop<ST>::R, class PIn = const typename Pop<ST>::T*>
+template <int ST, class RO = typename Pop<ST>::R, class PIn = const typename Pop<ST>::T*>
 __device__ __forceinline__ void gather_pre(PIn fin, const Geom& g, const FluidParams& fl,
                                            const void* bcraw, int use_raw, int x, int y, int z, RO* f) {
     using P = Pop<ST>;
@@ -355,7 +362,7 @@ __device__ __forceinline__ R strain_rate(const R* ss, R ux, R uy, R uz, R gx, R 
 // when a.vt is set (batched cells of mixed rheology).  probe = 1 (autotuning trials) leaves the
 // pending spread force, the tau field and u untouched so a trial launch does not change the state
 // a later step reads (its populations go to the lattice the next step overwrites).
-template <int ST, int COLL, int VT, bool WZ, bool SBASE_STORES = true>
+template <int ST, int COLL, int VT, bool NT, bool SBASE_STORES = true>
 __device__ __forceinline__ void collide_node(const CollideArgs& a, const void* fin, void* fout, int write_u_all, int x, int y,
                                              int z, int probe) {
     using P = Pop<ST>;
@@ -397,7 +404,7 @@ __device__ __forceinline__ void collide_node(const CollideArgs& a, const void* f
     }
 
     R f[Q];
-    gather_pre<ST, WZ>(to_global<T>(fin), g, a.fl, a.bcraw, a.use_bcraw, x, y, z, f);
+    gather_pre<ST>(to_global<T>(fin), g, a.fl, a.bcraw, a.use_bcraw, x, y, z, f);
     if constexpr (stage_tau) {
         if (local_tau) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     }
@@ -558,18 +565,18 @@ __device__ __forceinline__ void collide_node(const CollideArgs& a, const void* f
         if constexpr (SBASE_STORES) asm volatile("" : "+v"(ns));
         Unroll<0, Q>::run([&](auto qc) __attribute__((always_inline)) {
             constexpr int q = decltype(qc)::value;
-            P::template st_sr<q>(out + (long long)q * g.qstride, ns, fo[q], sr_word<q>(h));
+            P::template st_sr<q, NT>(out + (long long)q * g.qstride, ns, fo[q], sr_word<q>(h));
         });
     } else {
         if constexpr (SBASE_STORES) asm volatile("" : "+v"(ns));
         Unroll<0, Q>::run([&](auto qc) __attribute__((always_inline)) {
             constexpr int q = decltype(qc)::value;
-            P::template st<q>(out + (long long)q * g.qstride, ns, fo[q]);
+            P::template st<q, NT>(out + (long long)q * g.qstride, ns, fo[q]);
         });
     }
 }
 
-template <int ST, int COLL, int VT, bool WZ>
+template <int ST, int COLL, int VT, bool NT>
 __global__ void __launch_bounds__(256) collide_kernel(CollideArgs a) {
     // Tiles are taken in dispatch order (x fastest, then y, then z).  An XCD-contiguous remap of
     // the tiles (each XCD owning a z-slab) measured 2-7% slower on MI355X for 8-34 M node lattices:
@@ -585,7 +592,7 @@ __global__ void __launch_bounds__(256) collide_kernel(CollideArgs a) {
     const int x = txb * a.bx + (int)(threadIdx.x % a.bx);
     const int y = tyb * a.by + (int)(threadIdx.x / a.bx);
     if (x >= a.g.nx || y >= a.g.ny || z >= a.g.nz || z < 0) return;
-    collide_node<ST, COLL, VT, WZ>(a, a.fin, a.fout, a.write_u_all, x, y, z, 0);
+    collide_node<ST, COLL, VT, NT>(a, a.fin, a.fout, a.write_u_all, x, y, z, 0);
 }
 
 // Fused step: blocks [0, nsph) run the spheres' turn (sphere_turn: the force sums and contacts of
@@ -593,7 +600,7 @@ __global__ void __launch_bounds__(256) collide_kernel(CollideArgs a) {
 // the epoch after this one); the other blocks are the collide tiles of collide_kernel.  Nothing a
 // sphere block writes is read by a collide block of the same launch: the collide reads the marks of
 // this epoch and of the pending force, the turn writes the marks of the next epoch (a third array).
-template <int ST, int COLL, int VT, bool WZ>
+template <int ST, int COLL, int VT, bool NT>
 __device__ __forceinline__ void step_body(const CollideArgs& a, const PartArgs& p, int nsph, int do_post) {
     if ((int)blockIdx.x < nsph) {
         sphere_turn(p, blockIdx.x, do_post, p.epochs[0] + 2u);
@@ -609,21 +616,21 @@ __device__ __forceinline__ void step_body(const CollideArgs& a, const PartArgs& 
     const int x = txb * a.bx + (int)(threadIdx.x % a.bx);
     const int y = tyb * a.by + (int)(threadIdx.x / a.bx);
     if (x >= a.g.nx || y >= a.g.ny || z >= a.g.nz || z < 0) return;
-    collide_node<ST, COLL, VT, WZ>(a, a.fin, a.fout, a.write_u_all, x, y, z, 0);
+    collide_node<ST, COLL, VT, NT>(a, a.fin, a.fout, a.write_u_all, x, y, z, 0);
 }
 
-template <int ST, int COLL, int VT, bool WZ>
+template <int ST, int COLL, int VT, bool NT>
 __global__ void __launch_bounds__(256) step_kernel(CollideArgs a, PartArgs p, int nsph, int do_post) {
-    step_body<ST, COLL, VT, WZ>(a, p, nsph, do_post);
+    step_body<ST, COLL, VT, NT>(a, p, nsph, do_post);
 }
 
 // The same step with the register budget of 6 waves per SIMD (80 VGPRs), which the BGK forms in
 // fp32 / fp16 / bf16 storage meet without spilling; left alone, the sphere turn's code raises the
 // whole kernel to 89 VGPRs (5 waves) while the collide tiles need 77.
-template <int ST, int COLL, int VT, bool WZ>
+template <int ST, int COLL, int VT, bool NT>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) step_kernel_w6(CollideArgs a, PartArgs p, int nsph,
                                                                                             int do_post) {
-    step_body<ST, COLL, VT, WZ>(a, p, nsph, do_post);
+    step_body<ST, COLL, VT, NT>(a, p, nsph, do_post);
 }
 
 // flat node index of a cell -> (x, y, z); exact for n < 2^24 (float estimate within +-1, corrected)
@@ -644,7 +651,7 @@ __device__ __forceinline__ void split_node(const CellDesc& cd, int n, int& x, in
 // VT selects the tile range: 0 the constant-tau cells' tiles [0, ntiles_ct), 1 the per-node-tau
 // cells' tiles [ntiles_ct, ntiles).  Two specialised launches keep the constant-tau cells free of
 // the per-node-tau code's registers (occupancy).
-template <int ST, int COLL, int VT>
+template <int ST, int COLL, int VT, bool NT>
 __device__ __forceinline__ void collide_batch_tiles(const BatchTables& t, const CellDesc* __restrict__ cells, int sp, int write_u_all,
                                                     int probe, int first, int stride) {
     const int lo = VT ? t.cnt->ntiles_ct : 0;
@@ -660,28 +667,28 @@ __device__ __forceinline__ void collide_batch_tiles(const BatchTables& t, const 
             if (n >= cd.nint) break;
             int x, y, z;
             split_node(cd, n, x, y, z);
-            collide_node<ST, COLL, VT, true, false>(cd.c, fin, fout, write_u_all, x, y, z, probe);
+            collide_node<ST, COLL, VT, NT, false>(cd.c, fin, fout, write_u_all, x, y, z, probe);
         }
     }
 }
 
-template <int ST, int COLL, int VT>
+template <int ST, int COLL, int VT, bool NT>
 __global__ void __launch_bounds__(256) collide_batch_kernel(BatchTables t, const CellDesc* __restrict__ cells, int sp, int write_u_all,
                                                             int probe) {
-    collide_batch_tiles<ST, COLL, VT>(t, cells, sp, write_u_all, probe, blockIdx.x, gridDim.x);
+    collide_batch_tiles<ST, COLL, VT, NT>(t, cells, sp, write_u_all, probe, blockIdx.x, gridDim.x);
 }
 
 // With events ev0/ev1, they are recorded at the kernel's own start and end (hipExtLaunchKernelGGL),
 // so their interval is the kernel's duration without the dispatch gap before it.
 template <int ST, int COLL, int VT>
 static void launch_one(const CollideArgs& a, int threads, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1) {
-    const bool wz = a.g.wall == 2 && a.g.per[0] && a.g.per[1] && a.fl.tan_mask == 0;
+    const bool nt = a.nt_stores;
     if (ev0) {
-        if (wz) hipExtLaunchKernelGGL((collide_kernel<ST, COLL, VT, true>), dim3(a.nblk), dim3(threads), 0, st, ev0, ev1, 0, a);
+        if (nt) hipExtLaunchKernelGGL((collide_kernel<ST, COLL, VT, true>), dim3(a.nblk), dim3(threads), 0, st, ev0, ev1, 0, a);
         else hipExtLaunchKernelGGL((collide_kernel<ST, COLL, VT, false>), dim3(a.nblk), dim3(threads), 0, st, ev0, ev1, 0, a);
         return;
     }
-    if (wz) hipLaunchKernelGGL((collide_kernel<ST, COLL, VT, true>), dim3(a.nblk), dim3(threads), 0, st, a);
+    if (nt) hipLaunchKernelGGL((collide_kernel<ST, COLL, VT, true>), dim3(a.nblk), dim3(threads), 0, st, a);
     else hipLaunchKernelGGL((collide_kernel<ST, COLL, VT, false>), dim3(a.nblk), dim3(threads), 0, st, a);
 }
 
@@ -722,6 +729,7 @@ CollideArgs make_collide_args(Sim* s, const void* fin, void* fout, int write_u_a
     a.nby = (s->g.ny + a.by - 1) / a.by;
     a.nblk = a.nbx * a.nby * s->g.nz;
     a.z0 = 0;
+    a.nt_stores = s->nt_stores ? 1 : 0;
     return a;
 }
 
@@ -751,13 +759,13 @@ static void launch_maybe_timed(K kernel, dim3 grid, dim3 block, hipStream_t st, 
 template <int ST, int COLL, int VT>
 static void launch_fused_one(const CollideArgs& a, const PartArgs& p, int nsph, int do_post, int threads, hipStream_t st,
                              hipEvent_t ev0, hipEvent_t ev1) {
-    const bool wz = a.g.wall == 2 && a.g.per[0] && a.g.per[1] && a.fl.tan_mask == 0;
+    const bool nt = a.nt_stores;
     const dim3 grid(a.nblk + nsph);
     if constexpr (COLL == D3Q19_COLL_BGK && ST != D3Q19_STORE_FP64) {
-        if (wz) launch_maybe_timed(step_kernel_w6<ST, COLL, VT, true>, grid, dim3(threads), st, ev0, ev1, a, p, nsph, do_post);
+        if (nt) launch_maybe_timed(step_kernel_w6<ST, COLL, VT, true>, grid, dim3(threads), st, ev0, ev1, a, p, nsph, do_post);
         else launch_maybe_timed(step_kernel_w6<ST, COLL, VT, false>, grid, dim3(threads), st, ev0, ev1, a, p, nsph, do_post);
     } else {
-        if (wz) launch_maybe_timed(step_kernel<ST, COLL, VT, true>, grid, dim3(threads), st, ev0, ev1, a, p, nsph, do_post);
+        if (nt) launch_maybe_timed(step_kernel<ST, COLL, VT, true>, grid, dim3(threads), st, ev0, ev1, a, p, nsph, do_post);
         else launch_maybe_timed(step_kernel<ST, COLL, VT, false>, grid, dim3(threads), st, ev0, ev1, a, p, nsph, do_post);
     }
 }
@@ -807,14 +815,20 @@ int launch_step_fused(Sim* s, const void* fin, void* fout, int write_u_all, int 
     return 0;
 }
 
-template <int ST, int COLL>
-static void collide_batch_coll(const BatchLaunch& L, int sp, int write_u_all, int probe, hipStream_t st) {
+template <int ST, int COLL, bool NT>
+static void collide_batch_nt(const BatchLaunch& L, int sp, int write_u_all, int probe, hipStream_t st) {
     if (L.grid_collide > 0)
-        hipLaunchKernelGGL((collide_batch_kernel<ST, COLL, 0>), dim3(L.grid_collide), dim3(L.tile), 0, st, L.t, L.t.cells, sp,
+        hipLaunchKernelGGL((collide_batch_kernel<ST, COLL, 0, NT>), dim3(L.grid_collide), dim3(L.tile), 0, st, L.t, L.t.cells, sp,
                            write_u_all, probe);
     if (L.grid_collide_vt > 0)
-        hipLaunchKernelGGL((collide_batch_kernel<ST, COLL, 1>), dim3(L.grid_collide_vt), dim3(L.tile), 0, st, L.t, L.t.cells, sp,
+        hipLaunchKernelGGL((collide_batch_kernel<ST, COLL, 1, NT>), dim3(L.grid_collide_vt), dim3(L.tile), 0, st, L.t, L.t.cells, sp,
                            write_u_all, probe);
+}
+
+template <int ST, int COLL>
+static void collide_batch_coll(const BatchLaunch& L, int sp, int write_u_all, int probe, hipStream_t st) {
+    if (L.nt_stores) collide_batch_nt<ST, COLL, true>(L, sp, write_u_all, probe, st);
+    else collide_batch_nt<ST, COLL, false>(L, sp, write_u_all, probe, st);
 }
 
 template <int ST>
@@ -894,7 +908,7 @@ __global__ void export_kernel(const typename Pop<ST>::T* __restrict__ fin, Geom 
     const int y = (int)((idx / g.nx) % g.ny);
     const int z = (int)(idx / ((long long)g.nx * g.ny));
     R f[Q];
-    gather_pre<ST, false>(fin, g, fl, raw, use_raw, x, y, z, f);
+    gather_pre<ST>(fin, g, fl, raw, use_raw, x, y, z, f);
     const long long ri = (x + 1) + (long long)Nx * ((y + 1) + (long long)Ny * (z + 1));
     Unroll<0, Q>::run([&](auto qc) __attribute__((always_inline)) { ref[decltype(qc)::value * nn + ri] = (OUT)f[decltype(qc)::value]; });
 }
@@ -1081,7 +1095,7 @@ __global__ void moments_kernel(const typename Pop<ST>::T* __restrict__ fin, Geom
     for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < g.nint; idx += (long long)gridDim.x * blockDim.x) {
         const int x = (int)(idx % g.nx), y = (int)((idx / g.nx) % g.ny), z = (int)(idx / ((long long)g.nx * g.ny));
         R f[Q];
-        gather_pre<ST, false>(fin, g, fl, raw, use_raw, x, y, z, f);
+        gather_pre<ST>(fin, g, fl, raw, use_raw, x, y, z, f);
         double r = 0, jx = 0, jy = 0, jz = 0;
         Unroll<0, Q>::run([&](auto qc) __attribute__((always_inline)) {
             constexpr int q = decltype(qc)::value;
@@ -1146,7 +1160,7 @@ __device__ double node_stress(const CellDesc& cd, PIn fin, int x, int y, int z) 
     using R = typename Pop<ST>::R;
     const CollideArgs& c = cd.c;
     double f[Q];
-    gather_pre<ST, false, double>(fin, c.g, c.fl, c.bcraw, c.use_bcraw, x, y, z, f);
+    gather_pre<ST, double>(fin, c.g, c.fl, c.bcraw, c.use_bcraw, x, y, z, f);
     const int A = cd.o.axisA, F = cd.o.axisF;
     double rho = 0, jx = 0, jy = 0, jz = 0, pi = 0;
     Unroll<0, Q>::run([&](auto qc) __attribute__((always_inline)) {
