@@ -70,7 +70,7 @@ static int validate(const int_param_struct* ip, const flp_param_struct* fp, cons
             return fail(D3Q19_E_INVALID, "unknown rounding mode");
         if (opt->overlap_slabs < 0 || opt->overlap_slabs > Sim::MAX_SLABS)
             return fail(D3Q19_E_INVALID, "overlap_slabs must be 0 (automatic) to 16");
-        if (opt->ibm_layout < D3Q19_IBM_LAYOUT_AUTO || opt->ibm_layout > D3Q19_IBM_LAYOUT_ROWS_G16)
+        if (opt->ibm_layout < D3Q19_IBM_LAYOUT_AUTO || opt->ibm_layout > D3Q19_IBM_LAYOUT_ROWS_LARGE)
             return fail(D3Q19_E_INVALID, "ibm_layout must be 0 (automatic) to 3");
         if (opt->contact_detection != D3Q19_CONTACT_ZONES && opt->contact_detection != D3Q19_CONTACT_ALL_PAIRS)
             return fail(D3Q19_E_INVALID, "unknown contact detection");
@@ -548,12 +548,10 @@ int sim_init(Sim* s, int device, const int_param_struct* ip, const flp_param_str
     // the other (by z, then azimuth), and each takes its gm - 1 nearest unassigned markers by the
     // largest per-axis distance (ties by index, so the grouping is deterministic).  Against chunks
     // of a Morton order this shrinks the largest box on the 192-marker, d = 12 set from 1890 to 832
-    // nodes.  A single simulation with the Peskin-4 kernel takes groups of TILE_MARKERS_MAX markers
-    // when all their boxes fit in TILE_NODES (fewer, fuller blocks and a larger overlap: the
-    // 256x256x128 workload's IBM pass 74 -> 60 us), otherwise TILE_MARKERS, the size cells of a
-    // batch always use.  The grouping decides only which block adds a contribution: integer sums make
-    // the spread force independent of it.  Staging is enabled where every periodic extent exceeds
-    // twice the stencil reach (|offset| + 3 nodes) plus a margin.
+    // nodes (groups of 8).  The group size is chosen below.  The grouping decides only which block
+    // adds a contribution: integer sums make the spread force independent of it.  Staging is
+    // enabled where every periodic extent exceeds twice the stencil reach (|offset| + 3 nodes) plus
+    // a margin.
     std::vector<int> perm, gbox;
     if (s->np > 0) {
         double rmax = 0.0, rmin = 1e300;
@@ -626,29 +624,40 @@ int sim_init(Sim* s, int device, const int_param_struct* ip, const flp_param_str
             return vmax;
         };
         // Peskin-4 markers four to a wavefront when there are many (a quarter of the wavefronts,
-        // each with a third of the instructions per marker: the 256x256x128 workload's 64,000
-        // markers 77 -> 57 us with groups of 16); with few markers one per wavefront keeps more
-        // wavefronts in flight (the 128^3 workload's 6,144 markers: 12.4 against 14.2 us).  Both
-        // sum a marker's stencil in the same order, so the choice changes no result bit.
+        // each with 40% of the instructions per marker); with few markers one per wavefront keeps
+        // more wavefronts in flight (the 128^3 workload's 6,144 markers: 12.4 against 14.2 us).
+        // In the row layout a single simulation takes the largest groups, up to TILE_MARKERS_MAX,
+        // whose boxes all fit in TILE_NODES while at least IBM_LARGE_GROUP_MIN_BLOCKS blocks remain:
+        // fewer global atomics, since a group's box adds each node once.  The 256x256x128
+        // workload's IBM pass (64,000 markers): groups of 8 74 us, 16 57 us, 32 52 us, 64 50 us.
+        // Cells of a batch always take groups of TILE_MARKERS, boxes up to TILE_NODES_BATCH.  Both
+        // layouts sum a marker's stencil in the same order and the grouping decides only which
+        // block adds a contribution, so no choice here changes a result bit.
         const int layout = s->opt.ibm_layout;
         const bool peskin = s->opt.ibm_kernel == D3Q19_IBM_PESKIN4;
-        const long long blocks16 = (long long)s->np * ((s->ppp + TILE_MARKERS_MAX - 1) / TILE_MARKERS_MAX);
-        bool try16;
+        bool large;
         if (layout == D3Q19_IBM_LAYOUT_AUTO) {
             s->ibm_rows = peskin && (long long)s->np * s->ppp >= IBM_ROWS_MIN_MARKERS;
-            try16 = s->ibm_rows && blocks16 >= IBM_GROUP16_MIN_BLOCKS;
+            large = s->ibm_rows;
         } else {
             s->ibm_rows = peskin && layout != D3Q19_IBM_LAYOUT_WAVE;
-            try16 = s->ibm_rows && layout == D3Q19_IBM_LAYOUT_ROWS_G16;
+            large = s->ibm_rows && layout == D3Q19_IBM_LAYOUT_ROWS_LARGE;
         }
+        const int cap = shared_stream ? TILE_NODES_BATCH : TILE_NODES;
         s->group_markers = TILE_MARKERS;
-        if (!shared_stream && try16 && build_groups(TILE_MARKERS_MAX, perm, gbox) <= TILE_NODES)
-            s->group_markers = TILE_MARKERS_MAX;
-        else
-            (void)build_groups(TILE_MARKERS, perm, gbox);
+        bool chosen = false;
+        for (int gm = TILE_MARKERS_MAX; large && !shared_stream && gm > TILE_MARKERS && !chosen; gm /= 2) {
+            const long long blocks = (long long)s->np * ((s->ppp + gm - 1) / gm);
+            if (blocks < IBM_LARGE_GROUP_MIN_BLOCKS && layout == D3Q19_IBM_LAYOUT_AUTO) continue;
+            if (build_groups(gm, perm, gbox) <= cap) {
+                s->group_markers = gm;
+                chosen = true;
+            }
+        }
+        if (!chosen) (void)build_groups(TILE_MARKERS, perm, gbox);
         s->tile_nodes = 0;
         for (size_t k = 0; k < gbox.size(); k += 8)
-            if (gbox[k + 6] <= TILE_NODES) s->tile_nodes = std::max(s->tile_nodes, gbox[k + 6]);
+            if (gbox[k + 6] <= cap) s->tile_nodes = std::max(s->tile_nodes, gbox[k + 6]);
         const int need = 2 * ((int)std::ceil(rmax) + 3) + 4;
         s->tile_ok = true;
         const int nd[3] = {s->g.nx, s->g.ny, s->g.nz};

@@ -416,14 +416,15 @@ __device__ __forceinline__ void ibm_tile_group(const IbmArgs& a, const D3Q19_GLO
     if constexpr (rows_layout<MODE, ROWS>()) ibm_rows<R>(a, pos, vel, p, m, has, lane, sink);  // every lane: row sums
     else if (has) ibm_marker<R, MODE>(a, pos, vel, p, m, lane, sink);
     __syncthreads();
-    // box position of node i by multiply and shift: for i < 1024 and extents up to 1024,
-    // (i * ceil(2^20 / e)) >> 20 is i / e exactly (the excess stays below i / 2^20 < 1 / e)
-    static_assert(TILE_NODES <= 1024, "the box index split below is exact for boxes up to 1024 nodes");
-    const int mx = vol ? ((1 << 20) + ex - 1) / ex : 0, my = vol ? ((1 << 20) + ey - 1) / ey : 0;
+    // box position of node i by multiply and shift: for i < 2048 and extents up to 1024,
+    // (i * ceil(2^21 / e)) >> 21 is i / e exactly (the excess stays below i / 2^21 < 1 / e), and
+    // the product stays below 2^32
+    static_assert(TILE_NODES <= 2048 && TILE_NODES_BATCH <= 2048, "the box index split below is exact for boxes up to 2048 nodes");
+    const unsigned mx = vol ? ((1u << 21) + ex - 1) / ex : 0, my = vol ? ((1u << 21) + ey - 1) / ey : 0;
     for (int i = threadIdx.x; i < vol; i += blockDim.x) {
         const long long v0 = t0[i], v1 = t1[i], v2 = t2[i];
         if ((v0 | v1 | v2) == 0) continue;
-        const int ixy = (i * mx) >> 20, iz = (ixy * my) >> 20;
+        const int ixy = (int)(((unsigned)i * mx) >> 21), iz = (int)(((unsigned)ixy * my) >> 21);
         const int t3[3] = {i - ixy * ex, ixy - iz * ey, iz};
         int n3[3];
         for (int d = 0; d < 3; d++) {
@@ -468,11 +469,11 @@ __global__ void __launch_bounds__((group_threads<MODE, ROWS>())) ibm_tile_kernel
 // Batched, spreading staged in LDS: a persistent grid of one-group blocks walks the group list of
 // all cells, (sphere << 12) | first position of the group in the sphere's group order, and runs each
 // group as ibm_tile_group does for a single simulation (the same arithmetic: a cell's bits are those
-// of the cell alone).  The box takes TILE_NODES per component (every cell's capacity fits).  The
+// of the cell alone).  The box takes TILE_NODES_BATCH per component (every cell's capacity fits).  The
 // batch's epoch counter is committed by the first thread.
 template <class R, int MODE, bool ROWS>
 __global__ void __launch_bounds__((group_threads<MODE, ROWS>())) ibm_batch_tile_kernel(BatchTables t, int sp) {
-    extern __shared__ long long tile_mem[];  // 3 x TILE_NODES
+    extern __shared__ long long tile_mem[];  // 3 x TILE_NODES_BATCH
     const int lane = threadIdx.x & 63, w = (int)threadIdx.x / marker_threads<MODE, ROWS>();
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         const uint32_t e = t.epochs[0] + 1u;
@@ -492,7 +493,7 @@ __global__ void __launch_bounds__((group_threads<MODE, ROWS>())) ibm_batch_tile_
         const bool has = j < ppp;
         // x(t+1), v(t+1): written by this step's turn
         ibm_tile_group<R, MODE, ROWS>(ia, cd.posb[par ^ sp ^ 1], cd.velb[par ^ sp ^ 1], sph, j0 / TILE_MARKERS, has ? ia.perm[j] : 0, has,
-                                lane, tile_mem, TILE_NODES);
+                                lane, tile_mem, TILE_NODES_BATCH);
         __syncthreads();  // the next group reuses the tile
     }
 }
@@ -586,7 +587,7 @@ int launch_ibm(Sim* s, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1, int zlo, 
 
 template <class R, int MODE, bool ROWS>
 static void launch_ibm_batch_as(const BatchLaunch& L, int sp, hipStream_t st) {
-    const size_t lds = 3 * sizeof(long long) * (size_t)TILE_NODES;
+    const size_t lds = 3 * sizeof(long long) * (size_t)TILE_NODES_BATCH;
     hipLaunchKernelGGL((ibm_batch_tile_kernel<R, MODE, ROWS>), dim3(L.grid_ibm), dim3(marker_threads<MODE, ROWS>() * TILE_MARKERS), lds, st,
                        L.t, sp);
 }

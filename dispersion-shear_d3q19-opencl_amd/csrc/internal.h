@@ -36,9 +36,10 @@ constexpr int IBM_WAVES = 4;                          // markers (wavefronts) pe
 constexpr int TILE_MARKERS = 8;                       // markers per block of the LDS-staged spreading
 constexpr long long INFINITY_CACHE_BYTES = 256ll << 20;  // MI355X's last-level (memory-side) cache
 constexpr long long IBM_ROWS_MIN_MARKERS = 16384;        // markers from which four per wavefront pay
-constexpr int IBM_GROUP16_MIN_BLOCKS = 1024;             // blocks (groups) needed to take groups of 16
-constexpr int TILE_MARKERS_MAX = 16;                  // ... of a single simulation's Peskin-4 groups whose boxes fit
-constexpr int TILE_NODES = 1024;                      // LDS box of that spreading (nodes)
+constexpr int IBM_LARGE_GROUP_MIN_BLOCKS = 512;          // blocks needed to take groups larger than TILE_MARKERS
+constexpr int TILE_MARKERS_MAX = 64;                  // ... of a single simulation's Peskin-4 groups whose boxes fit
+constexpr int TILE_NODES = 2048;                      // LDS box of that spreading (nodes), single simulations
+constexpr int TILE_NODES_BATCH = 1024;                // ... cells of a batch (one-group blocks of 8 markers)
 constexpr float FP16_SCALE = 16384.0f;                // 2^14, relative-deviation scale of fp16 storage
 constexpr int MARK_BRICKS = 64;                       // bricks per axis a sphere's marking can list
 
@@ -220,8 +221,8 @@ struct IbmArgs {
     int np, ppp, mode;
     int mstride;  // marker visiting order: wavefront k of sphere p takes marker (k * mstride) mod ppp
     int tile_ok;  // spreading may be staged in LDS (every periodic extent exceeds twice the stencil reach)
-    int tile_nodes;  // LDS box capacity of this simulation's launches (the largest group box, at most TILE_NODES)
-    int gm;          // markers per group (TILE_MARKERS, or TILE_MARKERS_MAX for a single simulation's Peskin-4 groups)
+    int tile_nodes;  // LDS box capacity of this simulation's launches (the largest group box that fits)
+    int gm;          // markers per group (TILE_MARKERS; up to TILE_MARKERS_MAX for a single simulation's Peskin-4 rows)
     const D3Q19_GLOBAL int* perm;  // ppp marker indices in Morton order of their offsets
     const D3Q19_GLOBAL int* gbox;  // per group of TILE_MARKERS: LDS box relative to the centre node (lo[3], extent[3], volume, 0)
     double dA;
@@ -377,7 +378,7 @@ struct Sim {
     int* perm = nullptr;       // ppp marker indices in Morton order of their offsets (LDS-staged spreading)
     int* gbox = nullptr;       // the LDS box of each group of TILE_MARKERS markers (8 ints per group)
     bool tile_ok = false;      // every periodic extent exceeds twice the spheres' stencil reach
-    int tile_nodes = 0;        // the largest group box (nodes), capped at TILE_NODES: the LDS the IBM launches reserve
+    int tile_nodes = 0;        // the largest group box (nodes) within the LDS cap: the LDS the IBM launches reserve
     int group_markers = TILE_MARKERS;  // markers per group of the LDS-staged spreading
     bool ibm_rows = false;     // Peskin-4 markers run four to a wavefront (many markers); else one per wavefront
     bool nt_stores = false;    // population stores non-temporal (both population buffers exceed the Infinity Cache)

@@ -148,7 +148,7 @@ class Result(C.Structure):
 
 COLL_MRT, COLL_BGK = 0, 1
 IBM_LINEAR, IBM_PESKIN4 = 0, 1
-IBM_LAYOUT_AUTO, IBM_LAYOUT_WAVE, IBM_LAYOUT_ROWS, IBM_LAYOUT_ROWS_G16 = 0, 1, 2, 3
+IBM_LAYOUT_AUTO, IBM_LAYOUT_WAVE, IBM_LAYOUT_ROWS, IBM_LAYOUT_ROWS_LARGE = 0, 1, 2, 3
 STORE_FP32, STORE_FP16, STORE_BF16, STORE_FP64 = 0, 1, 2, 3
 CONTACT_ZONES, CONTACT_ALL_PAIRS = 0, 1
 ROUND_STOCHASTIC, ROUND_NEAREST = 0, 1

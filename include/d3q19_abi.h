@@ -95,14 +95,15 @@ typedef struct { float x, y, z, w; } d3q19_float4;
 
 #define D3Q19_IBM_LINEAR_SMOOTHED 0  /* trilinear + 27-point force smoothing, GPU_program.cl:81-256, 356-377 */
 #define D3Q19_IBM_PESKIN4         1  /* Peskin 4-point regularised delta for interpolation and spreading */
-/* ibm_layout: 0 automatic (four markers per wavefront from 16384 markers, groups of 16 markers
- * when the simulation has at least 1024 of them and their LDS boxes fit, else 8); 1 one marker per
- * wavefront, groups of 8; 2 four markers per wavefront (one per 16-lane row), groups of 8; 3 four
- * per wavefront, groups of 16 where the boxes fit.  The trilinear kernel always takes layout 1. */
+/* ibm_layout: 0 automatic (four markers per wavefront from 16384 markers, then the largest groups
+ * of up to 64 markers whose LDS boxes fit in 2048 nodes while 512 groups remain, else 8); 1 one
+ * marker per wavefront, groups of 8; 2 four markers per wavefront (one per 16-lane row), groups of
+ * 8; 3 four per wavefront, the largest groups (up to 64 markers) whose boxes fit.  The trilinear
+ * kernel always takes layout 1. */
 #define D3Q19_IBM_LAYOUT_AUTO 0
 #define D3Q19_IBM_LAYOUT_WAVE 1
 #define D3Q19_IBM_LAYOUT_ROWS 2
-#define D3Q19_IBM_LAYOUT_ROWS_G16 3
+#define D3Q19_IBM_LAYOUT_ROWS_LARGE 3
 
 #define D3Q19_STORE_FP32        0
 #define D3Q19_STORE_FP16_SHIFT  1    /* binary16 of (f_q - w_q)/w_q * 2^14, fp32 arithmetic */

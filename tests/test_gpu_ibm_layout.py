@@ -3,7 +3,7 @@
 d3q19_options.ibm_layout maps markers onto the hardware in one of three ways: one marker per
 wavefront (a lane per stencil node) in blocks of 8-marker groups, four markers per wavefront (a
 16-lane row each, a lane per stencil row of four nodes) in the same groups, or four per wavefront in
-16-marker groups.  Both layouts sum a marker's 64 weighted velocities with the same association,
+the largest groups (up to 64 markers) whose LDS boxes fit.  Both layouts sum a marker's 64 weighted velocities with the same association,
 and the spreading is integer, so populations, velocities, relaxation times and sphere states must
 be bitwise equal.  The LDS-staged kernel runs on a wall-bounded Casson box; the directly spreading
 kernel (a periodic extent too short for the LDS boxes) on a narrow periodic box.
@@ -16,7 +16,7 @@ from workloads import deck
 
 pytestmark = pytest.mark.gpu
 
-LAYOUTS = (d3q19.IBM_LAYOUT_WAVE, d3q19.IBM_LAYOUT_ROWS, d3q19.IBM_LAYOUT_ROWS_G16)
+LAYOUTS = (d3q19.IBM_LAYOUT_WAVE, d3q19.IBM_LAYOUT_ROWS, d3q19.IBM_LAYOUT_ROWS_LARGE)
 
 
 def casson_box(size, np_, walls=(0, 0, 1), storage=0):
