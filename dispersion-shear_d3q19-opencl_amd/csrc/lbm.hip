@@ -31,15 +31,16 @@ __device__ __forceinline__ uint16_t f32_to_bf16(float f) {
     return (uint16_t)(u >> 16);
 }
 
-// Storage formats.  st rounds to nearest; st_sr takes 32 random bits r and rounds stochastically
-// (unbiased: the result's expectation over uniform r is the stored value), which only the 16-bit
-// formats need.  NT: the store is non-temporal.  The collide's population stores are, on lattices
-// whose two population buffers exceed the 256 MB Infinity Cache (the kernels' NT argument, chosen by
-// CollideArgs::nt_stores; a runtime branch between the two forms was merged by the compiler): the next
-// read of a population is a whole sweep later, and the stores otherwise fill the L2 with dirty lines
-// that the IBM pass, launched right after the collide, has to write back first.  On the
-// 256x256x128 workload the fused step kernel went from 348 to 334 us (fp32) and the graph-replayed
-// step by 5%; on a cache-resident lattice (128^3 in fp16) they lost the cache residency: 10% slower.
+// Storage formats.  st rounds to nearest; st_sr takes B random bits r (the low bits of r; B = SR_BITS,
+// the bits the 16-bit format drops from a float32) and rounds stochastically (unbiased: the result's
+// expectation over uniform r is the stored value), which only the 16-bit formats need.  NT: the store
+// is non-temporal.  The collide's population stores are, on lattices whose two population buffers
+// exceed the 256 MB Infinity Cache (the kernels' NT argument, chosen by CollideArgs::nt_stores; a
+// runtime branch between the two forms was merged by the compiler): the next read of a population is a
+// whole sweep later, and the stores otherwise fill the L2 with dirty lines that the IBM pass, launched
+// right after the collide, has to write back first.  On the 256x256x128 workload the fused step kernel
+// went from 348 to 334 us (fp32) and the graph-replayed step by 5%; on a cache-resident lattice (128^3
+// in fp16) they lost the cache residency: 10% slower.
 template <bool NT, class T, class PT> __device__ __forceinline__ void st_pop(PT p, unsigned off, T v) {
     if constexpr (NT) __builtin_nontemporal_store(v, plane_at(p, off));
     else *plane_at(p, off) = v;
@@ -49,6 +50,7 @@ template <> struct Pop<D3Q19_STORE_FP32> {
     using T = float;
     using R = float;
     static constexpr bool SR = false;
+    static constexpr int SR_BITS = 0;
     template <int QI, class PT> __device__ static R ld(PT p, unsigned off) { return *plane_at(p, off); }
     template <int QI, class PT> __device__ static double ldd(PT p, unsigned off) { return (double)*plane_at(p, off); }
     template <int QI, bool NT = false, class PT> __device__ static void st(PT p, unsigned off, R v) { st_pop<NT>(p, off, v); }
@@ -58,28 +60,34 @@ template <> struct Pop<D3Q19_STORE_FP64> {
     using T = double;
     using R = double;
     static constexpr bool SR = false;
+    static constexpr int SR_BITS = 0;
     template <int QI, class PT> __device__ static R ld(PT p, unsigned off) { return *plane_at(p, off); }
     template <int QI, class PT> __device__ static double ldd(PT p, unsigned off) { return *plane_at(p, off); }
     template <int QI, bool NT = false, class PT> __device__ static void st(PT p, unsigned off, R v) { st_pop<NT>(p, off, v); }
     template <int QI, bool NT = false, class PT> __device__ static void st_sr(PT p, unsigned off, R v, uint32_t) { st_pop<NT>(p, off, v); }
 };
 // binary16 of the relative deviation (f_q - w_q)/w_q scaled by 2^14: deviations down to ~1e-8
-// stay normal numbers and the headroom reaches |dev| = 4.
+// stay normal numbers and the headroom reaches |dev| = 4.  (A power-of-two scale per population,
+// which would make the shift and scale one exact FMA, measured much less accurate over long runs:
+// relative viscosity 1.7e-3 from the fp64 build after 5000 steps of the 128^3 power-law workload,
+// against 1.6e-5 with this scale.)
 template <> struct Pop<D3Q19_STORE_FP16_SHIFT> {
     using T = _Float16;
     using R = float;
     template <int QI, class PT> __device__ static R ld(PT p, unsigned off) {
         constexpr float w = (float)wq(QI);
-        constexpr float sc = (float)(wq(QI) / 16384.0);
+        constexpr float sc = (float)(wq(QI) / (double)FP16_SCALE);
         return fmaf((float)*plane_at(p, off), sc, w);
     }
+    // decoded in double around the float32 w_q the encoding is relative to
     template <int QI, class PT> __device__ static double ldd(PT p, unsigned off) {
-        return wq(QI) + (double)*plane_at(p, off) * (wq(QI) / 16384.0);
+        return (double)(float)wq(QI) + (double)*plane_at(p, off) * (wq(QI) / (double)FP16_SCALE);
     }
     static constexpr bool SR = true;
+    static constexpr int SR_BITS = 13;
     template <int QI, bool NT = false, class PT> __device__ static void st(PT p, unsigned off, R v) {
         constexpr float w = (float)wq(QI);
-        constexpr float isc = (float)(16384.0 / wq(QI));
+        constexpr float isc = (float)((double)FP16_SCALE / wq(QI));
         st_pop<NT>(p, off, (_Float16)((v - w) * isc));
     }
     // the 13 magnitude bits that binary16 drops, plus 13 random bits, carry into the kept ones with
@@ -88,8 +96,8 @@ template <> struct Pop<D3Q19_STORE_FP16_SHIFT> {
     // collide measured 2% faster this way than with v_cvt_sr_f16_f32.
     template <int QI, bool NT = false, class PT> __device__ static void st_sr(PT p, unsigned off, R v, uint32_t r) {
         constexpr float w = (float)wq(QI);
-        constexpr float isc = (float)(16384.0 / wq(QI));
-        const uint32_t b = (__float_as_uint((v - w) * isc) + (r >> 19)) & 0xFFFFE000u;
+        constexpr float isc = (float)((double)FP16_SCALE / wq(QI));
+        const uint32_t b = (__float_as_uint((v - w) * isc) + r) & 0xFFFFE000u;
         st_pop<NT>(p, off, (_Float16)__uint_as_float(b));
     }
 };
@@ -102,9 +110,10 @@ template <> struct Pop<D3Q19_STORE_BF16_SHIFT> {
         return w + bf16_to_f32(*plane_at(p, off));
     }
     template <int QI, class PT> __device__ static double ldd(PT p, unsigned off) {
-        return wq(QI) + (double)bf16_to_f32(*plane_at(p, off));
+        return (double)(float)wq(QI) + (double)bf16_to_f32(*plane_at(p, off));
     }
     static constexpr bool SR = true;
+    static constexpr int SR_BITS = 16;
     template <int QI, bool NT = false, class PT> __device__ static void st(PT p, unsigned off, R v) {
         constexpr float w = (float)wq(QI);
         st_pop<NT>(p, off, f32_to_bf16(v - w));
@@ -113,7 +122,7 @@ template <> struct Pop<D3Q19_STORE_BF16_SHIFT> {
     // probability equal to the discarded fraction (finite inputs)
     template <int QI, bool NT = false, class PT> __device__ static void st_sr(PT p, unsigned off, R v, uint32_t r) {
         constexpr float w = (float)wq(QI);
-        st_pop<NT>(p, off, (uint16_t)((__float_as_uint(v - w) + (r >> 16)) >> 16));
+        st_pop<NT>(p, off, (uint16_t)((__float_as_uint(v - w) + r) >> 16));
     }
 };
 
@@ -128,15 +137,18 @@ __device__ __forceinline__ uint32_t fmix32(uint32_t h) {
     return h;
 }
 
-// Random word of population q for stochastic rounding at one node: seven products of the node's
-// hash h with fixed odd constants, each read through three rotations (11 bits apart) so that the
-// high bits that decide a rounding differ between the populations.
-template <int q> __device__ __forceinline__ uint32_t sr_word(uint32_t h) {
+// Random bits of population q for stochastic rounding at one node, B of them in the low bits: the
+// top B bits of one of seven products of the node's hash h with fixed odd constants, read through
+// three rotations (11 bits apart) so that the bits that decide a rounding differ between the
+// populations.  Each rotated read is one funnel shift (v_alignbit_b32) and a mask: written as shifts
+// of the product, the compiler turned every rotation into another quarter-rate multiply (19 per node
+// instead of 7).
+template <int q, int B> __device__ __forceinline__ uint32_t sr_bits(uint32_t h) {
     constexpr uint32_t K[7] = {0x9E3779B1u, 0x85EBCA77u, 0xC2B2AE3Du, 0x27D4EB2Fu, 0x165667B1u, 0xD3A2646Du, 0xFD7046C5u};
     const uint32_t x = h * K[q / 3];
-    constexpr int s = 11 * (q % 3);
-    if constexpr (s == 0) return x;
-    else return (x << s) | (x >> (32 - s));
+    // (rotate left by 11 (q mod 3)) >> (32 - B)  ==  (rotate right by (32 - B - 11 (q mod 3)) mod 32) & (2^B - 1)
+    constexpr unsigned sh = (unsigned)((32 - B - 11 * (q % 3) + 64) % 32);
+    return __builtin_amdgcn_alignbit(x, x, sh) & ((1u << B) - 1u);
 }
 
 // Arithmetic helpers: the float build uses the hardware reciprocal, square root, exp2 and log2
@@ -559,13 +571,17 @@ __device__ __forceinline__ void collide_node(const CollideArgs& a, const void* f
     if (P::SR && a.fl.sr) {
         // random bits from the node index and the bit patterns of the incoming populations: the
         // same for a cell alone and in a batch, and new whenever an input changes
-        uint32_t x = (uint32_t)n * 0x9E3779B1u;
-        Unroll<0, Q>::run([&](auto qc) __attribute__((always_inline)) { x ^= __float_as_uint((float)f[decltype(qc)::value]); });
+        // (three-input exclusive ors: gfx950's v_bitop3_b32 with truth table 0x96)
+        uint32_t x = ((uint32_t)n * 0x9E3779B1u) ^ __float_as_uint((float)f[0]);
+        Unroll<0, 9>::run([&](auto kc) __attribute__((always_inline)) {
+            constexpr int k = decltype(kc)::value;
+            x = __builtin_amdgcn_bitop3_b32(x, __float_as_uint((float)f[2 * k + 1]), __float_as_uint((float)f[2 * k + 2]), 0x96);
+        });
         const uint32_t h = fmix32(x);
         if constexpr (SBASE_STORES) asm volatile("" : "+v"(ns));
         Unroll<0, Q>::run([&](auto qc) __attribute__((always_inline)) {
             constexpr int q = decltype(qc)::value;
-            P::template st_sr<q, NT>(out + (long long)q * g.qstride, ns, fo[q], sr_word<q>(h));
+            P::template st_sr<q, NT>(out + (long long)q * g.qstride, ns, fo[q], sr_bits<q, P::SR_BITS>(h));
         });
     } else {
         if constexpr (SBASE_STORES) asm volatile("" : "+v"(ns));
