@@ -173,7 +173,7 @@ int sim_watchdog(Sim* s, int* nonfinite, int* moved) {
 }
 
 void sim_release(Sim* s) {
-    void* ptrs[] = {s->fA, s->fB, s->u, s->tau, s->facc, s->mask[0], s->mask[1], s->mask[2], s->bcraw, s->pos, s->vel,
+    void* ptrs[] = {s->fA, s->fB, s->u, s->tau, s->facc, s->ftag, s->mask[0], s->mask[1], s->mask[2], s->bcraw, s->pos, s->vel,
                     s->pos_alt, s->vel_alt, s->quat, s->omega, s->ffluid, s->tfluid, s->fcont, s->tcont, s->pts, s->mforce,
                     s->red, s->d_desc, s->d_obs, s->d_flag, s->zone, s->perm, s->gbox};
     for (void* p : ptrs)
@@ -442,6 +442,7 @@ static void alloc_list(Sim* s, std::vector<AllocItem>& v) {
     if (s->np > 0) {
         const size_t npd = (size_t)s->np;
         v.push_back(item(&s->facc, 3 * nint * sizeof(long long)));
+        v.push_back(item(&s->ftag, nint));
         for (int k = 0; k < 3; k++) v.push_back(item(&s->mask[k], (size_t)s->nbricks * sizeof(uint32_t)));
         v.push_back(item(&s->pos, 3 * npd * 8));
         v.push_back(item(&s->vel, 3 * npd * 8));
@@ -668,6 +669,7 @@ int sim_init(Sim* s, int device, const int_param_struct* ip, const flp_param_str
     if (e == hipSuccess) e = hipMemsetAsync(s->d_flag, 0, sizeof(int), s->stream);
     if (e == hipSuccess && s->np > 0) {
         e = hipMemsetAsync(s->facc, 0, 3 * nint * sizeof(long long), s->stream);
+        if (e == hipSuccess) e = hipMemsetAsync(s->ftag, 0, nint, s->stream);
         for (int k = 0; k < 3; k++)
             if (e == hipSuccess) e = hipMemsetAsync(s->mask[k], 0, s->nbricks * sizeof(uint32_t), s->stream);
         static const uint32_t ep0[2] = {0u, 0u};  // next step has epoch 1; no spread force pending

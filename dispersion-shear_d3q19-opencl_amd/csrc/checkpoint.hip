@@ -18,7 +18,8 @@ using namespace d3q19;
 namespace {
 
 constexpr char MAGIC[8] = {'D', '3', 'Q', '1', '9', 'C', 'K', '1'};
-constexpr uint32_t VERSION = 3;  // 2: u stored with its components interleaved per node; 3: displacement watchdog
+constexpr uint32_t VERSION = 4;  // 2: u stored with its components interleaved per node; 3: displacement watchdog;
+                                 // 4: per-node force tags
 
 struct Header {
     char magic[8];
@@ -44,7 +45,7 @@ struct Section {
 
 enum : uint32_t {
     S_F = 1, S_U, S_TAU, S_FACC, S_MASK0, S_MASK1, S_MASK2, S_EPOCHS, S_POS, S_VEL, S_QUAT, S_OMEGA, S_FFLUID,
-    S_TFLUID, S_FCONT, S_TCONT, S_MFORCE, S_ZONE, S_BCRAW, S_PTS, S_DFLAG
+    S_TFLUID, S_FCONT, S_TCONT, S_MFORCE, S_ZONE, S_BCRAW, S_PTS, S_DFLAG, S_FTAG
 };
 
 struct Array {
@@ -71,6 +72,7 @@ std::vector<Array> state_arrays(Sim* s) {
     if (s->np > 0) {
         const size_t np = (size_t)s->np;
         v.push_back({S_FACC, s->facc, 3 * nint * sizeof(long long), false});
+        v.push_back({S_FTAG, s->ftag, nint, false});
         v.push_back({S_MASK0, s->mask[0], (size_t)s->nbricks * sizeof(uint32_t), false});
         v.push_back({S_MASK1, s->mask[1], (size_t)s->nbricks * sizeof(uint32_t), false});
         v.push_back({S_MASK2, s->mask[2], (size_t)s->nbricks * sizeof(uint32_t), false});

@@ -55,13 +55,20 @@ __device__ __forceinline__ long long round_to_ll(double x) {
 __device__ __forceinline__ long long spread_value(double w, double Fc) { return round_to_ll((-w * Fc) * FIX_SCALE); }
 
 // ... added with 64-bit integer atomics (order independent)
-__device__ __forceinline__ void spread_add(D3Q19_GLOBAL long long* facc, long long nint, int node, double w, const double* F) {
+// (a node that receives a nonzero contribution gets its force tag; every writer stores the same
+// value, so concurrent stores agree)
+__device__ __forceinline__ void spread_add(D3Q19_GLOBAL long long* facc, D3Q19_GLOBAL uint8_t* ftag, long long nint, int node, double w,
+                                           const double* F) {
+    bool any = false;
     for (int c = 0; c < 3; c++) {
         const long long iv = spread_value(w, F[c]);
-        if (iv != 0)
+        if (iv != 0) {
             __hip_atomic_fetch_add((D3Q19_GLOBAL unsigned long long*)(facc + c * nint + node), (unsigned long long)iv,
                                    __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            any = true;
+        }
     }
+    if (any) ftag[node] = 1;
 }
 
 // v of the lane that the DPP control CTRL names, within a row of 16 lanes
@@ -329,7 +336,7 @@ __global__ void __launch_bounds__(64 * IBM_WAVES) ibm_marker_kernel(IbmArgs a) {
         const int m = (int)(((long long)k * a.mstride) % a.ppp);
         const long long nint = a.g.nint;
         ibm_rows<R>(a, a.pos, a.vel, p, m, has, lane, [&](const int* sn, double w, const double* F) {
-            spread_add(a.facc, nint, sn[0] + a.g.nx * (sn[1] + a.g.ny * sn[2]), w, F);
+            spread_add(a.facc, a.ftag, nint, sn[0] + a.g.nx * (sn[1] + a.g.ny * sn[2]), w, F);
         });
         return;
     }
@@ -349,7 +356,7 @@ __global__ void __launch_bounds__(64 * IBM_WAVES) ibm_marker_kernel(IbmArgs a) {
     const int m = (int)(((long long)k * a.mstride) % a.ppp);
     const long long nint = a.g.nint;
     ibm_marker<R, MODE>(a, a.pos, a.vel, p, m, lane, [&](const int* sn, double w, const double* F) {
-        spread_add(a.facc, nint, sn[0] + a.g.nx * (sn[1] + a.g.ny * sn[2]), w, F);
+        spread_add(a.facc, a.ftag, nint, sn[0] + a.g.nx * (sn[1] + a.g.ny * sn[2]), w, F);
     });
 }
 
@@ -410,7 +417,7 @@ __device__ __forceinline__ void ibm_tile_group(const IbmArgs& a, const D3Q19_GLO
                 if (iv != 0) atomicAdd((unsigned long long*)&tc[c][t], (unsigned long long)iv);
             }
         } else {
-            spread_add(a.facc, g.nint, sn[0] + g.nx * (sn[1] + g.ny * sn[2]), w, F);
+            spread_add(a.facc, a.ftag, g.nint, sn[0] + g.nx * (sn[1] + g.ny * sn[2]), w, F);
         }
     };
     if constexpr (rows_layout<MODE, ROWS>()) ibm_rows<R>(a, pos, vel, p, m, has, lane, sink);  // every lane: row sums
@@ -438,6 +445,7 @@ __device__ __forceinline__ void ibm_tile_group(const IbmArgs& a, const D3Q19_GLO
             if (vs[c] != 0)
                 __hip_atomic_fetch_add((D3Q19_GLOBAL unsigned long long*)(a.facc + c * (long long)g.nint + node), (unsigned long long)vs[c],
                                        __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        a.ftag[node] = 1;
     }
 }
 
@@ -532,6 +540,7 @@ IbmArgs make_ibm_args(Sim* s) {
     a.pts = gp(s->pts);
     a.u = gp(s->u);
     a.facc = gp(s->facc);
+    a.ftag = gp(s->ftag);
     a.mforce = gp(s->mforce);
     a.epochs = gp(s->epochs);
     a.zreach = s->mark_rout;

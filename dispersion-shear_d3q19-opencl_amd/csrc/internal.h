@@ -9,11 +9,12 @@
 //   u, tau       R[3*nint], R[nint]           (R = float, or double for the fp64 build)
 //   force        int64[3*nint] fixed point (2^-40 units), spread with integer atomics so the
 //                sum is independent of order; consumed and zeroed by the next collide.
+//   force tags   uint8[nint]: 1 where the spreading added a nonzero contribution; the next collide
+//                reads the force only where its node's tag is set, and clears both.
 //   masks        uint32 epoch stamp per 4x4x4 brick, three arrays by epoch mod 3: a brick is
 //                marked for the step whose epoch it holds (no per-step clearing).  The step with
-//                epoch e = epochs[0] + 1 writes u in the bricks marked e, consumes the pending
-//                spread force (epoch epochs[1]) in the bricks marked epochs[1], and its spreading
-//                kernel commits epochs[0] = epochs[1] = e.  In the fused step the spheres' turn,
+//                epoch e = epochs[0] + 1 writes u in the bricks marked e, and its spreading kernel
+//                commits epochs[0] = epochs[1] = e.  In the fused step the spheres' turn,
 //                running beside the collide, marks epoch e + 1 (array (e+1) mod 3, which no other
 //                block of the launch reads) for its new positions dilated by one lattice unit.
 //   particles    double SoA; positions and velocities double-buffered (the fused turn reads
@@ -197,6 +198,7 @@ struct CollideArgs {
     void* u;
     void* tau;
     long long* facc;
+    uint8_t* ftag;         // per node: 1 where the last IBM pass spread force (cleared by the collide that consumes it)
     const uint32_t* mask0;
     const uint32_t* mask1;
     const uint32_t* mask2;
@@ -232,6 +234,7 @@ struct IbmArgs {
     const D3Q19_GLOBAL float* pts;
     const D3Q19_GLOBAL void* u;
     D3Q19_GLOBAL long long* facc;
+    D3Q19_GLOBAL uint8_t* ftag;  // set to 1 at every node that receives a nonzero contribution
     D3Q19_GLOBAL double* mforce;
     D3Q19_GLOBAL uint32_t* epochs;
     // slab-owned passes: this launch runs the spheres whose stencils reach at most up to a z plane
@@ -358,6 +361,7 @@ struct Sim {
     void* u = nullptr;    // R[3*nint]
     void* tau = nullptr;  // R[nint]
     long long* facc = nullptr;
+    uint8_t* ftag = nullptr;
     uint32_t* mask[3] = {nullptr, nullptr, nullptr};  // brick epoch stamps, by epoch mod 3
     uint32_t* epochs = nullptr;              // [current epoch, epoch of the pending spread force]
     int nbricks = 0;
