@@ -216,7 +216,7 @@ def ensemble(args):
 
 def serving(args):
     return serving_run(args.requests, args.max_edge, args.check_interval, args.max_steps, args.load, args.rate,
-                       args.serve_storage, args.arena_gb)
+                       args.serve_storage, args.arena_gb, min_edge=args.min_edge)
 
 
 def serving_summary(args):
@@ -232,7 +232,7 @@ def serving_summary(args):
                       "serving runs the stream to convergence)"}
 
 
-def serving_run(requests, max_edge, check_interval, max_steps, load, rate_arg, serve_storage, arena_gb, quiet=False):
+def serving_run(requests, max_edge, check_interval, max_steps, load, rate_arg, serve_storage, arena_gb, quiet=False, min_edge=32):
     """Shear-cell solve requests as sequential plain fp32 solves (single simulations, one after another),
     then the same requests arriving as a Poisson stream into the continuous-batching server.  The
     arrival rate is `load` times the sequential capacity measured first (offered load above what the
@@ -240,7 +240,7 @@ def serving_run(requests, max_edge, check_interval, max_steps, load, rate_arg, s
     the sequential latencies are those of a FIFO single server with the measured service times under
     the same arrivals."""
     import d3q19
-    decks = c5_requests(count=requests, edge_range=(32, max_edge), storage=0)
+    decks = c5_requests(count=requests, edge_range=(min_edge, max_edge), storage=0)
     cases = [d3q19.Case(lines=d) for d in decks]
     sp = d3q19.solve_params(tolerance=1e-4, check_interval=check_interval, min_steps=min(10 * check_interval, max_steps),
                             max_steps=max_steps)
@@ -283,6 +283,21 @@ def serving_run(requests, max_edge, check_interval, max_steps, load, rate_arg, s
     agree = max(abs(done[i].obs.wall_stress - seq[i].obs.wall_stress) / max(abs(seq[i].obs.wall_stress), 1e-300)
                 for i in range(len(cases)))
     steps_equal = sum(1 for i in range(len(cases)) if done[i].steps == seq[i].steps)
+    # Work-conservation bound of any scheduler on this request set at the same arithmetic: the
+    # makespan is at least the total lattice updates over the best rate one solve reached alone
+    # (the lattice bandwidth of the GPU), and at least the longest solve alone.
+    nodes = [(c.N[0] - 2) * (c.N[1] - 2) * (c.N[2] - 2) for c in cases]
+    work = [nodes[i] * seq[i].steps for i in range(len(cases))]
+    rate_best = max(work[i] / service[i] for i in range(len(cases)))
+    bound_s = max(sum(work) / rate_best, max(service))
+    ceiling = {"node_steps": int(sum(work)), "best_single_solve_glups": round(rate_best / 1e9, 3),
+               "makespan_bound_s": round(bound_s, 3), "longest_solve_alone_s": round(max(service), 3),
+               "solves_per_s_bound": round(len(cases) / bound_s, 4),
+               "vs_sequential_bound": round(sum(service) / bound_s, 3),
+               "work_share_edges_ge_96": round(sum(w for w, c in zip(work, cases) if c.N[0] - 2 >= 96) / max(sum(work), 1), 3)}
+    per_request = [{"edge": cases[i].N[0] - 2, "spheres": cases[i].np, "steps": seq[i].steps,
+                    "sequential_s": round(service[i], 4), "glups_alone": round(work[i] / service[i] / 1e9, 3),
+                    "server_latency_s": round(lat[i], 4)} for i in range(len(cases))]
     names = {0: "fp32", 1: "fp16_shift", 2: "bf16_shift", 3: "fp64"}
     return {"requests": len(cases), "arrival_rate_per_s": round(rate, 4), "offered_load_vs_sequential": round(rate / cap_seq, 3),
             "edges": [c.N[0] - 2 for c in cases], "spheres": [c.np for c in cases], "check_interval": check_interval,
@@ -293,7 +308,8 @@ def serving_run(requests, max_edge, check_interval, max_steps, load, rate_arg, s
             "sequential": {"solves_per_s": round(cap_seq, 4), "p50_latency_s": round(pct(lat_seq, 50), 4),
                            "p99_latency_s": round(pct(lat_seq, 99), 4), "busy_s": round(sum(service), 3)},
             "speedup_solves_per_s": round((len(cases) / span) / cap_seq, 3),
-            "max_rel_diff_wall_stress_vs_sequential": agree, "same_step_count": steps_equal}
+            "max_rel_diff_wall_stress_vs_sequential": agree, "same_step_count": steps_equal,
+            "ceiling": ceiling, "per_request": per_request}
 
 
 def large(args):
@@ -412,6 +428,7 @@ def main():
     ap.add_argument("--rate", type=float, default=0.0, help="arrivals per second (0: load x sequential capacity)")
     ap.add_argument("--load", type=float, default=4.0)
     ap.add_argument("--serve-storage", type=int, default=0)
+    ap.add_argument("--min-edge", type=int, default=32)
     ap.add_argument("--max-edge", type=int, default=160)
     ap.add_argument("--check-interval", type=int, default=100)
     ap.add_argument("--max-steps", type=int, default=40000)

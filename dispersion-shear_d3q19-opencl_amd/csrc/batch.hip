@@ -73,6 +73,7 @@ struct Batch {
     size_t cap_mark = 0;
     BatchCounts* d_cnt = nullptr;
     double* d_obs = nullptr;
+    int* d_wd = nullptr;  // watchdog words, 2 per cell slot
     // host mirrors, kept alive until the next rebuild (which first synchronises the stream)
     std::vector<CellDesc> h_cells;
     std::vector<uint2> h_tiles;
@@ -89,6 +90,9 @@ struct Batch {
     hipGraphExec_t graph2 = nullptr;
     bool graph_valid = false;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // the spheres' turn runs on `aux` beside the collide: fork before both, join before the IBM pass
+    hipStream_t aux = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
 };
 
 static BatchLaunch launch_of(const Batch* b) {
@@ -251,10 +255,22 @@ static int upload_tables(Batch* b) {
 
 // One fused step of every cell with step parity sp: the spheres' turn beside the collide tiles,
 // then the IBM pass.
+// One batched step.  The spheres' turn and the collide are independent, as in the single
+// simulation's fused step (the turn writes the next epoch's marks and the other position and
+// velocity buffers; the collide reads this epoch's marks and the pending force), so the turn runs on
+// a second stream beside the collide (in a captured graph: two parallel branches) and the IBM pass,
+// which needs both, waits for the turn.  The next step's fork follows the IBM pass, whose marker
+// forces the turn sums.  On a serving mix of 64 small cells the turn took 22 us per step as a launch
+// ahead of the collide.
 static int enqueue_batch_step(Batch* b, int sp, int write_u_all, int do_post, hipStream_t st) {
     const BatchLaunch L = launch_of(b);
     int rc;
-    if ((rc = launch_step_batch(L, sp, write_u_all, do_post, st))) return rc;
+    HIP_OK(hipEventRecord(b->fork, st));
+    HIP_OK(hipStreamWaitEvent(b->aux, b->fork, 0));
+    if ((rc = launch_turn_batch(L, sp, do_post, b->aux))) return rc;
+    HIP_OK(hipEventRecord(b->join, b->aux));
+    if ((rc = launch_collide_batch(L, sp, write_u_all, 0, st))) return rc;
+    HIP_OK(hipStreamWaitEvent(st, b->join, 0));
     return launch_ibm_batch(L, sp, st);
 }
 
@@ -409,6 +425,32 @@ static int batch_observables(Batch* b, const int* ids, int n, d3q19_observables*
     return 0;
 }
 
+// watchdog words of cells ids[0..n) on their current lattices: one launch for all cell slots, one
+// copy, one synchronisation
+static int batch_watchdog(Batch* b, const int* ids, int n, int* nonfinite, int* moved) {
+    int rc;
+    if (b->dirty && (rc = upload_tables(b))) return rc;
+    const int nslots = active_slots(b);
+    if (nslots == 0) return 0;
+    long long maxn = 1;
+    for (int c = 0; c < nslots; c++)
+        if (b->cells[c].sim) maxn = std::max(maxn, (long long)b->cells[c].sim->g.nint);
+    // blocks per cell: about 16 k nodes per block for the largest cell, 1 to 256
+    const int bpc = (int)std::min<long long>(256, std::max<long long>(1, (maxn + 16383) / 16384));
+    HIP_OK(hipMemsetAsync(b->d_wd, 0, (size_t)2 * nslots * sizeof(int), b->stream));
+    if ((rc = launch_watchdog(b->d_cells, nslots, b->bo.storage, b->phase, bpc, b->d_wd, b->stream))) return rc;
+    std::vector<int> w((size_t)2 * nslots);
+    HIP_OK(hipMemcpyAsync(w.data(), b->d_wd, w.size() * sizeof(int), hipMemcpyDeviceToHost, b->stream));
+    HIP_OK(hipStreamSynchronize(b->stream));
+    for (int i = 0; i < n; i++) {
+        const int c = ids[i];
+        if (c < 0 || c >= nslots || !b->cells[c].sim) return bfail(D3Q19_E_INVALID, "no such cell");
+        nonfinite[i] = w[2 * (size_t)c];
+        moved[i] = w[2 * (size_t)c + 1];
+    }
+    return 0;
+}
+
 static int batch_add(Batch* b, const int_param_struct* ip, const flp_param_struct* fp, const d3q19_options* opt,
                      const d3q19_float4* pts, const d3q19_float4* kin, int* id) {
     if (!ip || !fp || !id) return bfail(D3Q19_E_INVALID, "null argument");
@@ -492,12 +534,18 @@ static void batch_free(Batch* b) {
             delete m.sim;
         }
     b->cells.clear();
-    void* ptrs[] = {b->arena.base, b->epochs, b->d_cells, b->d_tiles, b->d_sph, b->d_msph, b->d_cnt, b->d_obs};
+    void* ptrs[] = {b->arena.base, b->epochs, b->d_cells, b->d_tiles, b->d_sph, b->d_msph, b->d_cnt, b->d_obs, b->d_wd};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (b->graph2) (void)hipGraphExecDestroy(b->graph2);
     if (b->ev0) (void)hipEventDestroy(b->ev0);
     if (b->ev1) (void)hipEventDestroy(b->ev1);
+    if (b->fork) (void)hipEventDestroy(b->fork);
+    if (b->join) (void)hipEventDestroy(b->join);
+    if (b->aux) {
+        (void)hipStreamSynchronize(b->aux);
+        (void)hipStreamDestroy(b->aux);
+    }
     if (b->stream) (void)hipStreamDestroy(b->stream);
 }
 
@@ -538,8 +586,12 @@ static int batch_create(int device, const d3q19_batch_options* bo, Batch** out) 
     if (e == hipSuccess) e = hipMalloc((void**)&b->d_cells, (size_t)b->max_cells * sizeof(CellDesc));
     if (e == hipSuccess) e = hipMalloc((void**)&b->d_cnt, sizeof(BatchCounts));
     if (e == hipSuccess) e = hipMalloc((void**)&b->d_obs, (size_t)OBS_STRIDE * b->max_cells * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc((void**)&b->d_wd, (size_t)2 * b->max_cells * sizeof(int));
     if (e == hipSuccess) e = hipEventCreate(&b->ev0);
     if (e == hipSuccess) e = hipEventCreate(&b->ev1);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->aux, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->fork, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->join, hipEventDisableTiming);
     if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
     if (e != hipSuccess) {
         const int code = (e == hipErrorOutOfMemory) ? D3Q19_E_NOMEM : D3Q19_E_DEVICE;
@@ -649,14 +701,9 @@ static int server_progress(Server* v, d3q19_result* done, int cap, int* n_done) 
     for (Request* r : v->resident) ids.push_back(r->cell);
     std::vector<d3q19_observables> obs(ids.size());
     if ((rc = batch_observables(b, ids.data(), (int)ids.size(), obs.data()))) return rc;
-    // watchdog of every resident cell on its current lattice: passes enqueued, one synchronisation
+    // watchdog of every resident cell on its current lattice (one launch for all)
     std::vector<int> bad(ids.size(), 0), moved(ids.size(), 0);
-    for (size_t i = 0; i < ids.size(); i++) {
-        Member& m = b->cells[ids[i]];
-        sync_view(m, b->phase);
-        if ((rc = sim_watchdog_enqueue(m.sim, &bad[i], &moved[i]))) return rc;
-    }
-    HIP_OK(hipStreamSynchronize(b->stream));
+    if ((rc = batch_watchdog(b, ids.data(), (int)ids.size(), bad.data(), moved.data()))) return rc;
     std::vector<Request*> keep;
     for (size_t i = 0; i < v->resident.size(); i++) {
         Request* r = v->resident[i];

@@ -474,6 +474,10 @@ int launch_fill_tau(Sim* s);
 // plane, node count, same for the upper plane, sum over spheres of |fluid force|, spheres, 0, 0]
 constexpr int OBS_STRIDE = 8;
 int launch_observables(const CellDesc* cells, int ncells, int storage, int sp, double* out, hipStream_t st);
+// watchdog of every cell slot in one launch: out[2c] |= 1 when cell c's current lattice holds a
+// non-finite or non-positive density, out[2c+1] = its sphere displacement flag (out zeroed first);
+// empty slots (nint 0) are skipped; bpc blocks scan each cell
+int launch_watchdog(const CellDesc* cells, int ncells, int storage, int sp, int bpc, int* out, hipStream_t st);
 
 // batched (persistent) kernels over the work lists of BatchTables; grid sizes are fixed per batch
 struct BatchLaunch {
@@ -490,7 +494,7 @@ struct BatchLaunch {
 };
 int launch_collide_batch(const BatchLaunch& L, int sp, int write_u_all, int probe, hipStream_t st);
 // a batched step's sphere turn (its own launch) and collide
-int launch_step_batch(const BatchLaunch& L, int sp, int write_u_all, int do_post, hipStream_t st);
+int launch_turn_batch(const BatchLaunch& L, int sp, int do_post, hipStream_t st);
 int launch_ibm_batch(const BatchLaunch& L, int sp, hipStream_t st);
 int launch_sphere_prime_batch(const BatchLaunch& L, int sp, hipStream_t st);
 int launch_sphere_post_batch(const BatchLaunch& L, int sp, hipStream_t st);

@@ -46,8 +46,9 @@ __global__ void __launch_bounds__(256) sphere_prime_batch_kernel(BatchTables t, 
 }
 
 // The spheres' turn of a batched step (as step_kernel's sphere blocks).  In the batch it is a launch
-// of its own before the collide: read from the descriptor table rather than from kernel arguments,
-// the sphere code would take registers that the batched collide needs for occupancy.
+// of its own, on a second stream beside the collide: read from the descriptor table rather than
+// from kernel arguments, the sphere code would take registers that the batched collide needs for
+// occupancy.
 __global__ void __launch_bounds__(256) sphere_turn_batch_kernel(BatchTables t, int sp, int do_post) {
     const int nsph = t.cnt->nsph;
     const uint32_t e = t.epochs[0] + 2u;
@@ -156,10 +157,10 @@ int launch_sphere_prime_batch(const BatchLaunch& L, int sp, hipStream_t st) {
     return 0;
 }
 
-int launch_step_batch(const BatchLaunch& L, int sp, int write_u_all, int do_post, hipStream_t st) {
+int launch_turn_batch(const BatchLaunch& L, int sp, int do_post, hipStream_t st) {
     hipLaunchKernelGGL(sphere_turn_batch_kernel, dim3(L.grid_sphere), dim3(256), 0, st, L.t, sp, do_post);
     HIP_OK(hipGetLastError());
-    return launch_collide_batch(L, sp, write_u_all, 0, st);
+    return 0;
 }
 
 int launch_sphere_post_batch(const BatchLaunch& L, int sp, hipStream_t st) {

@@ -268,3 +268,29 @@ def test_displacement_watchdog_ends_requests_with_an_error():
     print("server: statuses %s" % {k: (v.status, v.steps) for k, v in done.items()}, flush=True)
     assert done[ids[1]].status == d3q19.E_STATE and done[ids[1]].steps == 20
     assert done[ids[0]].status in (0, 1)
+
+
+def test_nonfinite_density_ends_requests_with_an_error():
+    """A request whose lattice turns non-finite (a wall speed that is not a number: wall speeds are
+    not validated, as in the reference) ends with D3Q19_E_NONFINITE at the first check in the
+    sequential solve and in the server, whose one-launch watchdog scans every resident cell; a calm
+    request beside it in the server is unaffected."""
+    wild = case(size=16, walls=(0, 0, 1), tau=0.8, uw=float("nan"), coll=1, ibm=1)
+    sp = d3q19.solve_params(tolerance=1e-4, check_interval=20, min_steps=0, max_steps=2000)
+    r = d3q19.solve(wild, sp)
+    print("solve: status %d after %d steps" % (r.status, r.steps), flush=True)
+    assert r.status == d3q19.E_NONFINITE and r.steps == 20
+    calm = shear_cell(16, np_=1, diam=6.0, seed=3)
+    srv = d3q19.Server(storage=0, arena_bytes=1 << 28)
+    ids = [srv.submit(calm, sp), srv.submit(wild, sp)]
+    done = {}
+    for _ in range(200):
+        for res in srv.progress():
+            done[res.id] = res
+        if len(done) == 2:
+            break
+    srv.close()
+    print("server: statuses %s" % {k: (v.status, v.steps) for k, v in done.items()}, flush=True)
+    assert done[ids[1]].status == d3q19.E_NONFINITE and done[ids[1]].steps == r.steps
+    seq_calm = d3q19.solve(calm, sp)
+    assert done[ids[0]].status == seq_calm.status and done[ids[0]].steps == seq_calm.steps
