@@ -379,12 +379,16 @@ def stream_sweep(args):
 COPY_ELEMS = 1 << 28  # float32 elements of the calibration copy (1 GiB read + 1 GiB written)
 
 
+PROBE_NODES = 256 * 256 * 128
+
+
 def traffic_probe():
     """Launches for the HBM-traffic counter passes (rocprofv3 --pmc FETCH_SIZE, then WRITE_SIZE):
     5 timed real steps of the 256x256x128 workload in fp32 and in fp16-shifted storage (after 10
     warm-up steps), then a plain elementwise copy of known size (torch.neg over 1 GiB of float32)
     whose counter readings calibrate the counters (expected 1 GiB fetched and 1 GiB written per
-    launch)."""
+    launch), then the pull-pattern bandwidth probe on the same box (19 planes read and written per
+    launch, 642 MB each way), a calibration with the step kernel's own access pattern."""
     import d3q19
     import torch
     for storage in (0, 1):
@@ -403,6 +407,12 @@ def traffic_probe():
         torch.neg(src, out=dst)
     torch.cuda.synchronize()
     print("probe: copy of %d bytes read + %d bytes written, 3 launches" % (4 * COPY_ELEMS, 4 * COPY_ELEMS), flush=True)
+    # the attainable-bandwidth probe with the collide's own access pattern (19 fp32 planes read at
+    # the pull offsets, 19 written, 256x256x128 box): 2 warm-up and 3 timed launches of known bytes,
+    # the calibration whose request mix is the step kernel's
+    d3q19.stream_probe(256, 256, 128, streams=19, width=1, pull=True, launches=3)
+    print("probe: pull-pattern probe, %d bytes read + %d bytes written per launch, 5 launches" % (19 * 4 * PROBE_NODES, 19 * 4 * PROBE_NODES),
+          flush=True)
 
 
 def main():
