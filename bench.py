@@ -216,7 +216,7 @@ def ensemble(args):
 
 def serving(args):
     return serving_run(args.requests, args.max_edge, args.check_interval, args.max_steps, args.load, args.rate,
-                       args.serve_storage, args.arena_gb, min_edge=args.min_edge)
+                       args.serve_storage, args.arena_gb, min_edge=args.min_edge, sequential=not args.no_sequential)
 
 
 def serving_summary(args):
@@ -232,7 +232,26 @@ def serving_summary(args):
                       "serving runs the stream to convergence)"}
 
 
-def serving_run(requests, max_edge, check_interval, max_steps, load, rate_arg, serve_storage, arena_gb, quiet=False, min_edge=32):
+def server_backlog(cases, sp, serve_storage, arena_gb):
+    """The server alone on a backlog (every request submitted at once): makespan, solves/s and the
+    aggregate lattice-update rate (for profiling the server's kernels without the sequential solves)."""
+    import d3q19
+    srv = d3q19.Server(storage=serve_storage, collision=d3q19.COLL_BGK, ibm=d3q19.IBM_PESKIN4, arena_bytes=arena_gb << 30)
+    t0 = d3q19.clock()
+    ids = [srv.submit(c, sp) for c in cases]
+    done = {}
+    while len(done) < len(cases):
+        for r in srv.progress():
+            done[r.id] = r
+    span = d3q19.clock() - t0
+    srv.close()
+    work = sum((c.N[0] - 2) * (c.N[1] - 2) * (c.N[2] - 2) * done[i].steps for c, i in zip(cases, ids))
+    return {"requests": len(cases), "server_only": True, "makespan_s": round(span, 3), "solves_per_s": round(len(cases) / span, 4),
+            "glups": round(work / span / 1e9, 3), "converged": sum(1 for r in done.values() if r.status == 0)}
+
+
+def serving_run(requests, max_edge, check_interval, max_steps, load, rate_arg, serve_storage, arena_gb, quiet=False, min_edge=32,
+                sequential=True):
     """Shear-cell solve requests as sequential plain fp32 solves (single simulations, one after another),
     then the same requests arriving as a Poisson stream into the continuous-batching server.  The
     arrival rate is `load` times the sequential capacity measured first (offered load above what the
@@ -244,6 +263,8 @@ def serving_run(requests, max_edge, check_interval, max_steps, load, rate_arg, s
     cases = [d3q19.Case(lines=d) for d in decks]
     sp = d3q19.solve_params(tolerance=1e-4, check_interval=check_interval, min_steps=min(10 * check_interval, max_steps),
                             max_steps=max_steps)
+    if not sequential:
+        return server_backlog(cases, sp, serve_storage, arena_gb)
     service, seq = [], {}
     for i, c in enumerate(cases):
         a = time.perf_counter()
@@ -439,6 +460,8 @@ def main():
     ap.add_argument("--load", type=float, default=4.0)
     ap.add_argument("--serve-storage", type=int, default=0)
     ap.add_argument("--min-edge", type=int, default=32)
+    ap.add_argument("--no-sequential", action="store_true",
+                    help="serving: the server alone on a backlog of the requests (no sequential solves)")
     ap.add_argument("--max-edge", type=int, default=160)
     ap.add_argument("--check-interval", type=int, default=100)
     ap.add_argument("--max-steps", type=int, default=40000)
