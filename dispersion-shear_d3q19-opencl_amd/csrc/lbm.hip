@@ -615,7 +615,8 @@ __global__ void __launch_bounds__(256) collide_kernel(CollideArgs a) {
 // the last step when do_post, the dynamics into the other position buffer, the dilated marks of
 // the epoch after this one); the other blocks are the collide tiles of collide_kernel.  Nothing a
 // sphere block writes is read by a collide block of the same launch: the collide reads the marks of
-// this epoch and of the pending force, the turn writes the marks of the next epoch (a third array).
+// this epoch and the pending force with its tags (written by the last IBM pass), the turn writes the
+// marks of the next epoch (another array) and the other position and velocity buffers.
 template <int ST, int COLL, int VT, bool NT>
 __device__ __forceinline__ void step_body(const CollideArgs& a, const PartArgs& p, int nsph, int do_post) {
     if ((int)blockIdx.x < nsph) {
