@@ -258,13 +258,14 @@ CellDesc make_cell_desc(Sim* s) {
     memset(&d, 0, sizeof(d));
     d.c = make_collide_args(s, nullptr, nullptr, 0, 1);
     d.i = make_ibm_args(s);
-    d.p = make_part_args(s);
+    d.p[0] = make_part_args(s);
     d.f[0] = s->fA;
     d.f[1] = s->fB;
     d.posb[0] = gp(s->pos);
     d.posb[1] = gp(s->pos_alt);
     d.velb[0] = gp(s->vel);
     d.velb[1] = gp(s->vel_alt);
+    set_cell_spheres(d);
     d.par = s->parity;
     d.nint = s->g.nint;
     d.nx = s->g.nx;

@@ -90,9 +90,6 @@ struct Batch {
     hipGraphExec_t graph2 = nullptr;
     bool graph_valid = false;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    // the spheres' turn runs on `aux` beside the collide: fork before both, join before the IBM pass
-    hipStream_t aux = nullptr;
-    hipEvent_t fork = nullptr, join = nullptr;
 };
 
 static BatchLaunch launch_of(const Batch* b) {
@@ -182,6 +179,7 @@ static int upload_tables(Batch* b) {
             d.posb[k] = gp(m.pb[k]);
             d.velb[k] = gp(m.vb[k]);
         }
+        set_cell_spheres(d);
         b->h_cells[c] = d;
     }
     // tiles: the constant-tau cells first, then the per-node-tau cells
@@ -253,24 +251,17 @@ static int upload_tables(Batch* b) {
     return 0;
 }
 
-// One fused step of every cell with step parity sp: the spheres' turn beside the collide tiles,
-// then the IBM pass.
-// One batched step.  The spheres' turn and the collide are independent, as in the single
-// simulation's fused step (the turn writes the next epoch's marks and the other position and
-// velocity buffers; the collide reads this epoch's marks and the pending force), so the turn runs on
-// a second stream beside the collide (in a captured graph: two parallel branches) and the IBM pass,
-// which needs both, waits for the turn.  The next step's fork follows the IBM pass, whose marker
-// forces the turn sums.  On a serving mix of 64 small cells the turn took 22 us per step as a launch
-// ahead of the collide.
+// One batched step of every cell with step parity sp: one launch whose leading blocks run the
+// spheres' turn while the others collide (independent, as in the single simulation's fused step:
+// the turn writes the next epoch's marks and the other position and velocity buffers, the collide
+// reads this epoch's marks and the pending force), then the IBM pass.  The turn as a launch of its
+// own on a second stream beside the collide (two graph branches) cost the fork and join: on one
+// 127x127x128 cell with 63 spheres 115.0 us per step against 101.2 us fused; on the 64-cell fp16
+// ensemble 0.660 against 0.649 ms.
 static int enqueue_batch_step(Batch* b, int sp, int write_u_all, int do_post, hipStream_t st) {
     const BatchLaunch L = launch_of(b);
     int rc;
-    HIP_OK(hipEventRecord(b->fork, st));
-    HIP_OK(hipStreamWaitEvent(b->aux, b->fork, 0));
-    if ((rc = launch_turn_batch(L, sp, do_post, b->aux))) return rc;
-    HIP_OK(hipEventRecord(b->join, b->aux));
-    if ((rc = launch_collide_batch(L, sp, write_u_all, 0, st))) return rc;
-    HIP_OK(hipStreamWaitEvent(st, b->join, 0));
+    if ((rc = launch_step_batch(L, sp, write_u_all, do_post, st))) return rc;
     return launch_ibm_batch(L, sp, st);
 }
 
@@ -540,12 +531,6 @@ static void batch_free(Batch* b) {
     if (b->graph2) (void)hipGraphExecDestroy(b->graph2);
     if (b->ev0) (void)hipEventDestroy(b->ev0);
     if (b->ev1) (void)hipEventDestroy(b->ev1);
-    if (b->fork) (void)hipEventDestroy(b->fork);
-    if (b->join) (void)hipEventDestroy(b->join);
-    if (b->aux) {
-        (void)hipStreamSynchronize(b->aux);
-        (void)hipStreamDestroy(b->aux);
-    }
     if (b->stream) (void)hipStreamDestroy(b->stream);
 }
 
@@ -589,9 +574,6 @@ static int batch_create(int device, const d3q19_batch_options* bo, Batch** out) 
     if (e == hipSuccess) e = hipMalloc((void**)&b->d_wd, (size_t)2 * b->max_cells * sizeof(int));
     if (e == hipSuccess) e = hipEventCreate(&b->ev0);
     if (e == hipSuccess) e = hipEventCreate(&b->ev1);
-    if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->aux, hipStreamNonBlocking);
-    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->fork, hipEventDisableTiming);
-    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->join, hipEventDisableTiming);
     if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
     if (e != hipSuccess) {
         const int code = (e == hipErrorOutOfMemory) ? D3Q19_E_NOMEM : D3Q19_E_DEVICE;

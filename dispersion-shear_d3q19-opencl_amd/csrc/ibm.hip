@@ -478,9 +478,13 @@ __global__ void __launch_bounds__((group_threads<MODE, ROWS>())) ibm_tile_kernel
 // all cells, (sphere << 12) | first position of the group in the sphere's group order, and runs each
 // group as ibm_tile_group does for a single simulation (the same arithmetic: a cell's bits are those
 // of the cell alone).  The box takes TILE_NODES_BATCH per component (every cell's capacity fits).  The
-// batch's epoch counter is committed by the first thread.
+// batch's epoch counter is committed by the first thread.  The tables come in as restrict
+// arguments (nothing here writes them): the descriptor fields are scalar loads that the force
+// stores do not invalidate.
 template <class R, int MODE, bool ROWS>
-__global__ void __launch_bounds__((group_threads<MODE, ROWS>())) ibm_batch_tile_kernel(BatchTables t, int sp) {
+__global__ void __launch_bounds__((group_threads<MODE, ROWS>())) ibm_batch_tile_kernel(BatchTables t, const CellDesc* __restrict__ cells,
+                                                                                      const int4* __restrict__ sphs,
+                                                                                      const int* __restrict__ msph, int sp) {
     extern __shared__ long long tile_mem[];  // 3 x TILE_NODES_BATCH
     const int lane = threadIdx.x & 63, w = (int)threadIdx.x / marker_threads<MODE, ROWS>();
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -490,11 +494,11 @@ __global__ void __launch_bounds__((group_threads<MODE, ROWS>())) ibm_batch_tile_
     }
     const int ngrp = t.cnt->nmark;
     for (int k = blockIdx.x; k < ngrp; k += gridDim.x) {  // uniform over the block
-        const int v = __builtin_amdgcn_readfirstlane(t.msph[k]);
-        const int4 d = t.sph[v >> 12];  // cell, local sphere, first global marker, markers per sphere
+        const int v = __builtin_amdgcn_readfirstlane(msph[k]);
+        const int4 d = sphs[v >> 12];  // cell, local sphere, first global marker, markers per sphere
         const int cell = __builtin_amdgcn_readfirstlane(d.x), sph = __builtin_amdgcn_readfirstlane(d.y);
         const int ppp = __builtin_amdgcn_readfirstlane(d.w);
-        const CellDesc& cd = t.cells[cell];
+        const CellDesc& cd = cells[cell];
         const IbmArgs& ia = cd.i;
         const int par = __builtin_amdgcn_readfirstlane(cd.par);
         const int j0 = v & 4095, j = j0 + w;
@@ -598,7 +602,7 @@ template <class R, int MODE, bool ROWS>
 static void launch_ibm_batch_as(const BatchLaunch& L, int sp, hipStream_t st) {
     const size_t lds = 3 * sizeof(long long) * (size_t)TILE_NODES_BATCH;
     hipLaunchKernelGGL((ibm_batch_tile_kernel<R, MODE, ROWS>), dim3(L.grid_ibm), dim3(marker_threads<MODE, ROWS>() * TILE_MARKERS), lds, st,
-                       L.t, sp);
+                       L.t, L.t.cells, L.t.sph, L.t.msph, sp);
 }
 
 int launch_ibm_batch(const BatchLaunch& L, int sp, hipStream_t st) {

@@ -301,7 +301,10 @@ struct ObsArgs {
 struct CellDesc {
     CollideArgs c;
     IbmArgs i;
-    PartArgs p;
+    // the sphere arguments for either buffer of x(t), v(t): p[k] reads posb[k] / velb[k] and its
+    // dynamics writes posb[k ^ 1] / velb[k ^ 1].  Kept whole in the table (read through a restrict
+    // kernel argument, so with scalar loads) rather than patched into a kernel's own copy.
+    PartArgs p[2];
     ObsArgs o;
     const void* f[2];
     D3Q19_GLOBAL double* posb[2];  // sphere positions / velocities: [par ^ sp] is x(t) before a step of parity sp
@@ -311,6 +314,17 @@ struct CellDesc {
     int nx, nxy;
     float inv_nx, inv_nxy;  // flat node index -> (x, y, z) by float reciprocal + exact correction
 };
+
+// p[1] from p[0] and both buffer pairs (host, after posb / velb are set)
+inline void set_cell_spheres(CellDesc& d) {
+    d.p[1] = d.p[0];
+    for (int k = 0; k < 2; k++) {
+        d.p[k].pos = d.posb[k];
+        d.p[k].vel = d.velb[k];
+        d.p[k].pos_out = d.posb[k ^ 1];
+        d.p[k].vel_out = d.velb[k ^ 1];
+    }
+}
 
 // Work-list sizes of a batch, read by the persistent kernels from device memory so that a captured
 // graph stays valid while cells join and leave.
@@ -493,8 +507,8 @@ struct BatchLaunch {
     int nt_stores;         // non-temporal population stores (members' lattices beyond the Infinity Cache)
 };
 int launch_collide_batch(const BatchLaunch& L, int sp, int write_u_all, int probe, hipStream_t st);
-// a batched step's sphere turn (its own launch) and collide
-int launch_turn_batch(const BatchLaunch& L, int sp, int do_post, hipStream_t st);
+// a batched step's collide with the spheres' turn in the leading blocks of its first launch
+int launch_step_batch(const BatchLaunch& L, int sp, int write_u_all, int do_post, hipStream_t st);
 int launch_ibm_batch(const BatchLaunch& L, int sp, hipStream_t st);
 int launch_sphere_prime_batch(const BatchLaunch& L, int sp, hipStream_t st);
 int launch_sphere_post_batch(const BatchLaunch& L, int sp, hipStream_t st);

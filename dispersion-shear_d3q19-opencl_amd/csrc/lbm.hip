@@ -695,6 +695,42 @@ __global__ void __launch_bounds__(256) collide_batch_kernel(BatchTables t, const
     collide_batch_tiles<ST, COLL, VT, NT>(t, cells, sp, write_u_all, probe, blockIdx.x, gridDim.x);
 }
 
+// Batched step: blocks [0, nturn) walk the sphere list of all cells and run each sphere's turn
+// (as step_body's sphere blocks: x(t), v(t) from p[par ^ sp], x(t+1), v(t+1) into the other
+// buffer, the marks of the epoch after this one); the other blocks walk the collide tiles.  As in
+// step_body, nothing a sphere block writes is read by a collide block of the same launch.
+template <int ST, int COLL, int VT, bool NT>
+__device__ __forceinline__ void step_batch_body(const BatchTables& t, const CellDesc* __restrict__ cells, const int4* __restrict__ sph,
+                                                int sp, int write_u_all, int nturn, int do_post) {
+    if ((int)blockIdx.x < nturn) {
+        const int nsph = t.cnt->nsph;
+        const uint32_t e = t.epochs[0] + 2u;
+        for (int k = blockIdx.x; k < nsph; k += nturn) {
+            const int4 d = sph[k];
+            const CellDesc& cd = cells[d.x];
+            sphere_turn(cd.p[cd.par ^ sp], d.y, do_post, e);
+            __syncthreads();
+        }
+        return;
+    }
+    collide_batch_tiles<ST, COLL, VT, NT>(t, cells, sp, write_u_all, 0, (int)blockIdx.x - nturn, (int)gridDim.x - nturn);
+}
+
+template <int ST, int COLL, int VT, bool NT>
+__global__ void __launch_bounds__(256) step_batch_kernel(BatchTables t, const CellDesc* __restrict__ cells, const int4* __restrict__ sph,
+                                                         int sp, int write_u_all, int nturn, int do_post) {
+    step_batch_body<ST, COLL, VT, NT>(t, cells, sph, sp, write_u_all, nturn, do_post);
+}
+
+// the same with the register budget of 6 waves per SIMD (80 VGPRs), which the batched BGK collide
+// in fp32 / fp16 / bf16 storage and the sphere turn meet without spilling
+template <int ST, int COLL, int VT, bool NT>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6)))
+step_batch_kernel_w6(BatchTables t, const CellDesc* __restrict__ cells, const int4* __restrict__ sph, int sp, int write_u_all, int nturn,
+                     int do_post) {
+    step_batch_body<ST, COLL, VT, NT>(t, cells, sph, sp, write_u_all, nturn, do_post);
+}
+
 // With events ev0/ev1, they are recorded at the kernel's own start and end (hipExtLaunchKernelGGL),
 // so their interval is the kernel's duration without the dispatch gap before it.
 template <int ST, int COLL, int VT>
@@ -853,6 +889,47 @@ template <int ST>
 static void collide_batch_st(const BatchLaunch& L, int sp, int write_u_all, int probe, hipStream_t st) {
     if (L.collision == D3Q19_COLL_BGK) collide_batch_coll<ST, D3Q19_COLL_BGK>(L, sp, write_u_all, probe, st);
     else collide_batch_coll<ST, D3Q19_COLL_MRT>(L, sp, write_u_all, probe, st);
+}
+
+// the step's collide launches with the sphere blocks in the first of them
+template <int ST, int COLL, int VT, bool NT>
+static void step_batch_one(const BatchLaunch& L, int grid, int nturn, int sp, int write_u_all, int do_post, hipStream_t st) {
+    const dim3 g(grid + nturn), blk(L.tile);
+    if constexpr (COLL == D3Q19_COLL_BGK && ST != D3Q19_STORE_FP64)
+        hipLaunchKernelGGL((step_batch_kernel_w6<ST, COLL, VT, NT>), g, blk, 0, st, L.t, L.t.cells, L.t.sph, sp, write_u_all, nturn, do_post);
+    else
+        hipLaunchKernelGGL((step_batch_kernel<ST, COLL, VT, NT>), g, blk, 0, st, L.t, L.t.cells, L.t.sph, sp, write_u_all, nturn, do_post);
+}
+
+template <int ST, int COLL, bool NT>
+static void step_batch_nt(const BatchLaunch& L, int sp, int write_u_all, int do_post, hipStream_t st) {
+    const int nturn = L.grid_sphere;
+    if (L.grid_collide > 0 || L.grid_collide_vt == 0) step_batch_one<ST, COLL, 0, NT>(L, L.grid_collide, nturn, sp, write_u_all, do_post, st);
+    if (L.grid_collide_vt > 0)
+        step_batch_one<ST, COLL, 1, NT>(L, L.grid_collide_vt, L.grid_collide > 0 ? 0 : nturn, sp, write_u_all, do_post, st);
+}
+
+template <int ST>
+static void step_batch_st(const BatchLaunch& L, int sp, int write_u_all, int do_post, hipStream_t st) {
+    const bool bgk = L.collision == D3Q19_COLL_BGK;
+    if (L.nt_stores) {
+        if (bgk) step_batch_nt<ST, D3Q19_COLL_BGK, true>(L, sp, write_u_all, do_post, st);
+        else step_batch_nt<ST, D3Q19_COLL_MRT, true>(L, sp, write_u_all, do_post, st);
+    } else {
+        if (bgk) step_batch_nt<ST, D3Q19_COLL_BGK, false>(L, sp, write_u_all, do_post, st);
+        else step_batch_nt<ST, D3Q19_COLL_MRT, false>(L, sp, write_u_all, do_post, st);
+    }
+}
+
+int launch_step_batch(const BatchLaunch& L, int sp, int write_u_all, int do_post, hipStream_t st) {
+    switch (L.storage) {
+    case D3Q19_STORE_FP16_SHIFT: step_batch_st<D3Q19_STORE_FP16_SHIFT>(L, sp, write_u_all, do_post, st); break;
+    case D3Q19_STORE_BF16_SHIFT: step_batch_st<D3Q19_STORE_BF16_SHIFT>(L, sp, write_u_all, do_post, st); break;
+    case D3Q19_STORE_FP64: step_batch_st<D3Q19_STORE_FP64>(L, sp, write_u_all, do_post, st); break;
+    default: step_batch_st<D3Q19_STORE_FP32>(L, sp, write_u_all, do_post, st); break;
+    }
+    HIP_OK(hipGetLastError());
+    return 0;
 }
 
 int launch_collide_batch(const BatchLaunch& L, int sp, int write_u_all, int probe, hipStream_t st) {
@@ -1229,7 +1306,7 @@ __global__ void __launch_bounds__(256) observables_kernel(const CellDesc* __rest
             }
         }
     } else {
-        const PartArgs& p = cd.p;
+        const PartArgs& p = cd.p[0];
         for (int k = threadIdx.x; k < p.np; k += blockDim.x) {
             const auto F = p.ffluid + 3 * k;
             v0 += sqrt(F[0] * F[0] + F[1] * F[1] + F[2] * F[2]);
@@ -1287,7 +1364,7 @@ __global__ void __launch_bounds__(256) watchdog_kernel(const CellDesc* __restric
         if (!(r > 0.0) || !isfinite(r) || !isfinite(jx) || !isfinite(jy) || !isfinite(jz)) bad = 1;
     }
     if (bad) atomicOr(&out[2 * cell], 1);
-    if (part == 0 && threadIdx.x == 0 && cd.p.np > 0 && cd.p.flag) out[2 * cell + 1] = *cd.p.flag;
+    if (part == 0 && threadIdx.x == 0 && cd.p[0].np > 0 && cd.p[0].flag) out[2 * cell + 1] = *cd.p[0].flag;
 }
 
 int launch_watchdog(const CellDesc* cells, int ncells, int storage, int sp, int bpc, int* out, hipStream_t st) {

@@ -27,50 +27,27 @@ __global__ void __launch_bounds__(256) sphere_prime_kernel(PartArgs a) {
 }
 
 // Batched: a persistent grid of blocks walks the sphere list of all cells.  sp is the batch's step
-// parity: a cell's current positions are posb[par ^ sp].
-__device__ __forceinline__ PartArgs batch_part_args(const CellDesc& cd, int sp) {
-    PartArgs p = cd.p;
-    p.pos = p.pos_out = cd.posb[cd.par ^ sp];
-    p.vel = p.vel_out = cd.velb[cd.par ^ sp];
-    return p;
-}
-
-__global__ void __launch_bounds__(256) sphere_prime_batch_kernel(BatchTables t, int sp) {
+// parity: a cell's current positions are posb[par ^ sp], and p[par ^ sp] are its sphere arguments
+// for them.  The tables come in as restrict arguments: nothing in these kernels writes them, so the
+// descriptor fields are scalar loads that stores to the sphere arrays do not invalidate.
+__global__ void __launch_bounds__(256) sphere_prime_batch_kernel(BatchTables t, const CellDesc* __restrict__ cells,
+                                                                 const int4* __restrict__ sph, int sp) {
     const int nsph = t.cnt->nsph;
     for (int k = blockIdx.x; k < nsph; k += gridDim.x) {
-        const int4 d = t.sph[k];
-        const PartArgs p = batch_part_args(t.cells[d.x], sp);
-        sphere_prime(p, d.y, t.epochs[0] + 1u);
+        const int4 d = sph[k];
+        const CellDesc& cd = cells[d.x];
+        sphere_prime(cd.p[cd.par ^ sp], d.y, t.epochs[0] + 1u);
         __syncthreads();
     }
 }
 
-// The spheres' turn of a batched step (as step_kernel's sphere blocks).  In the batch it is a launch
-// of its own, on a second stream beside the collide: read from the descriptor table rather than
-// from kernel arguments, the sphere code would take registers that the batched collide needs for
-// occupancy.
-__global__ void __launch_bounds__(256) sphere_turn_batch_kernel(BatchTables t, int sp, int do_post) {
-    const int nsph = t.cnt->nsph;
-    const uint32_t e = t.epochs[0] + 2u;
-    for (int k = blockIdx.x; k < nsph; k += gridDim.x) {
-        const int4 d = t.sph[k];
-        const CellDesc& cd = t.cells[d.x];
-        PartArgs p = cd.p;
-        p.pos = cd.posb[cd.par ^ sp];
-        p.vel = cd.velb[cd.par ^ sp];
-        p.pos_out = cd.posb[cd.par ^ sp ^ 1];
-        p.vel_out = cd.velb[cd.par ^ sp ^ 1];
-            sphere_turn(p, d.y, do_post, e);
-        __syncthreads();
-    }
-}
-
-__global__ void __launch_bounds__(256) sphere_post_batch_kernel(BatchTables t, int sp) {
+__global__ void __launch_bounds__(256) sphere_post_batch_kernel(BatchTables t, const CellDesc* __restrict__ cells,
+                                                                const int4* __restrict__ sph, int sp) {
     const int nsph = t.cnt->nsph;
     for (int k = blockIdx.x; k < nsph; k += gridDim.x) {
-        const int4 d = t.sph[k];
-        const PartArgs p = batch_part_args(t.cells[d.x], sp);
-        sphere_post(p, d.y, 1, 1);
+        const int4 d = sph[k];
+        const CellDesc& cd = cells[d.x];
+        sphere_post(cd.p[cd.par ^ sp], d.y, 1, 1);
         __syncthreads();
     }
 }
@@ -152,19 +129,13 @@ int launch_sphere_prime(Sim* s, hipStream_t st) {
 }
 
 int launch_sphere_prime_batch(const BatchLaunch& L, int sp, hipStream_t st) {
-    hipLaunchKernelGGL(sphere_prime_batch_kernel, dim3(L.grid_sphere), dim3(256), 0, st, L.t, sp);
-    HIP_OK(hipGetLastError());
-    return 0;
-}
-
-int launch_turn_batch(const BatchLaunch& L, int sp, int do_post, hipStream_t st) {
-    hipLaunchKernelGGL(sphere_turn_batch_kernel, dim3(L.grid_sphere), dim3(256), 0, st, L.t, sp, do_post);
+    hipLaunchKernelGGL(sphere_prime_batch_kernel, dim3(L.grid_sphere), dim3(256), 0, st, L.t, L.t.cells, L.t.sph, sp);
     HIP_OK(hipGetLastError());
     return 0;
 }
 
 int launch_sphere_post_batch(const BatchLaunch& L, int sp, hipStream_t st) {
-    hipLaunchKernelGGL(sphere_post_batch_kernel, dim3(L.grid_sphere), dim3(256), 0, st, L.t, sp);
+    hipLaunchKernelGGL(sphere_post_batch_kernel, dim3(L.grid_sphere), dim3(256), 0, st, L.t, L.t.cells, L.t.sph, sp);
     HIP_OK(hipGetLastError());
     return 0;
 }
