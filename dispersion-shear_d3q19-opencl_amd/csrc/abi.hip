@@ -48,7 +48,9 @@ static int validate(const int_param_struct* ip, const flp_param_struct* fp, cons
         return fail(D3Q19_E_INVALID, "more than one pair of velocity-boundary faces");
     const long long nint = (long long)(ip->LatticeSize[0] - 2) * (ip->LatticeSize[1] - 2) * (ip->LatticeSize[2] - 2);
     if (nint >= (1LL << 31)) return fail(D3Q19_E_UNSUPPORTED, "more than 2^31 interior nodes");
-    // kernels address a population plane by a 32-bit byte offset (no options: fp32 storage)
+    // per-node arrays and planar population planes are addressed by 32-bit byte offsets (no
+    // options: fp32 storage); chunked populations by a 32-bit lane offset within one z-plane (Q x
+    // nx x ny values, below 4 GiB for any extents up to 4096)
     if (nint * (long long)pop_size(opt ? opt->storage : 0) >= (1LL << 32))
         return fail(D3Q19_E_UNSUPPORTED, "a population plane above 4 GiB (interior nodes x bytes per value)");
     if (!(fp->NewtonianTau > 0.5f)) return fail(D3Q19_E_INVALID, "NewtonianTau must exceed 0.5");
@@ -93,7 +95,19 @@ static void geom_from(Sim* s) {
     g.ny = s->Ny - 2;
     g.nz = s->Nz - 2;
     g.nint = g.nx * g.ny * g.nz;
-    g.qstride = g.nint;
+    // rows per population chunk (pop_index): the smallest power of two R dividing ny with R * nx a
+    // multiple of 64 nodes; failing that, the largest R dividing ny with R * nx below 128
+    g.lr = -1;
+    for (int k = 0; k <= 6 && g.ny % (1 << k) == 0; k++)
+        if ((g.nx << k) % 64 == 0) {
+            g.lr = k;
+            break;
+        }
+    if (g.lr < 0) {
+        g.lr = 0;
+        for (int k = 1; k <= 6 && g.ny % (1 << k) == 0; k++)
+            if ((g.nx << k) < 128) g.lr = k;
+    }
     g.wall = -1;
     for (int d = 0; d < 3; d++) {
         g.per[d] = s->ip.BoundaryConds[d] == 0 ? 1 : 0;
@@ -507,6 +521,7 @@ int sim_init(Sim* s, int device, const int_param_struct* ip, const flp_param_str
     s->device = device;
     s->arena = arena;
     sim_setup(s, ip, fp, opt);
+    if (shared_stream) s->g.lr = -1;  // batch members: planar populations (pop_index)
     const size_t nint = (size_t)s->g.nint;
     const size_t rs = real_size(s->storage);
     s->nbricks = s->g.bnx * s->g.bny * s->g.bnz;

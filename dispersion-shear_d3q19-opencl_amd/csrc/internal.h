@@ -3,9 +3,14 @@
 // Device layout (interior nodes only; the reference's one-node buffer shell is not stored):
 //   node n = x + nx*(y + ny*z), 0 <= x < nx = LatticeSize[0]-2, etc.  Lattice index i_1D of the
 //   reference = (x+1) + Nx*((y+1) + Ny*(z+1)).
-//   populations  f[q*qstride + n], two lattices A/B (pull ping-pong).  The state between steps
-//                is the post-collision field; a step gathers f[q](n - c_q) (periodic wrap,
-//                moving-wall unknowns rebuilt from the knowns), collides and writes at n.
+//   populations  two lattices A/B (pull ping-pong).  Single simulations: chunk-interleaved
+//                (pop_index): chunks of R rows (R * nx a multiple of 64 nodes where ny allows,
+//                R = 1 for nx = 256), each holding its 19 populations as 19 consecutive runs of
+//                R * nx values, so a wavefront's loads and stores touch one block of 19 runs
+//                instead of 38 planes a lattice apart.  Batch members: planar, f[q*nint + n].
+//                The state between steps is the post-collision field;
+//                a step gathers f[q](n - c_q) (periodic wrap, moving-wall unknowns rebuilt from
+//                the knowns), collides and writes at n.
 //   u, tau       R[3*nint], R[nint]           (R = float, or double for the fp64 build)
 //   force        int64[3*nint] fixed point (2^-40 units), spread with integer atomics so the
 //                sum is independent of order; consumed and zeroed by the next collide.
@@ -140,11 +145,31 @@ __host__ __device__ constexpr int mnorm(int k) {
 struct Geom {
     int nx, ny, nz;       // interior extents
     int nint;             // nx*ny*nz
-    long long qstride;    // population stride
+    int lr;               // log2 of the rows per population chunk, -1 planar (pop_index)
     int per[3];           // 1 periodic, 0 wall
     int wall;             // wall axis or -1
     int bnx, bny, bnz;    // brick grid
     int sys[3];           // SystemSize
+};
+
+// Element index of population q at node (x, y, z).  Chunked (single simulations, lr >= 0): each
+// z-plane is cut into chunks of R = 2^lr consecutive rows (R divides ny); a chunk holds its Q
+// populations as Q consecutive runs of R * nx values (row y % R of the chunk at offset
+// (y % R) * nx in each run).  R is the smallest that makes a run a multiple of 64 nodes (one
+// wavefront) where ny allows.  The collide tiles are row segments, so a wavefront's access is one
+// stretch of a run and a chunk's 19 runs lie together.  Planar (batch members, lr = -1): plane q
+// holds population q of every node; batched tiles are runs of consecutive nodes of any row
+// length, which a planar field keeps contiguous.
+__host__ __device__ inline long long pop_index(const Geom& g, int q, int x, int y, int z) {
+    if (g.lr < 0) return (long long)q * g.nint + x + (long long)g.nx * (y + (long long)g.ny * z);
+    const long long run = (long long)g.nx << g.lr;
+    return (long long)Q * g.nx * g.ny * z + (long long)Q * run * (y >> g.lr) + (long long)q * run + (long long)(y & ((1 << g.lr) - 1)) * g.nx + x;
+}
+// the layout a kernel reads and how it forms population addresses
+enum : int {
+    ADDR_CHUNKED = 0,  // chunked, z uniform over the wavefront: the plane in each population's scalar base
+    ADDR_PLANAR = 1,   // planar: a 32-bit lane offset within each population's plane
+    ADDR_ANY = 2,      // the lattice's own layout (g.lr), any z per lane (cold kernels)
 };
 
 struct FluidParams {
