@@ -89,6 +89,20 @@ static int validate(const int_param_struct* ip, const flp_param_struct* fp, cons
     return 0;
 }
 
+// population layouts (pop_index): chunks of 2^lr rows, or planes
+static void set_chunked(Geom& g, int lr) {
+    g.lr = lr;
+    g.qs = (long long)g.nx << lr;
+    g.cs = (int)(Q * g.qs);
+    g.ps = (long long)Q * g.nx * g.ny;
+}
+static void set_planar(Geom& g) {
+    g.lr = PLANAR_LR;
+    g.qs = g.nint;
+    g.cs = 0;
+    g.ps = (long long)g.nx * g.ny;
+}
+
 static void geom_from(Sim* s) {
     Geom& g = s->g;
     g.nx = s->Nx - 2;
@@ -97,17 +111,18 @@ static void geom_from(Sim* s) {
     g.nint = g.nx * g.ny * g.nz;
     // rows per population chunk (pop_index): the smallest power of two R dividing ny with R * nx a
     // multiple of 64 nodes; failing that, the largest R dividing ny with R * nx below 128
-    g.lr = -1;
+    int lr = -1;
     for (int k = 0; k <= 6 && g.ny % (1 << k) == 0; k++)
         if ((g.nx << k) % 64 == 0) {
-            g.lr = k;
+            lr = k;
             break;
         }
-    if (g.lr < 0) {
-        g.lr = 0;
+    if (lr < 0) {
+        lr = 0;
         for (int k = 1; k <= 6 && g.ny % (1 << k) == 0; k++)
-            if ((g.nx << k) < 128) g.lr = k;
+            if ((g.nx << k) < 128) lr = k;
     }
+    set_chunked(g, lr);
     g.wall = -1;
     for (int d = 0; d < 3; d++) {
         g.per[d] = s->ip.BoundaryConds[d] == 0 ? 1 : 0;
@@ -521,7 +536,11 @@ int sim_init(Sim* s, int device, const int_param_struct* ip, const flp_param_str
     s->device = device;
     s->arena = arena;
     sim_setup(s, ip, fp, opt);
-    if (shared_stream) s->g.lr = -1;  // batch members: planar populations (pop_index)
+    // Batch members: the batched collide walks runs of consecutive nodes, so a member keeps the
+    // chunks only where each run is a whole number of wavefronts (and its populations stay within
+    // the 32-bit lane offsets of that kernel); otherwise planar populations.
+    if (shared_stream && (((s->g.nx << s->g.lr) % 64) != 0 || (long long)Q * s->g.nint * (long long)pop_size(s->storage) >= (1LL << 32)))
+        set_planar(s->g);
     const size_t nint = (size_t)s->g.nint;
     const size_t rs = real_size(s->storage);
     s->nbricks = s->g.bnx * s->g.bny * s->g.bnz;

@@ -3,12 +3,12 @@
 // Device layout (interior nodes only; the reference's one-node buffer shell is not stored):
 //   node n = x + nx*(y + ny*z), 0 <= x < nx = LatticeSize[0]-2, etc.  Lattice index i_1D of the
 //   reference = (x+1) + Nx*((y+1) + Ny*(z+1)).
-//   populations  two lattices A/B (pull ping-pong).  Single simulations: chunk-interleaved
-//                (pop_index): chunks of R rows (R * nx a multiple of 64 nodes where ny allows,
-//                R = 1 for nx = 256), each holding its 19 populations as 19 consecutive runs of
-//                R * nx values, so a wavefront's loads and stores touch one block of 19 runs
-//                instead of 38 planes a lattice apart.  Batch members: planar, f[q*nint + n].
-//                The state between steps is the post-collision field;
+//   populations  two lattices A/B (pull ping-pong).  Chunk-interleaved (pop_index): chunks of R
+//                rows (R * nx a multiple of 64 nodes where ny allows, R = 1 for nx = 256), each
+//                holding its 19 populations as 19 consecutive runs of R * nx values, so a
+//                wavefront's loads and stores touch one block of 19 runs instead of 38 planes a
+//                lattice apart.  Batch members whose runs are not whole wavefronts: planar,
+//                f[q*nint + n].  The state between steps is the post-collision field;
 //                a step gathers f[q](n - c_q) (periodic wrap, moving-wall unknowns rebuilt from
 //                the knowns), collides and writes at n.
 //   u, tau       R[3*nint], R[nint]           (R = float, or double for the fp64 build)
@@ -145,31 +145,36 @@ __host__ __device__ constexpr int mnorm(int k) {
 struct Geom {
     int nx, ny, nz;       // interior extents
     int nint;             // nx*ny*nz
-    int lr;               // log2 of the rows per population chunk, -1 planar (pop_index)
+    // population layout (pop_index): log2 of the rows per chunk, and the element strides from one
+    // population to the next at a node, from one z-plane to the next and from one chunk of rows
+    // to the next within a plane
+    int lr;
+    int cs;
+    long long qs, ps;
     int per[3];           // 1 periodic, 0 wall
     int wall;             // wall axis or -1
     int bnx, bny, bnz;    // brick grid
     int sys[3];           // SystemSize
 };
 
-// Element index of population q at node (x, y, z).  Chunked (single simulations, lr >= 0): each
-// z-plane is cut into chunks of R = 2^lr consecutive rows (R divides ny); a chunk holds its Q
-// populations as Q consecutive runs of R * nx values (row y % R of the chunk at offset
-// (y % R) * nx in each run).  R is the smallest that makes a run a multiple of 64 nodes (one
-// wavefront) where ny allows.  The collide tiles are row segments, so a wavefront's access is one
-// stretch of a run and a chunk's 19 runs lie together.  Planar (batch members, lr = -1): plane q
-// holds population q of every node; batched tiles are runs of consecutive nodes of any row
-// length, which a planar field keeps contiguous.
+// Element index of population q at node (x, y, z).  Chunked: each z-plane is cut into chunks of
+// R = 2^lr consecutive rows (R divides ny); a chunk holds its Q populations as Q consecutive runs
+// of R * nx values (row y % R of the chunk at offset (y % R) * nx in each run): qs = R * nx,
+// cs = Q * R * nx, ps = Q * nx * ny.  R is the smallest that makes a run a multiple of 64 nodes
+// (one wavefront) where ny allows.  The collide tiles of a single simulation are row segments, so
+// a wavefront's access is one stretch of a run and a chunk's 19 runs lie together.  Planar (plane
+// q holds population q of every node): qs = nint, ps = nx * ny, lr = PLANAR_LR (every row in
+// chunk 0), cs = 0; batched tiles are runs of consecutive nodes of any row length, which a planar
+// field keeps contiguous, so batch members whose chunk runs are not whole wavefronts stay planar.
+constexpr int PLANAR_LR = 30;
 __host__ __device__ inline long long pop_index(const Geom& g, int q, int x, int y, int z) {
-    if (g.lr < 0) return (long long)q * g.nint + x + (long long)g.nx * (y + (long long)g.ny * z);
-    const long long run = (long long)g.nx << g.lr;
-    return (long long)Q * g.nx * g.ny * z + (long long)Q * run * (y >> g.lr) + (long long)q * run + (long long)(y & ((1 << g.lr) - 1)) * g.nx + x;
+    return (long long)q * g.qs + (long long)z * g.ps + (long long)(y >> g.lr) * g.cs + (long long)(y & ((1 << g.lr) - 1)) * g.nx + x;
 }
-// the layout a kernel reads and how it forms population addresses
+// how a kernel forms population addresses
 enum : int {
-    ADDR_CHUNKED = 0,  // chunked, z uniform over the wavefront: the plane in each population's scalar base
-    ADDR_PLANAR = 1,   // planar: a 32-bit lane offset within each population's plane
-    ADDR_ANY = 2,      // the lattice's own layout (g.lr), any z per lane (cold kernels)
+    ADDR_ZPLANE = 0,  // z uniform over the wavefront: the plane in each population's scalar base
+    ADDR_CELL32 = 1,  // a 32-bit lane offset over the whole lattice (batch members, Q * nint values below 4 GB)
+    ADDR_WIDE = 2,    // any z per lane: the plane term in 64 bits (cold kernels)
 };
 
 struct FluidParams {

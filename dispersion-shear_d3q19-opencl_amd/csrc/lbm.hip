@@ -217,12 +217,11 @@ __host__ __device__ inline int plane_size(const Geom& g, int A) {
 // periodic wrap, then the wall unknowns (rebuilt, or the raw uploaded values once after an upload),
 // then the optional tangential-wall passes (sim_main.c:388-404).
 // RO = double decodes the stored populations in double precision (observables).
-// AM: the layout read and how addresses are formed (pop_index is the layout): ADDR_CHUNKED, z
-// uniform over the wavefront (the single-simulation collide tiles): the source plane goes into
-// each population's scalar base and the lane offset spans one plane; ADDR_PLANAR: a 32-bit lane
-// offset within each population's plane (batch members); ADDR_ANY: the lattice's own layout, any z
-// per lane.
-template <int ST, int AM = ADDR_ANY, class RO = typename Pop<ST>::R, class PIn = const typename Pop<ST>::T*>
+// AM: how the addresses are formed (pop_index is the layout, either form): ADDR_ZPLANE, z uniform
+// over the wavefront (the single-simulation collide tiles): the source plane goes into each
+// population's scalar base and the lane offset spans one plane; ADDR_CELL32: a 32-bit lane offset
+// over the whole lattice (batch members); ADDR_WIDE: any z per lane, the plane term in 64 bits.
+template <int ST, int AM = ADDR_WIDE, class RO = typename Pop<ST>::R, class PIn = const typename Pop<ST>::T*>
 __device__ __forceinline__ void gather_pre(PIn fin, const Geom& g, const FluidParams& fl,
                                            const void* bcraw, int use_raw, int x, int y, int z, RO* f) {
     using P = Pop<ST>;
@@ -230,55 +229,46 @@ __device__ __forceinline__ void gather_pre(PIn fin, const Geom& g, const FluidPa
     using RW = typename P::R;  // type of the raw wall-unknown buffer
     // Neighbour offsets with periodic wrap.  A source beyond a wall is replaced by the node
     // itself: those populations are wall unknowns, rebuilt below, so every load is in bounds and
-    // no load is predicated.  x shifts are +-1 element in either layout (periodic: across the row).
+    // no load is predicated.  An x shift is +-1 element (periodic: across the row); a row's offset
+    // within its plane, and so a y shift, is the same for every population.
     const int oxm = (x > 0) ? -1 : (g.per[0] ? g.nx - 1 : 0);
     const int oxp = (x < g.nx - 1) ? 1 : (g.per[0] ? -(g.nx - 1) : 0);
-    const int ym = (y > 0) ? y - 1 : (g.per[1] ? g.ny - 1 : y);
-    const int yp = (y < g.ny - 1) ? y + 1 : (g.per[1] ? 0 : y);
+    const int rmask = (1 << g.lr) - 1;
+    auto row_at = [&](int yy) __attribute__((always_inline)) { return (yy >> g.lr) * g.cs + (yy & rmask) * g.nx; };
+    const int e0 = row_at(y);
+    const int oym = row_at((y > 0) ? y - 1 : (g.per[1] ? g.ny - 1 : y)) - e0;
+    const int oyp = row_at((y < g.ny - 1) ? y + 1 : (g.per[1] ? 0 : y)) - e0;
     const auto f0 = (const D3Q19_GLOBAL typename P::T*)fin;
     auto load = [&](auto qc, const D3Q19_GLOBAL typename P::T* base, unsigned off) __attribute__((always_inline)) {
         constexpr int q = decltype(qc)::value;
         if constexpr (std::is_same<RO, typename P::R>::value) f[q] = P::template ld<q>(base, off);
         else f[q] = P::template ldd<q>(base, off);
     };
-    auto planar = [&]() __attribute__((always_inline)) {
-        const int sz1 = g.nx * g.ny;
-        const int oym = (ym - y) * g.nx, oyp = (yp - y) * g.nx;
-        const int ozm = (z > 0) ? -sz1 : (g.per[2] ? (g.nz - 1) * sz1 : 0);
-        const int ozp = (z < g.nz - 1) ? sz1 : (g.per[2] ? -(g.nz - 1) * sz1 : 0);
-        const int n = x + g.nx * (y + g.ny * z);
+    if constexpr (AM == ADDR_CELL32) {
+        const int ps = (int)g.ps;
+        const int ozm = (z > 0) ? -ps : (g.per[2] ? (g.nz - 1) * ps : 0);
+        const int ozp = (z < g.nz - 1) ? ps : (g.per[2] ? -(g.nz - 1) * ps : 0);
+        const int n = ps * z + e0 + x;
         Unroll<0, Q>::run([&](auto qc) __attribute__((always_inline)) {
             constexpr int q = decltype(qc)::value;
             // pull: source node = n - c_q
             const int ox = cx(q) == 0 ? 0 : (cx(q) > 0 ? oxm : oxp);
             const int oy = cy(q) == 0 ? 0 : (cy(q) > 0 ? oym : oyp);
             const int oz = cz(q) == 0 ? 0 : (cz(q) > 0 ? ozm : ozp);
-            load(qc, f0 + (long long)q * g.nint, (unsigned)(n + ox + oy + oz));
+            load(qc, f0 + q * g.qs, (unsigned)(n + ox + oy + oz));
         });
-    };
-    // chunked: a row's offset within its plane, and so a y shift, is the same for every population
-    auto chunked = [&](int zz) __attribute__((always_inline)) {
-        const int run = g.nx << g.lr, cs = Q * run, rmask = (1 << g.lr) - 1;
-        auto row_at = [&](int yy) __attribute__((always_inline)) { return (yy >> g.lr) * cs + (yy & rmask) * g.nx; };
-        const int e0 = row_at(y);
-        const int oym = row_at(ym) - e0, oyp = row_at(yp) - e0;
+    } else {
+        const int zz = AM == ADDR_ZPLANE ? __builtin_amdgcn_readfirstlane(z) : z;
         const int zm = (zz > 0) ? zz - 1 : (g.per[2] ? g.nz - 1 : zz);
         const int zp = (zz < g.nz - 1) ? zz + 1 : (g.per[2] ? 0 : zz);
-        const long long ps = (long long)Q * g.nx * g.ny;
         const int n = e0 + x;
         Unroll<0, Q>::run([&](auto qc) __attribute__((always_inline)) {
             constexpr int q = decltype(qc)::value;
             const int ox = cx(q) == 0 ? 0 : (cx(q) > 0 ? oxm : oxp);
             const int oy = cy(q) == 0 ? 0 : (cy(q) > 0 ? oym : oyp);
             const int zs = cz(q) == 0 ? zz : (cz(q) > 0 ? zm : zp);
-            load(qc, f0 + (long long)q * run + zs * ps, (unsigned)(n + ox + oy));
+            load(qc, f0 + q * g.qs + zs * g.ps, (unsigned)(n + ox + oy));
         });
-    };
-    if constexpr (AM == ADDR_PLANAR) planar();
-    else if constexpr (AM == ADDR_CHUNKED) chunked(__builtin_amdgcn_readfirstlane(z));
-    else {
-        if (g.lr < 0) planar();
-        else chunked(z);
     }
     const auto rw = to_global<RW>(bcraw);
     const int cr = fl.calc_rho;
@@ -452,7 +442,7 @@ __device__ __forceinline__ void collide_node(const CollideArgs& a, const void* f
     }
 
     R f[Q];
-    gather_pre<ST, SBASE_STORES ? ADDR_CHUNKED : ADDR_PLANAR>(to_global<T>(fin), g, a.fl, a.bcraw, a.use_bcraw, x, y, z, f);
+    gather_pre<ST, SBASE_STORES ? ADDR_ZPLANE : ADDR_CELL32>(to_global<T>(fin), g, a.fl, a.bcraw, a.use_bcraw, x, y, z, f);
     if constexpr (stage_tau) {
         if (local_tau) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     }
@@ -596,25 +586,22 @@ __device__ __forceinline__ void collide_node(const CollideArgs& a, const void* f
             fo[q] = f[q] + o[q] + fg[q];
         });
     }
-    // SBASE_STORES (the single-simulation tiles, whose z is uniform over the block; chunked
-    // layout): the plane goes into the scalar base, and the stores' lane offset is redefined in
-    // each store block by an empty asm statement (no instruction is emitted); otherwise the
-    // compiler computes its 64-bit form once, before the branch, and every store adds it to the
-    // base with a 64-bit VALU add instead of taking the scalar-base form.  The single-simulation
-    // step gains (fp16 128^3: 60.8 -> 58.1 us); the batched collide (planar layout) lost 4.7% with
-    // it and keeps the 64-bit adds.
+    // SBASE_STORES (the single-simulation tiles, whose z is uniform over the block): the plane
+    // goes into the scalar base (ADDR_ZPLANE), and the stores' lane offset is redefined in each
+    // store block by an empty asm statement (no instruction is emitted); otherwise the compiler
+    // computes its 64-bit form once, before the branch, and every store adds it to the base with
+    // a 64-bit VALU add instead of taking the scalar-base form.  The single-simulation step gains
+    // (fp16 128^3: 60.8 -> 58.1 us); the batched collide (ADDR_CELL32) lost 4.7% with it and keeps
+    // the 64-bit adds.
+    const int erow = (y >> g.lr) * g.cs + (y & ((1 << g.lr) - 1)) * g.nx + x;  // pop_index less the q and plane terms
     D3Q19_GLOBAL T* out;
     unsigned ns;
-    long long qst;  // elements from one population's run (or plane) to the next
     if constexpr (SBASE_STORES) {
-        const int run = g.nx << g.lr;
-        out = to_global<T>(fout) + (long long)__builtin_amdgcn_readfirstlane(z) * ((long long)Q * g.nx * g.ny);
-        ns = (unsigned)((y >> g.lr) * (Q * run) + (y & ((1 << g.lr) - 1)) * g.nx + x);
-        qst = run;
+        out = to_global<T>(fout) + (long long)__builtin_amdgcn_readfirstlane(z) * g.ps;
+        ns = (unsigned)erow;
     } else {
         out = to_global<T>(fout);
-        ns = (unsigned)n;
-        qst = g.nint;
+        ns = (unsigned)((int)g.ps * z + erow);
     }
     if (P::SR && a.fl.sr) {
         // random bits from the node index and the bit patterns of the incoming populations: the
@@ -629,13 +616,13 @@ __device__ __forceinline__ void collide_node(const CollideArgs& a, const void* f
         if constexpr (SBASE_STORES) asm volatile("" : "+v"(ns));
         Unroll<0, Q>::run([&](auto qc) __attribute__((always_inline)) {
             constexpr int q = decltype(qc)::value;
-            P::template st_sr<q, NT>(out + q * qst, ns, fo[q], sr_bits<q, P::SR_BITS>(h));
+            P::template st_sr<q, NT>(out + q * g.qs, ns, fo[q], sr_bits<q, P::SR_BITS>(h));
         });
     } else {
         if constexpr (SBASE_STORES) asm volatile("" : "+v"(ns));
         Unroll<0, Q>::run([&](auto qc) __attribute__((always_inline)) {
             constexpr int q = decltype(qc)::value;
-            P::template st<q, NT>(out + q * qst, ns, fo[q]);
+            P::template st<q, NT>(out + q * g.qs, ns, fo[q]);
         });
     }
 }
@@ -1306,7 +1293,7 @@ __device__ double node_stress(const CellDesc& cd, PIn fin, int x, int y, int z) 
     using R = typename Pop<ST>::R;
     const CollideArgs& c = cd.c;
     double f[Q];
-    gather_pre<ST, ADDR_ANY, double>(fin, c.g, c.fl, c.bcraw, c.use_bcraw, x, y, z, f);
+    gather_pre<ST, ADDR_WIDE, double>(fin, c.g, c.fl, c.bcraw, c.use_bcraw, x, y, z, f);
     const int A = cd.o.axisA, F = cd.o.axisF;
     double rho = 0, jx = 0, jy = 0, jz = 0, pi = 0;
     Unroll<0, Q>::run([&](auto qc) __attribute__((always_inline)) {
