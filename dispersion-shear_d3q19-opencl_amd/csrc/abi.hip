@@ -301,6 +301,11 @@ CellDesc make_cell_desc(Sim* s) {
     d.nxy = s->g.nx * s->g.ny;
     d.inv_nx = 1.0f / (float)d.nx;
     d.inv_nxy = 1.0f / (float)d.nxy;
+    d.pitch = s->batch_pitch > 0 ? s->batch_pitch : d.nx;
+    d.pny = d.pitch * s->g.ny;
+    d.nslots = d.pny * s->g.nz;
+    d.inv_pitch = 1.0f / (float)d.pitch;
+    d.inv_pny = 1.0f / (float)d.pny;
     ObsArgs& o = d.o;
     o.axisA = s->g.wall;
     o.coll = s->opt.collision;
@@ -536,11 +541,22 @@ int sim_init(Sim* s, int device, const int_param_struct* ip, const flp_param_str
     s->device = device;
     s->arena = arena;
     sim_setup(s, ip, fp, opt);
-    // Batch members: the batched collide walks runs of consecutive nodes, so a member keeps the
-    // chunks only where each run is a whole number of wavefronts (and its populations stay within
-    // the 32-bit lane offsets of that kernel); otherwise planar populations.
-    if (shared_stream && (((s->g.nx << s->g.lr) % 64) != 0 || (long long)Q * s->g.nint * (long long)pop_size(s->storage) >= (1LL << 32)))
-        set_planar(s->g);
+    // Batch members: the batched collide walks runs of consecutive slots.  A member keeps the
+    // chunks where each run is a whole number of wavefronts; else where its rows, padded to a
+    // multiple of 64 slots, stay at least 70% full (the padding slots idle, so that every
+    // wavefront stays within one row's run); else it takes planar populations.  Chunks also need
+    // the populations within the 32-bit lane offsets of that kernel.
+    if (shared_stream) {
+        const int nx = s->g.nx, pad = ((nx + 63) / 64) * 64;
+        const bool fits = (long long)Q * s->g.nint * (long long)pop_size(s->storage) < (1LL << 32) &&
+                          (long long)pad * s->g.ny * s->g.nz < (1LL << 31);
+        if (fits && ((nx << s->g.lr) % 64) == 0) s->batch_pitch = nx;
+        else if (fits && nx * 10 >= pad * 7) s->batch_pitch = pad;
+        else {
+            set_planar(s->g);
+            s->batch_pitch = nx;
+        }
+    }
     const size_t nint = (size_t)s->g.nint;
     const size_t rs = real_size(s->storage);
     s->nbricks = s->g.bnx * s->g.bny * s->g.bnz;

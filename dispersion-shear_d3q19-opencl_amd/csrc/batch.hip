@@ -188,7 +188,8 @@ static int upload_tables(Batch* b) {
         for (int c = 0; c < b->max_cells; c++) {
             const Sim* s = b->cells[c].sim;
             if (!s || (s->opt.variable_viscosity ? 1 : 0) != pass) continue;
-            for (int base = 0; base < s->g.nint; base += tile) b->h_tiles.push_back(make_uint2((unsigned)c, (unsigned)base));
+            const long long nslots = (long long)(s->batch_pitch > 0 ? s->batch_pitch : s->g.nx) * s->g.ny * s->g.nz;
+            for (long long base = 0; base < nslots; base += tile) b->h_tiles.push_back(make_uint2((unsigned)c, (unsigned)base));
         }
     }
     int max_ppp = 0;

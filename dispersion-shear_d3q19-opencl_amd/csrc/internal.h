@@ -343,6 +343,10 @@ struct CellDesc {
     int nint;
     int nx, nxy;
     float inv_nx, inv_nxy;  // flat node index -> (x, y, z) by float reciprocal + exact correction
+    // the batched collide's slots: rows of `pitch` slots (nx, or nx padded to a multiple of 64 so
+    // that each wavefront stays within one row), slot index -> (x, y, z) as for nodes
+    int pitch, pny, nslots;
+    float inv_pitch, inv_pny;
 };
 
 // p[1] from p[0] and both buffer pairs (host, after posb / velb are set)
@@ -460,6 +464,7 @@ struct Sim {
     // counter are used, and the batch owns neither
     Arena* arena = nullptr;
     bool shares_stream = false;
+    int batch_pitch = 0;  // batch members: slots per row of the batched collide (0: nx)
     bool shares_epochs = false;
     // single-cell descriptor for the observables kernel (host copy kept for the async upload)
     CellDesc* d_desc = nullptr;

@@ -696,8 +696,20 @@ __device__ __forceinline__ void split_node(const CellDesc& cd, int n, int& x, in
     x = xx; y = yy; z = zz;
 }
 
+// slot index of the batched collide -> (x, y, z); x >= nx is a padding slot
+__device__ __forceinline__ void split_slot(const CellDesc& cd, int n, int& x, int& y, int& z) {
+    int zz = (int)((float)n * cd.inv_pny);
+    int r = n - zz * cd.pny;
+    if (r < 0) { zz--; r += cd.pny; } else if (r >= cd.pny) { zz++; r -= cd.pny; }
+    int yy = (int)((float)r * cd.inv_pitch);
+    int xx = r - yy * cd.pitch;
+    if (xx < 0) { yy--; xx += cd.pitch; } else if (xx >= cd.pitch) { yy++; xx -= cd.pitch; }
+    x = xx; y = yy; z = zz;
+}
+
 // Batched collide+stream: a persistent grid walks the tile list of all cells (a tile is blockDim
-// consecutive nodes of one cell).  Cells are shear cells (periodic x and y, walls normal to z);
+// consecutive slots of one cell: rows of cd.pitch slots, of which the first nx are the row's
+// nodes).  Cells are shear cells (periodic x and y, walls normal to z);
 // each cell's tau law, wall speeds, spheres and lattice pointers come from its descriptor.  sp is
 // the step parity: the cell's state is read from f[par ^ sp] and written to the other lattice.
 // VT selects the tile range: 0 the constant-tau cells' tiles [0, ntiles_ct), 1 the per-node-tau
@@ -715,11 +727,11 @@ __device__ __forceinline__ void collide_batch_tiles(const BatchTables& t, const 
         const void* fin = cd.f[cd.par ^ sp];
         void* fout = (void*)cd.f[cd.par ^ sp ^ 1];
         for (int j = 0; j < span; j++) {
-            const int n = (int)tl.y + j * (int)blockDim.x + (int)threadIdx.x;
-            if (n >= cd.nint) break;
+            const int sl = (int)tl.y + j * (int)blockDim.x + (int)threadIdx.x;
+            if (sl >= cd.nslots) break;
             int x, y, z;
-            split_node(cd, n, x, y, z);
-            collide_node<ST, COLL, VT, NT, false>(cd.c, fin, fout, write_u_all, x, y, z, probe);
+            split_slot(cd, sl, x, y, z);
+            if (x < cd.nx) collide_node<ST, COLL, VT, NT, false>(cd.c, fin, fout, write_u_all, x, y, z, probe);
         }
     }
 }
