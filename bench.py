@@ -115,7 +115,8 @@ def run(args):
     # step-kernel roofline: algorithmic bytes 19 loads + 19 stores of fp32 per interior node
     bytes_per_launch = 152.0 * nodes
     achieved = bytes_per_launch / (r32["step_kernel_ms"] * 1e-3) / 1e9
-    attain, _ = d3q19.stream_probe(256, 256, 128, streams=19, width=1, pull=True, launches=20)
+    # the step kernel's own access pattern: 19 populations in chunks of one 256-node row
+    attain, _ = d3q19.stream_probe(256, 256, 128, streams=19, width=-256, pull=True, launches=20)
     r16 = step_run(c3_deck(storage=1), args.steps, args.warmup, graph_steps=args.graph_steps)
     ach16 = 76.0 * nodes / (r16["step_kernel_ms"] * 1e-3) / 1e9
     out = {
@@ -142,8 +143,9 @@ def run(args):
                      "kernel": "step_kernel_w6 (fused collide+stream with the spheres' turn), 152 B/node algorithmic",
                      "kernel_ms": round(r32["step_kernel_ms"], 5),
                      "attainable_GBs": round(attain, 1), "frac_of_attainable": round(achieved / attain, 4),
-                     "attainable_note": "d3q19_stream_probe: the same 19+19 fp32 plane pattern at the pull offsets with no "
-                                        "arithmetic, 256x256x128 box"},
+                     "attainable_note": "d3q19_stream_probe: the step kernel's access pattern (19 fp32 populations read at "
+                                        "the pull offsets and 19 written, in chunks of one 256-node row) with no arithmetic, "
+                                        "256x256x128 box"},
         "ibm_ms": round(r32["ibm_ms"], 5),
         "graph_replay": {"ms_per_step": round(r32.get("graph_ms_per_step", 0.0), 5),
                          "mlups": round(r32.get("graph_mlups", 0.0), 1)},
@@ -428,10 +430,11 @@ def traffic_probe():
         torch.neg(src, out=dst)
     torch.cuda.synchronize()
     print("probe: copy of %d bytes read + %d bytes written, 3 launches" % (4 * COPY_ELEMS, 4 * COPY_ELEMS), flush=True)
-    # the attainable-bandwidth probe with the collide's own access pattern (19 fp32 planes read at
-    # the pull offsets, 19 written, 256x256x128 box): 2 warm-up and 3 timed launches of known bytes,
-    # the calibration whose request mix is the step kernel's
-    d3q19.stream_probe(256, 256, 128, streams=19, width=1, pull=True, launches=3)
+    # the attainable-bandwidth probe with the step kernel's own access pattern (19 fp32 populations
+    # read at the pull offsets and 19 written, in chunks of one 256-node row, 256x256x128 box): 2
+    # warm-up and 3 timed launches of known bytes, the calibration whose request mix is the step
+    # kernel's
+    d3q19.stream_probe(256, 256, 128, streams=19, width=-256, pull=True, launches=3)
     print("probe: pull-pattern probe, %d bytes read + %d bytes written per launch, 5 launches" % (19 * 4 * PROBE_NODES, 19 * 4 * PROBE_NODES),
           flush=True)
 
