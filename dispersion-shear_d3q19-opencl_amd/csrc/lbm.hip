@@ -685,6 +685,15 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) s
     step_body<ST, COLL, VT, NT>(a, p, nsph, do_post);
 }
 
+// ... and with 7 waves per SIMD (72 VGPRs, 20 bytes spilled), which the 16-bit storages run
+// faster (fp16 on the 256x256x128 Casson workload: 152.3 against 158.9 us; fp32 258.8 against
+// 259.0, so fp32 keeps 6)
+template <int ST, int COLL, int VT, bool NT>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) step_kernel_w7(CollideArgs a, PartArgs p, int nsph,
+                                                                                            int do_post) {
+    step_body<ST, COLL, VT, NT>(a, p, nsph, do_post);
+}
+
 // flat node index of a cell -> (x, y, z); exact for n < 2^24 (float estimate within +-1, corrected)
 __device__ __forceinline__ void split_node(const CellDesc& cd, int n, int& x, int& y, int& z) {
     int zz = (int)((float)n * cd.inv_nxy);
@@ -862,7 +871,10 @@ static void launch_fused_one(const CollideArgs& a, const PartArgs& p, int nsph, 
                              hipEvent_t ev0, hipEvent_t ev1) {
     const bool nt = a.nt_stores;
     const dim3 grid(a.nblk + nsph);
-    if constexpr (COLL == D3Q19_COLL_BGK && ST != D3Q19_STORE_FP64) {
+    if constexpr (COLL == D3Q19_COLL_BGK && (ST == D3Q19_STORE_FP16_SHIFT || ST == D3Q19_STORE_BF16_SHIFT)) {
+        if (nt) launch_maybe_timed(step_kernel_w7<ST, COLL, VT, true>, grid, dim3(threads), st, ev0, ev1, a, p, nsph, do_post);
+        else launch_maybe_timed(step_kernel_w7<ST, COLL, VT, false>, grid, dim3(threads), st, ev0, ev1, a, p, nsph, do_post);
+    } else if constexpr (COLL == D3Q19_COLL_BGK && ST != D3Q19_STORE_FP64) {
         if (nt) launch_maybe_timed(step_kernel_w6<ST, COLL, VT, true>, grid, dim3(threads), st, ev0, ev1, a, p, nsph, do_post);
         else launch_maybe_timed(step_kernel_w6<ST, COLL, VT, false>, grid, dim3(threads), st, ev0, ev1, a, p, nsph, do_post);
     } else {
