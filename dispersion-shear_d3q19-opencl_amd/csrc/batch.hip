@@ -417,9 +417,9 @@ static int batch_observables(Batch* b, const int* ids, int n, d3q19_observables*
     return 0;
 }
 
-// watchdog words of cells ids[0..n) on their current lattices: one launch for all cell slots, one
-// copy, one synchronisation
-static int batch_watchdog(Batch* b, const int* ids, int n, int* nonfinite, int* moved) {
+// A server check: the observables and the watchdog words of cells ids[0..n), two launches, two
+// copies and one synchronisation.
+static int batch_check(Batch* b, const int* ids, int n, d3q19_observables* out, int* nonfinite, int* moved) {
     int rc;
     if (b->dirty && (rc = upload_tables(b))) return rc;
     const int nslots = active_slots(b);
@@ -427,16 +427,19 @@ static int batch_watchdog(Batch* b, const int* ids, int n, int* nonfinite, int* 
     long long maxn = 1;
     for (int c = 0; c < nslots; c++)
         if (b->cells[c].sim) maxn = std::max(maxn, (long long)b->cells[c].sim->g.nint);
-    // blocks per cell: about 16 k nodes per block for the largest cell, 1 to 256
     const int bpc = (int)std::min<long long>(256, std::max<long long>(1, (maxn + 16383) / 16384));
+    if ((rc = launch_observables(b->d_cells, nslots, b->bo.storage, b->phase, b->d_obs, b->stream))) return rc;
     HIP_OK(hipMemsetAsync(b->d_wd, 0, (size_t)2 * nslots * sizeof(int), b->stream));
     if ((rc = launch_watchdog(b->d_cells, nslots, b->bo.storage, b->phase, bpc, b->d_wd, b->stream))) return rc;
+    std::vector<double> raw((size_t)OBS_STRIDE * nslots);
     std::vector<int> w((size_t)2 * nslots);
+    HIP_OK(hipMemcpyAsync(raw.data(), b->d_obs, raw.size() * sizeof(double), hipMemcpyDeviceToHost, b->stream));
     HIP_OK(hipMemcpyAsync(w.data(), b->d_wd, w.size() * sizeof(int), hipMemcpyDeviceToHost, b->stream));
     HIP_OK(hipStreamSynchronize(b->stream));
     for (int i = 0; i < n; i++) {
         const int c = ids[i];
         if (c < 0 || c >= nslots || !b->cells[c].sim) return bfail(D3Q19_E_INVALID, "no such cell");
+        derive_observables(b->cells[c].sim, raw.data() + (size_t)OBS_STRIDE * c, &out[i]);
         nonfinite[i] = w[2 * (size_t)c];
         moved[i] = w[2 * (size_t)c + 1];
     }
@@ -682,11 +685,11 @@ static int server_progress(Server* v, d3q19_result* done, int cap, int* n_done) 
     if ((rc = batch_run(b, v->interval, 0))) return rc;
     std::vector<int> ids;
     for (Request* r : v->resident) ids.push_back(r->cell);
+    // observables and watchdog of every resident cell on its current lattice (one launch each for
+    // all cells, one synchronisation)
     std::vector<d3q19_observables> obs(ids.size());
-    if ((rc = batch_observables(b, ids.data(), (int)ids.size(), obs.data()))) return rc;
-    // watchdog of every resident cell on its current lattice (one launch for all)
     std::vector<int> bad(ids.size(), 0), moved(ids.size(), 0);
-    if ((rc = batch_watchdog(b, ids.data(), (int)ids.size(), bad.data(), moved.data()))) return rc;
+    if ((rc = batch_check(b, ids.data(), (int)ids.size(), obs.data(), bad.data(), moved.data()))) return rc;
     std::vector<Request*> keep;
     for (size_t i = 0; i < v->resident.size(); i++) {
         Request* r = v->resident[i];
