@@ -85,7 +85,6 @@ struct Batch {
     // launch grids in effect: a tuning field of 0 means "one work item per block / wavefront",
     // rounded up to a power of two so that the captured graph survives most membership changes
     int eff_collide = 1, eff_collide_vt = 0, eff_sphere = 1, eff_ibm = 1;
-    int eff_rows = 0;      // IBM row layout (decided from the member set's marker count)
     int eff_nt = 0;        // non-temporal population stores (decided from the members' lattice bytes)
     hipGraphExec_t graph2 = nullptr;
     bool graph_valid = false;
@@ -109,7 +108,6 @@ static BatchLaunch launch_of(const Batch* b) {
     L.grid_collide_vt = b->eff_collide_vt;
     L.grid_sphere = b->eff_sphere;
     L.grid_ibm = b->eff_ibm;
-    L.ibm_rows = b->eff_rows;
     L.nt_stores = b->eff_nt;
     return L;
 }
@@ -152,16 +150,11 @@ static int upload_tables(Batch* b) {
     b->h_sph.clear();
     b->h_msph.clear();
     // the members' population buffers together beyond the Infinity Cache: non-temporal population
-    // stores, as for a single simulation of that size; many markers: the IBM row layout (the same
-    // bits as one marker per wavefront)
-    long long pop_total = 0, markers_total = 0;
+    // stores, as for a single simulation of that size
+    long long pop_total = 0;
     for (int c = 0; c < b->max_cells; c++)
-        if (const Sim* s = b->cells[c].sim) {
-            pop_total += 2LL * Q * s->pop_bytes * s->g.nint;
-            markers_total += (long long)s->np * s->ppp;
-        }
+        if (const Sim* s = b->cells[c].sim) pop_total += 2LL * Q * s->pop_bytes * s->g.nint;
     const int nt = pop_total > INFINITY_CACHE_BYTES ? 1 : 0;
-    const int rows = (b->bo.ibm_kernel == D3Q19_IBM_PESKIN4 && markers_total >= IBM_ROWS_MIN_MARKERS) ? 1 : 0;
     for (int c = 0; c < b->max_cells; c++) {
         Member& m = b->cells[c];
         if (!m.sim) {
@@ -225,12 +218,11 @@ static int upload_tables(Batch* b) {
     const int ev = n_vt == 0 ? 0 : (b->tuning.grid_collide > 0 ? b->tuning.grid_collide : grid_for(n_vt));
     const int es = b->tuning.grid_sphere > 0 ? b->tuning.grid_sphere : grid_for(b->h_cnt.nsph);
     const int ei = b->tuning.grid_ibm > 0 ? b->tuning.grid_ibm : grid_for(b->h_cnt.nmark);  // one group per block
-    if (ec != b->eff_collide || ev != b->eff_collide_vt || es != b->eff_sphere || ei != b->eff_ibm || rows != b->eff_rows || nt != b->eff_nt) {
+    if (ec != b->eff_collide || ev != b->eff_collide_vt || es != b->eff_sphere || ei != b->eff_ibm || nt != b->eff_nt) {
         b->eff_collide = ec;
         b->eff_collide_vt = ev;
         b->eff_sphere = es;
         b->eff_ibm = ei;
-        b->eff_rows = rows;
         b->eff_nt = nt;
         b->graph_valid = false;
     }

@@ -538,7 +538,6 @@ struct BatchLaunch {
     int grid_collide_vt;   // blocks of the collide kernel over the per-node-tau cells' tiles
     int grid_sphere;       // persistent blocks of the sphere kernels
     int grid_ibm;          // persistent blocks (one marker group each per iteration) of the IBM kernel
-    int ibm_rows;          // Peskin-4 markers four to a wavefront (batches of many markers)
     int nt_stores;         // non-temporal population stores (members' lattices beyond the Infinity Cache)
 };
 int launch_collide_batch(const BatchLaunch& L, int sp, int write_u_all, int probe, hipStream_t st);
