@@ -709,6 +709,14 @@ int sim_init(Sim* s, int device, const int_param_struct* ip, const flp_param_str
         const int cap = shared_stream ? TILE_NODES_BATCH : TILE_NODES;
         s->group_markers = TILE_MARKERS;
         bool chosen = false;
+        // A batch member takes the largest groups whose boxes fit, with any layout and marker count:
+        // the batched IBM kernel's wavefronts loop over a group's markers, and the batch as a whole
+        // has blocks enough.
+        for (int gm = TILE_MARKERS_MAX; shared_stream && gm > TILE_MARKERS && !chosen; gm /= 2)
+            if (build_groups(gm, perm, gbox) <= cap) {
+                s->group_markers = gm;
+                chosen = true;
+            }
         for (int gm = TILE_MARKERS_MAX; large && !shared_stream && gm > TILE_MARKERS && !chosen; gm /= 2) {
             const long long blocks = (long long)s->np * ((s->ppp + gm - 1) / gm);
             if (blocks < IBM_LARGE_GROUP_MIN_BLOCKS && layout == D3Q19_IBM_LAYOUT_AUTO) continue;

@@ -199,13 +199,15 @@ static int upload_tables(Batch* b) {
     if (b->h_sph.size() >= (1u << 19) || nmark >= (1LL << 31))
         return bfail(D3Q19_E_CAPACITY, "batch holds more than 2^19 spheres or 2^31 markers");
     // LDS-staged IBM: group list, (sphere << 12) | first position of the group in the sphere's
-    // group order; round k takes group k of every sphere with more than k groups, so that blocks
-    // that run together spread into different spheres' nodes
+    // group order (groups of the cell's group_markers); round k takes group k of every sphere with
+    // more than k groups, so that blocks that run together spread into different spheres' nodes
     {
         b->h_msph.reserve((size_t)(nmark / TILE_MARKERS + b->h_sph.size()));
         for (int k = 0; k * TILE_MARKERS < max_ppp; k++)
-            for (size_t i = 0; i < b->h_sph.size(); i++)
-                if (k * TILE_MARKERS < b->h_sph[i].w) b->h_msph.push_back(((int)i << 12) | (k * TILE_MARKERS));
+            for (size_t i = 0; i < b->h_sph.size(); i++) {
+                const int gm = b->cells[b->h_sph[i].x].sim->group_markers;
+                if (k * gm < b->h_sph[i].w) b->h_msph.push_back(((int)i << 12) | (k * gm));
+            }
     }
     (void)nslots;
     b->h_cnt.ntiles = (int)b->h_tiles.size();

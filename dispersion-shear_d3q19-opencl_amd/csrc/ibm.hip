@@ -380,9 +380,12 @@ __device__ __forceinline__ int rel_offset(int n, int c, int nd, int per) {
     return o;
 }
 
-template <class R, int MODE, bool ROWS>
-__device__ __forceinline__ void ibm_tile_group(const IbmArgs& a, const D3Q19_GLOBAL double* pos, const D3Q19_GLOBAL double* vel, int p,
-                                               int grp, int m, bool has, int lane, long long* tile, int cap) {
+// One group of markers of sphere p staged in an LDS box: zero the box, spread(sink) adds the group's
+// contributions, then the box's nonzero nodes go to the spread force with one global atomic per
+// node and component, and their force tags are set.
+template <class Spread>
+__device__ __forceinline__ void ibm_tile_box(const IbmArgs& a, const D3Q19_GLOBAL double* pos, int p, int grp, long long* tile, int cap,
+                                             Spread spread) {
     const Geom& g = a.g;
     const int nd[3] = {g.nx, g.ny, g.nz};
     // the group's box, relative to the sphere's centre node, was computed from the offsets on the
@@ -420,8 +423,7 @@ __device__ __forceinline__ void ibm_tile_group(const IbmArgs& a, const D3Q19_GLO
             spread_add(a.facc, a.ftag, g.nint, sn[0] + g.nx * (sn[1] + g.ny * sn[2]), w, F);
         }
     };
-    if constexpr (rows_layout<MODE, ROWS>()) ibm_rows<R>(a, pos, vel, p, m, has, lane, sink);  // every lane: row sums
-    else if (has) ibm_marker<R, MODE>(a, pos, vel, p, m, lane, sink);
+    spread(sink);
     __syncthreads();
     // box position of node i by multiply and shift: for i < 2048 and extents up to 1024,
     // (i * ceil(2^21 / e)) >> 21 is i / e exactly (the excess stays below i / 2^21 < 1 / e), and
@@ -447,6 +449,16 @@ __device__ __forceinline__ void ibm_tile_group(const IbmArgs& a, const D3Q19_GLO
                                        __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         a.ftag[node] = 1;
     }
+}
+
+// a single simulation's group: one marker per row (ROWS) or wavefront, marker m of this slot
+template <class R, int MODE, bool ROWS>
+__device__ __forceinline__ void ibm_tile_group(const IbmArgs& a, const D3Q19_GLOBAL double* pos, const D3Q19_GLOBAL double* vel, int p,
+                                               int grp, int m, bool has, int lane, long long* tile, int cap) {
+    ibm_tile_box(a, pos, p, grp, tile, cap, [&](auto& sink) __attribute__((always_inline)) {
+        if constexpr (rows_layout<MODE, ROWS>()) ibm_rows<R>(a, pos, vel, p, m, has, lane, sink);  // every lane: row sums
+        else if (has) ibm_marker<R, MODE>(a, pos, vel, p, m, lane, sink);
+    });
 }
 
 // threads per marker of a group's block: a row or a wavefront; the most threads of such a block
@@ -475,18 +487,24 @@ __global__ void __launch_bounds__((group_threads<MODE, ROWS>())) ibm_tile_kernel
 }
 
 // Batched, spreading staged in LDS: a persistent grid of one-group blocks walks the group list of
-// all cells, (sphere << 12) | first position of the group in the sphere's group order, and runs each
-// group as ibm_tile_group does for a single simulation (the same arithmetic: a cell's bits are those
-// of the cell alone).  The box takes TILE_NODES_BATCH per component (every cell's capacity fits).  The
-// batch's epoch counter is committed by the first thread.  The tables come in as restrict
-// arguments (nothing here writes them): the descriptor fields are scalar loads that the force
-// stores do not invalidate.
-template <class R, int MODE, bool ROWS>
-__global__ void __launch_bounds__((group_threads<MODE, ROWS>())) ibm_batch_tile_kernel(BatchTables t, const CellDesc* __restrict__ cells,
-                                                                                      const int4* __restrict__ sphs,
-                                                                                      const int* __restrict__ msph, int sp) {
+// all cells, (sphere << 12) | first position of the group in the sphere's group order.  A cell's
+// groups take its own group_markers (ia.gm: up to TILE_MARKERS_MAX where its boxes fit
+// TILE_NODES_BATCH, else TILE_MARKERS).  A block of 512 threads takes a group in passes: Peskin-4
+// markers 32 at a time, one per 16-lane row (every row of the block runs each pass, rows past the
+// group's end contribute nothing); the trilinear kernel's 8 at a time, one per wavefront.  The
+// arithmetic per marker is a single simulation's, and the integer sums make the field independent
+// of the grouping: a cell's bits are those of the cell alone.  The batch's epoch counter is
+// committed by the first thread.  The tables come in as restrict arguments (nothing here writes
+// them): the descriptor fields are scalar loads that the force stores do not invalidate.
+constexpr int BATCH_IBM_THREADS = 512;
+template <class R, int MODE>
+__global__ void __launch_bounds__(BATCH_IBM_THREADS) ibm_batch_tile_kernel(BatchTables t, const CellDesc* __restrict__ cells,
+                                                                          const int4* __restrict__ sphs, const int* __restrict__ msph,
+                                                                          int sp) {
+    constexpr bool ROWS = rows_layout<MODE, true>();
+    constexpr int SLOTS = BATCH_IBM_THREADS / marker_threads<MODE, true>();  // markers per pass
     extern __shared__ long long tile_mem[];  // 3 x TILE_NODES_BATCH
-    const int lane = threadIdx.x & 63, w = (int)threadIdx.x / marker_threads<MODE, ROWS>();
+    const int lane = threadIdx.x & 63, slot = (int)threadIdx.x / marker_threads<MODE, true>();
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         const uint32_t e = t.epochs[0] + 1u;
         t.epochs[0] = e;
@@ -501,11 +519,19 @@ __global__ void __launch_bounds__((group_threads<MODE, ROWS>())) ibm_batch_tile_
         const CellDesc& cd = cells[cell];
         const IbmArgs& ia = cd.i;
         const int par = __builtin_amdgcn_readfirstlane(cd.par);
-        const int j0 = v & 4095, j = j0 + w;
-        const bool has = j < ppp;
+        const int gm = __builtin_amdgcn_readfirstlane(ia.gm);
+        const int j0 = v & 4095, cnt = min(gm, ppp - j0);
         // x(t+1), v(t+1): written by this step's turn
-        ibm_tile_group<R, MODE, ROWS>(ia, cd.posb[par ^ sp ^ 1], cd.velb[par ^ sp ^ 1], sph, j0 / TILE_MARKERS, has ? ia.perm[j] : 0, has,
-                                lane, tile_mem, TILE_NODES_BATCH);
+        const D3Q19_GLOBAL double* pos = cd.posb[par ^ sp ^ 1];
+        const D3Q19_GLOBAL double* vel = cd.velb[par ^ sp ^ 1];
+        ibm_tile_box(ia, pos, sph, j0 / gm, tile_mem, TILE_NODES_BATCH, [&](auto& sink) __attribute__((always_inline)) {
+            for (int i0 = 0; i0 < cnt; i0 += SLOTS) {  // uniform over the block
+                const int i = i0 + slot;
+                const bool has = i < cnt;
+                if constexpr (ROWS) ibm_rows<R>(ia, pos, vel, sph, has ? ia.perm[j0 + i] : 0, has, lane, sink);  // every lane: row sums
+                else if (has) ibm_marker<R, MODE>(ia, pos, vel, sph, ia.perm[j0 + i], lane, sink);
+            }
+        });
         __syncthreads();  // the next group reuses the tile
     }
 }
@@ -598,24 +624,24 @@ int launch_ibm(Sim* s, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1, int zlo, 
     return 0;
 }
 
-template <class R, int MODE, bool ROWS>
+template <class R, int MODE>
 static void launch_ibm_batch_as(const BatchLaunch& L, int sp, hipStream_t st) {
     const size_t lds = 3 * sizeof(long long) * (size_t)TILE_NODES_BATCH;
-    hipLaunchKernelGGL((ibm_batch_tile_kernel<R, MODE, ROWS>), dim3(L.grid_ibm), dim3(marker_threads<MODE, ROWS>() * TILE_MARKERS), lds, st,
-                       L.t, L.t.cells, L.t.sph, L.t.msph, sp);
+    hipLaunchKernelGGL((ibm_batch_tile_kernel<R, MODE>), dim3(L.grid_ibm), dim3(BATCH_IBM_THREADS), lds, st, L.t, L.t.cells, L.t.sph,
+                       L.t.msph, sp);
 }
 
 // Batched cells run one marker per wavefront whatever the marker count: with groups of 8 markers a
-// block of the row layout is only 2 wavefronts, and the small-cell server measured 63.8 against
+// block of the row layout was only 2 wavefronts, and the small-cell server measured 63.8 against
 // 59.5-59.9 solves/s without it (the 64-cell ensemble and the 24-cell C5 batch unchanged).
 int launch_ibm_batch(const BatchLaunch& L, int sp, hipStream_t st) {
     const bool dbl = L.storage == D3Q19_STORE_FP64;
     if (L.ibm_mode == D3Q19_IBM_PESKIN4) {
-        if (dbl) launch_ibm_batch_as<double, D3Q19_IBM_PESKIN4, false>(L, sp, st);
-        else launch_ibm_batch_as<float, D3Q19_IBM_PESKIN4, false>(L, sp, st);
+        if (dbl) launch_ibm_batch_as<double, D3Q19_IBM_PESKIN4>(L, sp, st);
+        else launch_ibm_batch_as<float, D3Q19_IBM_PESKIN4>(L, sp, st);
     } else {
-        if (dbl) launch_ibm_batch_as<double, D3Q19_IBM_LINEAR_SMOOTHED, false>(L, sp, st);
-        else launch_ibm_batch_as<float, D3Q19_IBM_LINEAR_SMOOTHED, false>(L, sp, st);
+        if (dbl) launch_ibm_batch_as<double, D3Q19_IBM_LINEAR_SMOOTHED>(L, sp, st);
+        else launch_ibm_batch_as<float, D3Q19_IBM_LINEAR_SMOOTHED>(L, sp, st);
     }
     HIP_OK(hipGetLastError());
     return 0;

@@ -45,7 +45,7 @@ constexpr long long IBM_ROWS_MIN_MARKERS = 16384;        // markers from which f
 constexpr int IBM_LARGE_GROUP_MIN_BLOCKS = 512;          // blocks needed to take groups larger than TILE_MARKERS
 constexpr int TILE_MARKERS_MAX = 64;                  // ... of a single simulation's Peskin-4 groups whose boxes fit
 constexpr int TILE_NODES = 2048;                      // LDS box of that spreading (nodes), single simulations
-constexpr int TILE_NODES_BATCH = 1024;                // ... cells of a batch (one-group blocks of 8 markers)
+constexpr int TILE_NODES_BATCH = 2048;                // ... cells of a batch (groups of up to TILE_MARKERS_MAX markers)
 constexpr float FP16_SCALE = 16384.0f;                // 2^14, relative-deviation scale of fp16 storage
 constexpr int MARK_BRICKS = 64;                       // bricks per axis a sphere's marking can list
 
