@@ -778,11 +778,19 @@ __global__ void __launch_bounds__(256) step_batch_kernel(BatchTables t, const Ce
     step_batch_body<ST, COLL, VT, NT>(t, cells, sph, sp, write_u_all, nturn, do_post);
 }
 
-// the same with the register budget of 6 waves per SIMD (80 VGPRs), which the batched BGK collide
-// in fp32 / fp16 / bf16 storage and the sphere turn meet without spilling
+// the same with the register budget of 6 waves per SIMD (80 VGPRs; the sphere code spills 60
+// bytes), for the batched BGK collide in fp16 / bf16 storage, and of 5 waves (96 VGPRs, 8 bytes spilled) in fp32:
+// on the 24-cell C5 batch (fp32) 0.748 / 0.735 against 0.778 / 0.760 ms per step at 6, on the
+// 64-cell fp16 ensemble 0.665 against 0.642 ms (so fp16 keeps 6)
 template <int ST, int COLL, int VT, bool NT>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6)))
 step_batch_kernel_w6(BatchTables t, const CellDesc* __restrict__ cells, const int4* __restrict__ sph, int sp, int write_u_all, int nturn,
+                     int do_post) {
+    step_batch_body<ST, COLL, VT, NT>(t, cells, sph, sp, write_u_all, nturn, do_post);
+}
+template <int ST, int COLL, int VT, bool NT>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5)))
+step_batch_kernel_w5(BatchTables t, const CellDesc* __restrict__ cells, const int4* __restrict__ sph, int sp, int write_u_all, int nturn,
                      int do_post) {
     step_batch_body<ST, COLL, VT, NT>(t, cells, sph, sp, write_u_all, nturn, do_post);
 }
@@ -954,7 +962,9 @@ static void collide_batch_st(const BatchLaunch& L, int sp, int write_u_all, int 
 template <int ST, int COLL, int VT, bool NT>
 static void step_batch_one(const BatchLaunch& L, int grid, int nturn, int sp, int write_u_all, int do_post, hipStream_t st) {
     const dim3 g(grid + nturn), blk(L.tile);
-    if constexpr (COLL == D3Q19_COLL_BGK && ST != D3Q19_STORE_FP64)
+    if constexpr (COLL == D3Q19_COLL_BGK && ST == D3Q19_STORE_FP32)
+        hipLaunchKernelGGL((step_batch_kernel_w5<ST, COLL, VT, NT>), g, blk, 0, st, L.t, L.t.cells, L.t.sph, sp, write_u_all, nturn, do_post);
+    else if constexpr (COLL == D3Q19_COLL_BGK && ST != D3Q19_STORE_FP64)
         hipLaunchKernelGGL((step_batch_kernel_w6<ST, COLL, VT, NT>), g, blk, 0, st, L.t, L.t.cells, L.t.sph, sp, write_u_all, nturn, do_post);
     else
         hipLaunchKernelGGL((step_batch_kernel<ST, COLL, VT, NT>), g, blk, 0, st, L.t, L.t.cells, L.t.sph, sp, write_u_all, nturn, do_post);
