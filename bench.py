@@ -140,7 +140,7 @@ def run(args):
                      "traffic_source": "profiles/hbm_traffic_step_c3_casson.md (rocprofv3 FETCH_SIZE and WRITE_SIZE of the "
                                        "step kernel, each scaled by the same counter's reading for a copy of known size)",
                      "algorithmic_bytes": int(bytes_per_launch),
-                     "kernel": "step_kernel_w6 (fused collide+stream with the spheres' turn), 152 B/node algorithmic",
+                     "kernel": "step_kernel_w5 (fused collide+stream with the spheres' turn), 152 B/node algorithmic",
                      "kernel_ms": round(r32["step_kernel_ms"], 5),
                      "attainable_GBs": round(attain, 1), "frac_of_attainable": round(achieved / attain, 4),
                      "attainable_note": "d3q19_stream_probe: the step kernel's access pattern (19 fp32 populations read at "

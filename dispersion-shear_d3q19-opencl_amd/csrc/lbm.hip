@@ -676,18 +676,17 @@ __global__ void __launch_bounds__(256) step_kernel(CollideArgs a, PartArgs p, in
     step_body<ST, COLL, VT, NT>(a, p, nsph, do_post);
 }
 
-// The same step with the register budget of 6 waves per SIMD (80 VGPRs), which the BGK forms in
-// fp32 / fp16 / bf16 storage meet without spilling; left alone, the sphere turn's code raises the
-// whole kernel to 89 VGPRs (5 waves) while the collide tiles need 77.
+// The same step with a register budget: 5 waves per SIMD for the BGK form in fp32 (85 VGPRs, no
+// spill; the 256x256x128 Casson workload: 254.2-255.5 us against 257.9-258.9 compiled for 6 waves,
+// 80 VGPRs, in alternating runs) ...
 template <int ST, int COLL, int VT, bool NT>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) step_kernel_w6(CollideArgs a, PartArgs p, int nsph,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) step_kernel_w5(CollideArgs a, PartArgs p, int nsph,
                                                                                             int do_post) {
     step_body<ST, COLL, VT, NT>(a, p, nsph, do_post);
 }
 
-// ... and with 7 waves per SIMD (72 VGPRs, 20 bytes spilled), which the 16-bit storages run
-// faster (fp16 on the 256x256x128 Casson workload: 152.3 against 158.9 us; fp32 258.8 against
-// 259.0, so fp32 keeps 6)
+// ... and 7 waves per SIMD (72 VGPRs, 20 bytes spilled) for the 16-bit storages (fp16 on the same
+// workload: 154.1-155.9 against 159.8-162.2 us at 6 waves in alternating runs)
 template <int ST, int COLL, int VT, bool NT>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) step_kernel_w7(CollideArgs a, PartArgs p, int nsph,
                                                                                             int do_post) {
@@ -882,9 +881,9 @@ static void launch_fused_one(const CollideArgs& a, const PartArgs& p, int nsph, 
     if constexpr (COLL == D3Q19_COLL_BGK && (ST == D3Q19_STORE_FP16_SHIFT || ST == D3Q19_STORE_BF16_SHIFT)) {
         if (nt) launch_maybe_timed(step_kernel_w7<ST, COLL, VT, true>, grid, dim3(threads), st, ev0, ev1, a, p, nsph, do_post);
         else launch_maybe_timed(step_kernel_w7<ST, COLL, VT, false>, grid, dim3(threads), st, ev0, ev1, a, p, nsph, do_post);
-    } else if constexpr (COLL == D3Q19_COLL_BGK && ST != D3Q19_STORE_FP64) {
-        if (nt) launch_maybe_timed(step_kernel_w6<ST, COLL, VT, true>, grid, dim3(threads), st, ev0, ev1, a, p, nsph, do_post);
-        else launch_maybe_timed(step_kernel_w6<ST, COLL, VT, false>, grid, dim3(threads), st, ev0, ev1, a, p, nsph, do_post);
+    } else if constexpr (COLL == D3Q19_COLL_BGK && ST == D3Q19_STORE_FP32) {
+        if (nt) launch_maybe_timed(step_kernel_w5<ST, COLL, VT, true>, grid, dim3(threads), st, ev0, ev1, a, p, nsph, do_post);
+        else launch_maybe_timed(step_kernel_w5<ST, COLL, VT, false>, grid, dim3(threads), st, ev0, ev1, a, p, nsph, do_post);
     } else {
         if (nt) launch_maybe_timed(step_kernel<ST, COLL, VT, true>, grid, dim3(threads), st, ev0, ev1, a, p, nsph, do_post);
         else launch_maybe_timed(step_kernel<ST, COLL, VT, false>, grid, dim3(threads), st, ev0, ev1, a, p, nsph, do_post);
