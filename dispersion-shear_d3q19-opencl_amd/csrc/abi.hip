@@ -693,9 +693,10 @@ int sim_init(Sim* s, int device, const int_param_struct* ip, const flp_param_str
         // whose boxes all fit in TILE_NODES while at least IBM_LARGE_GROUP_MIN_BLOCKS blocks remain:
         // fewer global atomics, since a group's box adds each node once.  The 256x256x128
         // workload's IBM pass (64,000 markers): groups of 8 74 us, 16 57 us, 32 52 us, 64 50 us.
-        // Cells of a batch always take groups of TILE_MARKERS, boxes up to TILE_NODES_BATCH.  Both
-        // layouts sum a marker's stencil in the same order and the grouping decides only which
-        // block adds a contribution, so no choice here changes a result bit.
+        // Cells of a batch take the largest groups whose boxes fit TILE_NODES_BATCH, whatever their
+        // marker count (the batched kernel runs a group in passes of its block).  Both layouts sum
+        // a marker's stencil in the same order and the grouping decides only which block adds a
+        // contribution, so no choice here changes a result bit.
         const int layout = s->opt.ibm_layout;
         const bool peskin = s->opt.ibm_kernel == D3Q19_IBM_PESKIN4;
         bool large;

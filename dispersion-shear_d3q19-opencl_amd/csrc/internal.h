@@ -254,9 +254,9 @@ struct IbmArgs {
     int mstride;  // marker visiting order: wavefront k of sphere p takes marker (k * mstride) mod ppp
     int tile_ok;  // spreading may be staged in LDS (every periodic extent exceeds twice the stencil reach)
     int tile_nodes;  // LDS box capacity of this simulation's launches (the largest group box that fits)
-    int gm;          // markers per group (TILE_MARKERS; up to TILE_MARKERS_MAX for a single simulation's Peskin-4 rows)
+    int gm;          // markers per group (TILE_MARKERS up to TILE_MARKERS_MAX)
     const D3Q19_GLOBAL int* perm;  // ppp marker indices in Morton order of their offsets
-    const D3Q19_GLOBAL int* gbox;  // per group of TILE_MARKERS: LDS box relative to the centre node (lo[3], extent[3], volume, 0)
+    const D3Q19_GLOBAL int* gbox;  // per group of gm markers: LDS box relative to the centre node (lo[3], extent[3], volume, 0)
     double dA;
     const D3Q19_GLOBAL double* pos;
     const D3Q19_GLOBAL double* vel;
@@ -367,7 +367,7 @@ struct BatchCounts {
     int ntiles_ct;  // the first ntiles_ct entries belong to cells of constant tau
     int span;       // tiles per entry
     int nsph;     // spheres over all cells
-    int nmark;    // entries of the group list (groups of TILE_MARKERS markers over all spheres)
+    int nmark;    // entries of the group list (each cell's groups over all its spheres)
     int ncells;
 };
 
@@ -428,7 +428,7 @@ struct Sim {
     bool marks_next = false;   // marks of the next epoch exist (made from the current positions)
     int* zone = nullptr;       // 2*np zone of each sphere (contact zones), by rebuild generation
     int* perm = nullptr;       // ppp marker indices in Morton order of their offsets (LDS-staged spreading)
-    int* gbox = nullptr;       // the LDS box of each group of TILE_MARKERS markers (8 ints per group)
+    int* gbox = nullptr;       // the LDS box of each group of group_markers markers (8 ints per group; sized for groups of TILE_MARKERS)
     bool tile_ok = false;      // every periodic extent exceeds twice the spheres' stencil reach
     int tile_nodes = 0;        // the largest group box (nodes) within the LDS cap: the LDS the IBM launches reserve
     int group_markers = TILE_MARKERS;  // markers per group of the LDS-staged spreading
