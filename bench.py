@@ -115,8 +115,9 @@ def run(args):
     # step-kernel roofline: algorithmic bytes 19 loads + 19 stores of fp32 per interior node
     bytes_per_launch = 152.0 * nodes
     achieved = bytes_per_launch / (r32["step_kernel_ms"] * 1e-3) / 1e9
-    # the step kernel's own access pattern: 19 populations in chunks of one 256-node row
-    attain, _ = d3q19.stream_probe(256, 256, 128, streams=19, width=-256, pull=True, launches=20)
+    # the step kernel's own access pattern: 19 populations in chunks of one 256-node row, read and
+    # written with non-temporal accesses
+    attain, _ = d3q19.stream_probe(256, 256, 128, streams=19, width=-256, pull=True, launches=20, nontemporal=True)
     r16 = step_run(c3_deck(storage=1), args.steps, args.warmup, graph_steps=args.graph_steps)
     ach16 = 76.0 * nodes / (r16["step_kernel_ms"] * 1e-3) / 1e9
     out = {
@@ -144,8 +145,8 @@ def run(args):
                      "kernel_ms": round(r32["step_kernel_ms"], 5),
                      "attainable_GBs": round(attain, 1), "frac_of_attainable": round(achieved / attain, 4),
                      "attainable_note": "d3q19_stream_probe: the step kernel's access pattern (19 fp32 populations read at "
-                                        "the pull offsets and 19 written, in chunks of one 256-node row) with no arithmetic, "
-                                        "256x256x128 box"},
+                                        "the pull offsets and 19 written, in chunks of one 256-node row, non-temporal loads "
+                                        "and stores) with no arithmetic, 256x256x128 box"},
         "ibm_ms": round(r32["ibm_ms"], 5),
         "graph_replay": {"ms_per_step": round(r32.get("graph_ms_per_step", 0.0), 5),
                          "mlups": round(r32.get("graph_mlups", 0.0), 1)},
@@ -373,17 +374,20 @@ def stream_sweep(args):
     streaming kernel with 19 fp32 planes read at the D3Q19 pull offsets and 19 written, no
     arithmetic, beside a plain copy, for 1 / 2 / 4 consecutive nodes per lane, on the
     256x256x128 box and on a 512x256x256 box (2.5 GB per direction); then the 19-plane pull
-    pattern with 1 node per lane at 2-8 resident 256-thread blocks per CU (dynamic LDS caps them)."""
+    pattern with 1 node per lane at 2-8 resident 256-thread blocks per CU (dynamic LDS caps them).
+    The chunk layouts (19 populations in runs of 64 or 256 nodes) run with plain accesses and, for
+    256, with the non-temporal loads and stores of the collide on lattices beyond the Infinity
+    Cache."""
     import d3q19
     rows = []
     for box in ((256, 256, 128), (512, 256, 256)):
-        for streams, width, pull in ((1, 4, False), (19, 1, False), (19, 2, False), (19, 4, False),
-                                     (19, 1, True), (19, 2, True), (19, 4, True), (19, -64, False), (19, -64, True),
-                                     (19, -256, True)):
+        for streams, width, pull, nt in ((1, 4, False, False), (19, 1, False, False), (19, 2, False, False), (19, 4, False, False),
+                                         (19, 1, True, False), (19, 2, True, False), (19, 4, True, False), (19, -64, False, False),
+                                         (19, -64, True, False), (19, -256, True, False), (19, -256, True, True)):
             # the copy moves the bytes of the 19 planes, so that neither fits the Infinity Cache
             cbox = (box[0] * 19, box[1], box[2]) if streams == 1 else box
-            gbs, ms = d3q19.stream_probe(*cbox, streams=streams, width=width, pull=pull, launches=args.probe_launches)
-            rows.append({"box": list(cbox), "streams": streams, "width": width, "pull": pull, "lds_bytes": 0,
+            gbs, ms = d3q19.stream_probe(*cbox, streams=streams, width=width, pull=pull, launches=args.probe_launches, nontemporal=nt)
+            rows.append({"box": list(cbox), "streams": streams, "width": width, "pull": pull, "nontemporal": nt, "lds_bytes": 0,
                          "layout": "chunks of %d nodes" % -width if width < 0 else "planes",
                          "GBs": round(gbs, 1), "ms": round(ms, 5), "frac_of_8TBs": round(gbs / HBM_PEAK_GBS, 4)})
             print("stream: %s" % json.dumps(rows[-1]), flush=True)
@@ -431,10 +435,10 @@ def traffic_probe():
     torch.cuda.synchronize()
     print("probe: copy of %d bytes read + %d bytes written, 3 launches" % (4 * COPY_ELEMS, 4 * COPY_ELEMS), flush=True)
     # the attainable-bandwidth probe with the step kernel's own access pattern (19 fp32 populations
-    # read at the pull offsets and 19 written, in chunks of one 256-node row, 256x256x128 box): 2
-    # warm-up and 3 timed launches of known bytes, the calibration whose request mix is the step
-    # kernel's
-    d3q19.stream_probe(256, 256, 128, streams=19, width=-256, pull=True, launches=3)
+    # read at the pull offsets and 19 written, in chunks of one 256-node row, non-temporal loads and
+    # stores, 256x256x128 box): 2 warm-up and 3 timed launches of known bytes, the calibration whose
+    # request mix is the step kernel's
+    d3q19.stream_probe(256, 256, 128, streams=19, width=-256, pull=True, launches=3, nontemporal=True)
     print("probe: pull-pattern probe, %d bytes read + %d bytes written per launch, 5 launches" % (19 * 4 * PROBE_NODES, 19 * 4 * PROBE_NODES),
           flush=True)
 

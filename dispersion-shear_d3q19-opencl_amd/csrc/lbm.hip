@@ -45,13 +45,21 @@ template <bool NT, class T, class PT> __device__ __forceinline__ void st_pop(PT 
     if constexpr (NT) __builtin_nontemporal_store(v, plane_at(p, off));
     else *plane_at(p, off) = v;
 }
+// The gather's population loads are non-temporal too on those lattices: each stored value is read
+// by the one node it streams to, and the stream otherwise evicts the u, force and tau lines that
+// the IBM pass and the next step read (256x256x128 Casson workload, fp32: step kernel 254 -> 227
+// us and the IBM pass after it 58 -> 51 us; fp16: IBM pass 57 -> 53 us, kernel unchanged).
+template <bool NT, class PT> __device__ __forceinline__ auto ld_pop(PT p, unsigned off) {
+    if constexpr (NT) return __builtin_nontemporal_load(plane_at(p, off));
+    else return *plane_at(p, off);
+}
 template <int ST> struct Pop;
 template <> struct Pop<D3Q19_STORE_FP32> {
     using T = float;
     using R = float;
     static constexpr bool SR = false;
     static constexpr int SR_BITS = 0;
-    template <int QI, class PT> __device__ static R ld(PT p, unsigned off) { return *plane_at(p, off); }
+    template <int QI, bool NT = false, class PT> __device__ static R ld(PT p, unsigned off) { return ld_pop<NT>(p, off); }
     template <int QI, class PT> __device__ static double ldd(PT p, unsigned off) { return (double)*plane_at(p, off); }
     template <int QI, bool NT = false, class PT> __device__ static void st(PT p, unsigned off, R v) { st_pop<NT>(p, off, v); }
     template <int QI, bool NT = false, class PT> __device__ static void st_sr(PT p, unsigned off, R v, uint32_t) { st_pop<NT>(p, off, v); }
@@ -61,7 +69,7 @@ template <> struct Pop<D3Q19_STORE_FP64> {
     using R = double;
     static constexpr bool SR = false;
     static constexpr int SR_BITS = 0;
-    template <int QI, class PT> __device__ static R ld(PT p, unsigned off) { return *plane_at(p, off); }
+    template <int QI, bool NT = false, class PT> __device__ static R ld(PT p, unsigned off) { return ld_pop<NT>(p, off); }
     template <int QI, class PT> __device__ static double ldd(PT p, unsigned off) { return *plane_at(p, off); }
     template <int QI, bool NT = false, class PT> __device__ static void st(PT p, unsigned off, R v) { st_pop<NT>(p, off, v); }
     template <int QI, bool NT = false, class PT> __device__ static void st_sr(PT p, unsigned off, R v, uint32_t) { st_pop<NT>(p, off, v); }
@@ -74,10 +82,10 @@ template <> struct Pop<D3Q19_STORE_FP64> {
 template <> struct Pop<D3Q19_STORE_FP16_SHIFT> {
     using T = _Float16;
     using R = float;
-    template <int QI, class PT> __device__ static R ld(PT p, unsigned off) {
+    template <int QI, bool NT = false, class PT> __device__ static R ld(PT p, unsigned off) {
         constexpr float w = (float)wq(QI);
         constexpr float sc = (float)(wq(QI) / (double)FP16_SCALE);
-        return fmaf((float)*plane_at(p, off), sc, w);
+        return fmaf((float)ld_pop<NT>(p, off), sc, w);
     }
     // decoded in double around the float32 w_q the encoding is relative to
     template <int QI, class PT> __device__ static double ldd(PT p, unsigned off) {
@@ -105,9 +113,9 @@ template <> struct Pop<D3Q19_STORE_FP16_SHIFT> {
 template <> struct Pop<D3Q19_STORE_BF16_SHIFT> {
     using T = uint16_t;
     using R = float;
-    template <int QI, class PT> __device__ static R ld(PT p, unsigned off) {
+    template <int QI, bool NT = false, class PT> __device__ static R ld(PT p, unsigned off) {
         constexpr float w = (float)wq(QI);
-        return w + bf16_to_f32(*plane_at(p, off));
+        return w + bf16_to_f32(ld_pop<NT>(p, off));
     }
     template <int QI, class PT> __device__ static double ldd(PT p, unsigned off) {
         return (double)(float)wq(QI) + (double)bf16_to_f32(*plane_at(p, off));
@@ -221,7 +229,7 @@ __host__ __device__ inline int plane_size(const Geom& g, int A) {
 // over the wavefront (the single-simulation collide tiles): the source plane goes into each
 // population's scalar base and the lane offset spans one plane; ADDR_CELL32: a 32-bit lane offset
 // over the whole lattice (batch members); ADDR_WIDE: any z per lane, the plane term in 64 bits.
-template <int ST, int AM = ADDR_WIDE, class RO = typename Pop<ST>::R, class PIn = const typename Pop<ST>::T*>
+template <int ST, int AM = ADDR_WIDE, bool NT = false, class RO = typename Pop<ST>::R, class PIn = const typename Pop<ST>::T*>
 __device__ __forceinline__ void gather_pre(PIn fin, const Geom& g, const FluidParams& fl,
                                            const void* bcraw, int use_raw, int x, int y, int z, RO* f) {
     using P = Pop<ST>;
@@ -241,7 +249,7 @@ __device__ __forceinline__ void gather_pre(PIn fin, const Geom& g, const FluidPa
     const auto f0 = (const D3Q19_GLOBAL typename P::T*)fin;
     auto load = [&](auto qc, const D3Q19_GLOBAL typename P::T* base, unsigned off) __attribute__((always_inline)) {
         constexpr int q = decltype(qc)::value;
-        if constexpr (std::is_same<RO, typename P::R>::value) f[q] = P::template ld<q>(base, off);
+        if constexpr (std::is_same<RO, typename P::R>::value) f[q] = P::template ld<q, NT>(base, off);
         else f[q] = P::template ldd<q>(base, off);
     };
     if constexpr (AM == ADDR_CELL32) {
@@ -442,7 +450,7 @@ __device__ __forceinline__ void collide_node(const CollideArgs& a, const void* f
     }
 
     R f[Q];
-    gather_pre<ST, SBASE_STORES ? ADDR_ZPLANE : ADDR_CELL32>(to_global<T>(fin), g, a.fl, a.bcraw, a.use_bcraw, x, y, z, f);
+    gather_pre<ST, SBASE_STORES ? ADDR_ZPLANE : ADDR_CELL32, NT>(to_global<T>(fin), g, a.fl, a.bcraw, a.use_bcraw, x, y, z, f);
     if constexpr (stage_tau) {
         if (local_tau) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     }
@@ -1326,7 +1334,7 @@ __device__ double node_stress(const CellDesc& cd, PIn fin, int x, int y, int z) 
     using R = typename Pop<ST>::R;
     const CollideArgs& c = cd.c;
     double f[Q];
-    gather_pre<ST, ADDR_WIDE, double>(fin, c.g, c.fl, c.bcraw, c.use_bcraw, x, y, z, f);
+    gather_pre<ST, ADDR_WIDE, false, double>(fin, c.g, c.fl, c.bcraw, c.use_bcraw, x, y, z, f);
     const int A = cd.o.axisA, F = cd.o.axisF;
     double rho = 0, jx = 0, jy = 0, jz = 0, pi = 0;
     Unroll<0, Q>::run([&](auto qc) __attribute__((always_inline)) {

@@ -30,7 +30,7 @@ __device__ __forceinline__ unsigned pop_elem(int q, unsigned n, unsigned nint) {
     else return ((n / LAYOUT) * S + (unsigned)q) * LAYOUT + n % LAYOUT;
 }
 
-template <int W, int S, bool PULL, int LAYOUT = 0>
+template <int W, int S, bool PULL, int LAYOUT = 0, bool NTM = false>
 __global__ void __launch_bounds__(256) stream_probe_kernel(const float* __restrict__ in, float* __restrict__ out, unsigned nint,
                                                            int nx, int nxy, float a) {
     extern __shared__ float lds_pad[];
@@ -52,27 +52,31 @@ __global__ void __launch_bounds__(256) stream_probe_kernel(const float* __restri
             src = (s + W <= (long long)nint) ? (unsigned)s : n;
         }
         if constexpr (LAYOUT == 0) v[q] = *(const D3Q19_GLOBAL V*)plane_at(gin + (long long)q * nint, src);
+        else if constexpr (NTM) v[q] = __builtin_nontemporal_load(&gin[pop_elem<LAYOUT, S>(q, src, nint)]);
         else v[q] = gin[pop_elem<LAYOUT, S>(q, src, nint)];
     });
     Unroll<0, S>::run([&](auto qc) __attribute__((always_inline)) {
         constexpr int q = decltype(qc)::value;
         if constexpr (LAYOUT == 0) *(D3Q19_GLOBAL V*)plane_at(gout + (long long)q * nint, n) = v[q] * a;
+        else if constexpr (NTM) __builtin_nontemporal_store(v[q] * a, &gout[pop_elem<LAYOUT, S>(q, n, nint)]);
         else gout[pop_elem<LAYOUT, S>(q, n, nint)] = v[q] * a;
     });
 }
 
-template <int W, int S, bool PULL, int LAYOUT = 0>
+template <int W, int S, bool PULL, int LAYOUT = 0, bool NTM = false>
 static void probe_launch(const float* in, float* out, unsigned nint, int nx, int nxy, int lds, hipStream_t st) {
     const unsigned threads = nint / W;
     const dim3 grid((threads + 255) / 256);
-    hipLaunchKernelGGL((stream_probe_kernel<W, S, PULL, LAYOUT>), grid, dim3(256), lds, st, in, out, nint, nx, nxy, 0.999f);
+    hipLaunchKernelGGL((stream_probe_kernel<W, S, PULL, LAYOUT, NTM>), grid, dim3(256), lds, st, in, out, nint, nx, nxy, 0.999f);
 }
 
 typedef void (*probe_fn)(const float*, float*, unsigned, int, int, int, hipStream_t);
 
 // width 1 / 2 / 4: nodes per lane, planes; width -64 / -256: one node per lane, chunk layout of
-// 64 / 256 nodes
-static probe_fn pick(int streams, int width, int pull) {
+// 64 / 256 nodes, non-temporal accesses when nt is set
+static probe_fn pick(int streams, int width, int pull, bool nt) {
+    if (width == -64 && nt) return pull ? probe_launch<1, Q, true, 64, true> : probe_launch<1, Q, false, 64, true>;
+    if (width == -256 && nt) return pull ? probe_launch<1, Q, true, 256, true> : probe_launch<1, Q, false, 256, true>;
     if (width == -64) return pull ? probe_launch<1, Q, true, 64> : probe_launch<1, Q, false, 64>;
     if (width == -256) return pull ? probe_launch<1, Q, true, 256> : probe_launch<1, Q, false, 256>;
     if (streams == 1) return width == 4 ? probe_launch<4, 1, false> : (width == 2 ? probe_launch<2, 1, false> : probe_launch<1, 1, false>);
@@ -87,6 +91,12 @@ using namespace d3q19;
 extern "C" int d3q19_stream_probe(int device, int nx, int ny, int nz, int streams, int width, int pull, int lds_bytes, int launches,
                                   double* gbs, double* ms) {
     const bool chunked = width == -64 || width == -256;
+    if (pull < 0 || pull > 3 || ((pull & 2) && !chunked)) {
+        set_error("d3q19_stream_probe: pull is 0..3 (bit 1, non-temporal accesses, with the chunk layouts only)");
+        return D3Q19_E_INVALID;
+    }
+    const bool nt = (pull & 2) != 0;
+    pull &= 1;
     if (chunked && streams != Q) {
         set_error("d3q19_stream_probe: the chunk layouts hold 19 populations");
         return D3Q19_E_INVALID;
@@ -124,7 +134,7 @@ extern "C" int d3q19_stream_probe(int device, int nx, int ny, int nz, int stream
             break;
         }
         if (hipMemsetAsync(in, 0, bytes, st) != hipSuccess) { rc = D3Q19_E_DEVICE; break; }
-        const probe_fn fn = pick(streams, width, pull);
+        const probe_fn fn = pick(streams, width, pull, nt);
         for (int i = 0; i < 2; i++) fn(in, out, (unsigned)nint, nx, nx * ny, lds_bytes, st);
         (void)hipEventRecord(e0, st);
         for (int i = 0; i < launches; i++) fn(in, out, (unsigned)nint, nx, nx * ny, lds_bytes, st);

@@ -675,10 +675,12 @@ def clock():
     return lib().d3q19_clock()
 
 
-def stream_probe(nx, ny, nz, streams=19, width=1, pull=True, lds_bytes=0, launches=20, device=0):
+def stream_probe(nx, ny, nz, streams=19, width=1, pull=True, lds_bytes=0, launches=20, device=0, nontemporal=False):
     """Attainable bandwidth of the collide's access pattern without its arithmetic
-    (d3q19_stream_probe): returns (GB/s counting every byte read and written, ms per launch)."""
+    (d3q19_stream_probe): returns (GB/s counting every byte read and written, ms per launch).
+    nontemporal: non-temporal loads and stores (chunk layouts, width -64 / -256)."""
     g, ms = C.c_double(), C.c_double()
-    check(lib().d3q19_stream_probe(device, nx, ny, nz, streams, width, int(pull), lds_bytes, launches, C.byref(g), C.byref(ms)),
+    flags = (1 if pull else 0) | (2 if nontemporal else 0)
+    check(lib().d3q19_stream_probe(device, nx, ny, nz, streams, width, flags, lds_bytes, launches, C.byref(g), C.byref(ms)),
           "stream_probe")
     return g.value, ms.value

@@ -427,8 +427,10 @@ int d3q19_time_steps(d3q19_sim* sim, int t_first, int n, double* total_ms, doubl
  * streaming kernel with the collide+stream kernel's memory traffic and none of its arithmetic
  * on an nx*ny*nz box of fp32 planes: `streams` (19, or 1 for a plain copy) planes read and as many
  * written, `width` (1, 2, 4) consecutive nodes per lane, read at the D3Q19 pull offsets when
- * `pull` is set; width -64 / -256 keeps one node per lane with the 19 populations interleaved in
- * chunks of 64 / 256 nodes instead of planes; `lds_bytes` (0..65536) of dynamic LDS per
+ * bit 0 of `pull` is set; width -64 / -256 keeps one node per lane with the 19 populations
+ * interleaved in chunks of 64 / 256 nodes instead of planes, and with bit 1 of `pull` set reads
+ * and writes them with non-temporal loads and stores (as the collide does on lattices beyond the
+ * Infinity Cache); `lds_bytes` (0..65536) of dynamic LDS per
  * 256-thread block caps the resident blocks.  Allocates and frees its own buffers.  *gbs = bytes read + written per launch / mean
  * launch time (HIP events around `launches` launches), *ms = that mean (ms may be NULL). */
 int d3q19_stream_probe(int device, int nx, int ny, int nz, int streams, int width, int pull, int lds_bytes, int launches,
