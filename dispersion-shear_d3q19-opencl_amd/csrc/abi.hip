@@ -109,8 +109,9 @@ static void geom_from(Sim* s) {
     g.ny = s->Ny - 2;
     g.nz = s->Nz - 2;
     g.nint = g.nx * g.ny * g.nz;
-    // rows per population chunk (pop_index): the smallest power of two R dividing ny with R * nx a
-    // multiple of 64 nodes; failing that, the largest R dividing ny with R * nx below 128
+    // rows per population chunk (pop_index) of a batch member: the smallest power of two R dividing
+    // ny with R * nx a multiple of 64 nodes; failing that, the largest R dividing ny with R * nx below
+    // 128 (sim_init sets one row per chunk for a single simulation)
     int lr = -1;
     for (int k = 0; k <= 6 && g.ny % (1 << k) == 0; k++)
         if ((g.nx << k) % 64 == 0) {
@@ -546,6 +547,9 @@ int sim_init(Sim* s, int device, const int_param_struct* ip, const flp_param_str
     // multiple of 64 slots, stay at least 70% full (the padding slots idle, so that every
     // wavefront stays within one row's run); else it takes planar populations.  Chunks also need
     // the populations within the 32-bit lane offsets of that kernel.
+    // Single simulations keep one row per chunk: their collide tiles cover whole rows, and runs of
+    // several rows measured slower (184^3 nodes: 189 against 225 us per step) with no shape faster.
+    if (!shared_stream) set_chunked(s->g, 0);
     if (shared_stream) {
         const int nx = s->g.nx, pad = ((nx + 63) / 64) * 64;
         const bool fits = (long long)Q * s->g.nint * (long long)pop_size(s->storage) < (1LL << 32) &&
@@ -560,8 +564,12 @@ int sim_init(Sim* s, int device, const int_param_struct* ip, const flp_param_str
     const size_t nint = (size_t)s->g.nint;
     const size_t rs = real_size(s->storage);
     s->nbricks = s->g.bnx * s->g.bny * s->g.bnz;
-    // non-temporal population stores beyond the Infinity Cache (a batch decides for its cells)
-    s->nt_stores = !shared_stream && (long long)(2 * Q * pop_size(s->storage) * nint) > INFINITY_CACHE_BYTES;
+    // non-temporal population stores beyond the Infinity Cache, loads too beyond NT_LOAD_BYTES where
+    // every row's run is a whole number of wavefronts (a batch decides for its cells).  On rows of
+    // other lengths non-temporal loads measured slower at every size (200^3 nodes, 1.2 GB: 257
+    // against 231 us per step; 256x256x128, rows of 256: 211 against 221 us)
+    s->nt_stores = shared_stream ? 0 : nt_level((long long)(2 * Q * pop_size(s->storage) * nint));
+    if (s->nt_stores == 2 && s->g.nx % 64 != 0) s->nt_stores = 1;
     hipError_t e = hipSuccess;
     if (shared_stream) {
         s->stream = shared_stream;

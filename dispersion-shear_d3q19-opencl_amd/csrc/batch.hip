@@ -154,7 +154,7 @@ static int upload_tables(Batch* b) {
     long long pop_total = 0;
     for (int c = 0; c < b->max_cells; c++)
         if (const Sim* s = b->cells[c].sim) pop_total += 2LL * Q * s->pop_bytes * s->g.nint;
-    const int nt = pop_total > INFINITY_CACHE_BYTES ? 1 : 0;
+    const int nt = nt_level(pop_total);
     for (int c = 0; c < b->max_cells; c++) {
         Member& m = b->cells[c];
         if (!m.sim) {

@@ -18,8 +18,9 @@ using namespace d3q19;
 namespace {
 
 constexpr char MAGIC[8] = {'D', '3', 'Q', '1', '9', 'C', 'K', '1'};
-constexpr uint32_t VERSION = 5;  // 2: u stored with its components interleaved per node; 3: displacement watchdog;
-                                 // 4: per-node force tags; 5: chunk-interleaved populations
+constexpr uint32_t VERSION = 6;  // 2: u stored with its components interleaved per node; 3: displacement watchdog;
+                                 // 4: per-node force tags; 5: chunk-interleaved populations; 6: one row per
+                                 // chunk for single simulations
 
 struct Header {
     char magic[8];

@@ -41,6 +41,12 @@ constexpr int BRICK = 4;                              // brick edge (nodes)
 constexpr int IBM_WAVES = 4;                          // markers (wavefronts) per IBM block
 constexpr int TILE_MARKERS = 8;                       // markers per block of the LDS-staged spreading
 constexpr long long INFINITY_CACHE_BYTES = 256ll << 20;  // MI355X's last-level (memory-side) cache
+constexpr long long NT_LOAD_BYTES = 1024ll << 20;        // population buffers from which non-temporal loads pay
+// Non-temporal population accesses of the collide for population buffers of `bytes` (both
+// lattices): none while they fit the Infinity Cache; stores beyond it; stores and loads from
+// NT_LOAD_BYTES (nearer the cache size the loads give up residency the ordinary ones keep; a
+// single simulation takes the loads only on rows of a multiple of 64 nodes).
+inline int nt_level(long long bytes) { return bytes > NT_LOAD_BYTES ? 2 : (bytes > INFINITY_CACHE_BYTES ? 1 : 0); }
 constexpr long long IBM_ROWS_MIN_MARKERS = 16384;        // markers from which four per wavefront pay
 constexpr int IBM_LARGE_GROUP_MIN_BLOCKS = 512;          // blocks needed to take groups larger than TILE_MARKERS
 constexpr int TILE_MARKERS_MAX = 64;                  // ... of a single simulation's Peskin-4 groups whose boxes fit
@@ -245,7 +251,7 @@ struct CollideArgs {
     int nbx, nby;          // tiles per plane
     int nblk;              // tiles of this launch
     int z0;                // first z plane of this launch's tiles (a z-slab of the lattice)
-    int nt_stores;         // population stores non-temporal (lattices beyond the Infinity Cache)
+    int nt_stores;         // non-temporal population accesses (nt_level): 1 stores, 2 stores and loads
 };
 
 struct IbmArgs {
@@ -433,7 +439,7 @@ struct Sim {
     int tile_nodes = 0;        // the largest group box (nodes) within the LDS cap: the LDS the IBM launches reserve
     int group_markers = TILE_MARKERS;  // markers per group of the LDS-staged spreading
     bool ibm_rows = false;     // Peskin-4 markers run four to a wavefront (many markers); else one per wavefront
-    bool nt_stores = false;    // population stores non-temporal (both population buffers exceed the Infinity Cache)
+    int nt_stores = 0;         // non-temporal population accesses of the collide (nt_level)
     double mark_rin = 0.0, mark_rout = 0.0;  // shortest / longest marker offset, with a margin
     long long h_epoch = 0;     // host count of committed epochs (one per step)
     int tofs = 0;              // step number t of the contacts = committed epoch + tofs
@@ -538,7 +544,7 @@ struct BatchLaunch {
     int grid_collide_vt;   // blocks of the collide kernel over the per-node-tau cells' tiles
     int grid_sphere;       // persistent blocks of the sphere kernels
     int grid_ibm;          // persistent blocks (one marker group each per iteration) of the IBM kernel
-    int nt_stores;         // non-temporal population stores (members' lattices beyond the Infinity Cache)
+    int nt_stores;         // non-temporal population accesses (nt_level of the members' lattices together)
 };
 int launch_collide_batch(const BatchLaunch& L, int sp, int write_u_all, int probe, hipStream_t st);
 // a batched step's collide with the spheres' turn in the leading blocks of its first launch

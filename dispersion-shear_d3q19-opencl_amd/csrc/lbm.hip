@@ -45,10 +45,14 @@ template <bool NT, class T, class PT> __device__ __forceinline__ void st_pop(PT 
     if constexpr (NT) __builtin_nontemporal_store(v, plane_at(p, off));
     else *plane_at(p, off) = v;
 }
-// The gather's population loads are non-temporal too on those lattices: each stored value is read
-// by the one node it streams to, and the stream otherwise evicts the u, force and tau lines that
-// the IBM pass and the next step read (256x256x128 Casson workload, fp32: step kernel 254 -> 227
-// us and the IBM pass after it 58 -> 51 us; fp16: IBM pass 57 -> 53 us, kernel unchanged).
+// The gather's population loads are non-temporal too on lattices well beyond the cache (nt_level):
+// each stored value is read by the one node it streams to, and the stream otherwise evicts the u,
+// force and tau lines that the IBM pass and the next step read (256x256x128 Casson workload, fp32:
+// step kernel 254 -> 227 us and the IBM pass after it 58 -> 51 us; fp16: IBM pass 57 -> 53 us,
+// kernel unchanged).  Below NT_LOAD_BYTES they cost the residency that ordinary loads keep (142^3
+// nodes, 415 MB: 104 against 87 us per step; 184^3, 903 MB: 240 against 223 us), and on rows that
+// are not a whole number of wavefronts they cost at every size (200^3, 1.2 GB: 257 against 231 us),
+// so a single simulation takes them only on rows of a multiple of 64 nodes.
 template <bool NT, class PT> __device__ __forceinline__ auto ld_pop(PT p, unsigned off) {
     if constexpr (NT) return __builtin_nontemporal_load(plane_at(p, off));
     else return *plane_at(p, off);
@@ -407,7 +411,7 @@ __device__ __forceinline__ R strain_rate(const R* ss, R ux, R uy, R uz, R gx, R 
 // when a.vt is set (batched cells of mixed rheology).  probe = 1 (autotuning trials) leaves the
 // pending spread force, the tau field and u untouched so a trial launch does not change the state
 // a later step reads (its populations go to the lattice the next step overwrites).
-template <int ST, int COLL, int VT, bool NT, bool SBASE_STORES = true>
+template <int ST, int COLL, int VT, int NT, bool SBASE_STORES = true>
 __device__ __forceinline__ void collide_node(const CollideArgs& a, const void* fin, void* fout, int write_u_all, int x, int y,
                                              int z, int probe) {
     using P = Pop<ST>;
@@ -450,7 +454,7 @@ __device__ __forceinline__ void collide_node(const CollideArgs& a, const void* f
     }
 
     R f[Q];
-    gather_pre<ST, SBASE_STORES ? ADDR_ZPLANE : ADDR_CELL32, NT>(to_global<T>(fin), g, a.fl, a.bcraw, a.use_bcraw, x, y, z, f);
+    gather_pre<ST, SBASE_STORES ? ADDR_ZPLANE : ADDR_CELL32, NT == 2>(to_global<T>(fin), g, a.fl, a.bcraw, a.use_bcraw, x, y, z, f);
     if constexpr (stage_tau) {
         if (local_tau) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     }
@@ -624,18 +628,18 @@ __device__ __forceinline__ void collide_node(const CollideArgs& a, const void* f
         if constexpr (SBASE_STORES) asm volatile("" : "+v"(ns));
         Unroll<0, Q>::run([&](auto qc) __attribute__((always_inline)) {
             constexpr int q = decltype(qc)::value;
-            P::template st_sr<q, NT>(out + q * g.qs, ns, fo[q], sr_bits<q, P::SR_BITS>(h));
+            P::template st_sr<q, NT != 0>(out + q * g.qs, ns, fo[q], sr_bits<q, P::SR_BITS>(h));
         });
     } else {
         if constexpr (SBASE_STORES) asm volatile("" : "+v"(ns));
         Unroll<0, Q>::run([&](auto qc) __attribute__((always_inline)) {
             constexpr int q = decltype(qc)::value;
-            P::template st<q, NT>(out + q * g.qs, ns, fo[q]);
+            P::template st<q, NT != 0>(out + q * g.qs, ns, fo[q]);
         });
     }
 }
 
-template <int ST, int COLL, int VT, bool NT>
+template <int ST, int COLL, int VT, int NT>
 __global__ void __launch_bounds__(256) collide_kernel(CollideArgs a) {
     // Tiles are taken in dispatch order (x fastest, then y, then z).  An XCD-contiguous remap of
     // the tiles (each XCD owning a z-slab) measured 2-7% slower on MI355X for 8-34 M node lattices:
@@ -660,7 +664,7 @@ __global__ void __launch_bounds__(256) collide_kernel(CollideArgs a) {
 // sphere block writes is read by a collide block of the same launch: the collide reads the marks of
 // this epoch and the pending force with its tags (written by the last IBM pass), the turn writes the
 // marks of the next epoch (another array) and the other position and velocity buffers.
-template <int ST, int COLL, int VT, bool NT>
+template <int ST, int COLL, int VT, int NT>
 __device__ __forceinline__ void step_body(const CollideArgs& a, const PartArgs& p, int nsph, int do_post) {
     if ((int)blockIdx.x < nsph) {
         sphere_turn(p, blockIdx.x, do_post, p.epochs[0] + 2u);
@@ -679,7 +683,7 @@ __device__ __forceinline__ void step_body(const CollideArgs& a, const PartArgs& 
     collide_node<ST, COLL, VT, NT>(a, a.fin, a.fout, a.write_u_all, x, y, z, 0);
 }
 
-template <int ST, int COLL, int VT, bool NT>
+template <int ST, int COLL, int VT, int NT>
 __global__ void __launch_bounds__(256) step_kernel(CollideArgs a, PartArgs p, int nsph, int do_post) {
     step_body<ST, COLL, VT, NT>(a, p, nsph, do_post);
 }
@@ -687,7 +691,7 @@ __global__ void __launch_bounds__(256) step_kernel(CollideArgs a, PartArgs p, in
 // The same step with a register budget: 5 waves per SIMD for the BGK form in fp32 (85 VGPRs, no
 // spill; the 256x256x128 Casson workload: 254.2-255.5 us against 257.9-258.9 compiled for 6 waves,
 // 80 VGPRs, in alternating runs) ...
-template <int ST, int COLL, int VT, bool NT>
+template <int ST, int COLL, int VT, int NT>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) step_kernel_w5(CollideArgs a, PartArgs p, int nsph,
                                                                                             int do_post) {
     step_body<ST, COLL, VT, NT>(a, p, nsph, do_post);
@@ -695,7 +699,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) s
 
 // ... and 7 waves per SIMD (72 VGPRs, 20 bytes spilled) for the 16-bit storages (fp16 on the same
 // workload: 154.1-155.9 against 159.8-162.2 us at 6 waves in alternating runs)
-template <int ST, int COLL, int VT, bool NT>
+template <int ST, int COLL, int VT, int NT>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) step_kernel_w7(CollideArgs a, PartArgs p, int nsph,
                                                                                             int do_post) {
     step_body<ST, COLL, VT, NT>(a, p, nsph, do_post);
@@ -731,7 +735,7 @@ __device__ __forceinline__ void split_slot(const CellDesc& cd, int n, int& x, in
 // VT selects the tile range: 0 the constant-tau cells' tiles [0, ntiles_ct), 1 the per-node-tau
 // cells' tiles [ntiles_ct, ntiles).  Two specialised launches keep the constant-tau cells free of
 // the per-node-tau code's registers (occupancy).
-template <int ST, int COLL, int VT, bool NT>
+template <int ST, int COLL, int VT, int NT>
 __device__ __forceinline__ void collide_batch_tiles(const BatchTables& t, const CellDesc* __restrict__ cells, int sp, int write_u_all,
                                                     int probe, int first, int stride) {
     const int lo = VT ? t.cnt->ntiles_ct : 0;
@@ -752,7 +756,7 @@ __device__ __forceinline__ void collide_batch_tiles(const BatchTables& t, const 
     }
 }
 
-template <int ST, int COLL, int VT, bool NT>
+template <int ST, int COLL, int VT, int NT>
 __global__ void __launch_bounds__(256) collide_batch_kernel(BatchTables t, const CellDesc* __restrict__ cells, int sp, int write_u_all,
                                                             int probe) {
     collide_batch_tiles<ST, COLL, VT, NT>(t, cells, sp, write_u_all, probe, blockIdx.x, gridDim.x);
@@ -762,7 +766,7 @@ __global__ void __launch_bounds__(256) collide_batch_kernel(BatchTables t, const
 // (as step_body's sphere blocks: x(t), v(t) from p[par ^ sp], x(t+1), v(t+1) into the other
 // buffer, the marks of the epoch after this one); the other blocks walk the collide tiles.  As in
 // step_body, nothing a sphere block writes is read by a collide block of the same launch.
-template <int ST, int COLL, int VT, bool NT>
+template <int ST, int COLL, int VT, int NT>
 __device__ __forceinline__ void step_batch_body(const BatchTables& t, const CellDesc* __restrict__ cells, const int4* __restrict__ sph,
                                                 int sp, int write_u_all, int nturn, int do_post) {
     if ((int)blockIdx.x < nturn) {
@@ -779,7 +783,7 @@ __device__ __forceinline__ void step_batch_body(const BatchTables& t, const Cell
     collide_batch_tiles<ST, COLL, VT, NT>(t, cells, sp, write_u_all, 0, (int)blockIdx.x - nturn, (int)gridDim.x - nturn);
 }
 
-template <int ST, int COLL, int VT, bool NT>
+template <int ST, int COLL, int VT, int NT>
 __global__ void __launch_bounds__(256) step_batch_kernel(BatchTables t, const CellDesc* __restrict__ cells, const int4* __restrict__ sph,
                                                          int sp, int write_u_all, int nturn, int do_post) {
     step_batch_body<ST, COLL, VT, NT>(t, cells, sph, sp, write_u_all, nturn, do_post);
@@ -789,13 +793,13 @@ __global__ void __launch_bounds__(256) step_batch_kernel(BatchTables t, const Ce
 // bytes), for the batched BGK collide in fp16 / bf16 storage, and of 5 waves (96 VGPRs, 8 bytes spilled) in fp32:
 // on the 24-cell C5 batch (fp32) 0.748 / 0.735 against 0.778 / 0.760 ms per step at 6, on the
 // 64-cell fp16 ensemble 0.665 against 0.642 ms (so fp16 keeps 6)
-template <int ST, int COLL, int VT, bool NT>
+template <int ST, int COLL, int VT, int NT>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6)))
 step_batch_kernel_w6(BatchTables t, const CellDesc* __restrict__ cells, const int4* __restrict__ sph, int sp, int write_u_all, int nturn,
                      int do_post) {
     step_batch_body<ST, COLL, VT, NT>(t, cells, sph, sp, write_u_all, nturn, do_post);
 }
-template <int ST, int COLL, int VT, bool NT>
+template <int ST, int COLL, int VT, int NT>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5)))
 step_batch_kernel_w5(BatchTables t, const CellDesc* __restrict__ cells, const int4* __restrict__ sph, int sp, int write_u_all, int nturn,
                      int do_post) {
@@ -804,16 +808,16 @@ step_batch_kernel_w5(BatchTables t, const CellDesc* __restrict__ cells, const in
 
 // With events ev0/ev1, they are recorded at the kernel's own start and end (hipExtLaunchKernelGGL),
 // so their interval is the kernel's duration without the dispatch gap before it.
+template <int ST, int COLL, int VT, int NT>
+static void launch_one_nt(const CollideArgs& a, int threads, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1) {
+    if (ev0) hipExtLaunchKernelGGL((collide_kernel<ST, COLL, VT, NT>), dim3(a.nblk), dim3(threads), 0, st, ev0, ev1, 0, a);
+    else hipLaunchKernelGGL((collide_kernel<ST, COLL, VT, NT>), dim3(a.nblk), dim3(threads), 0, st, a);
+}
 template <int ST, int COLL, int VT>
 static void launch_one(const CollideArgs& a, int threads, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1) {
-    const bool nt = a.nt_stores;
-    if (ev0) {
-        if (nt) hipExtLaunchKernelGGL((collide_kernel<ST, COLL, VT, true>), dim3(a.nblk), dim3(threads), 0, st, ev0, ev1, 0, a);
-        else hipExtLaunchKernelGGL((collide_kernel<ST, COLL, VT, false>), dim3(a.nblk), dim3(threads), 0, st, ev0, ev1, 0, a);
-        return;
-    }
-    if (nt) hipLaunchKernelGGL((collide_kernel<ST, COLL, VT, true>), dim3(a.nblk), dim3(threads), 0, st, a);
-    else hipLaunchKernelGGL((collide_kernel<ST, COLL, VT, false>), dim3(a.nblk), dim3(threads), 0, st, a);
+    if (a.nt_stores == 2) launch_one_nt<ST, COLL, VT, 2>(a, threads, st, ev0, ev1);
+    else if (a.nt_stores == 1) launch_one_nt<ST, COLL, VT, 1>(a, threads, st, ev0, ev1);
+    else launch_one_nt<ST, COLL, VT, 0>(a, threads, st, ev0, ev1);
 }
 
 template <int ST>
@@ -854,7 +858,7 @@ CollideArgs make_collide_args(Sim* s, const void* fin, void* fout, int write_u_a
     a.nby = (s->g.ny + a.by - 1) / a.by;
     a.nblk = a.nbx * a.nby * s->g.nz;
     a.z0 = 0;
-    a.nt_stores = s->nt_stores ? 1 : 0;
+    a.nt_stores = s->nt_stores;
     return a;
 }
 
@@ -881,21 +885,23 @@ static void launch_maybe_timed(K kernel, dim3 grid, dim3 block, hipStream_t st, 
     else hipLaunchKernelGGL(kernel, grid, block, 0, st, args...);
 }
 
+template <int ST, int COLL, int VT, int NT>
+static void launch_fused_nt(const CollideArgs& a, const PartArgs& p, int nsph, int do_post, int threads, hipStream_t st,
+                            hipEvent_t ev0, hipEvent_t ev1) {
+    const dim3 grid(a.nblk + nsph);
+    if constexpr (COLL == D3Q19_COLL_BGK && (ST == D3Q19_STORE_FP16_SHIFT || ST == D3Q19_STORE_BF16_SHIFT))
+        launch_maybe_timed(step_kernel_w7<ST, COLL, VT, NT>, grid, dim3(threads), st, ev0, ev1, a, p, nsph, do_post);
+    else if constexpr (COLL == D3Q19_COLL_BGK && ST == D3Q19_STORE_FP32)
+        launch_maybe_timed(step_kernel_w5<ST, COLL, VT, NT>, grid, dim3(threads), st, ev0, ev1, a, p, nsph, do_post);
+    else
+        launch_maybe_timed(step_kernel<ST, COLL, VT, NT>, grid, dim3(threads), st, ev0, ev1, a, p, nsph, do_post);
+}
 template <int ST, int COLL, int VT>
 static void launch_fused_one(const CollideArgs& a, const PartArgs& p, int nsph, int do_post, int threads, hipStream_t st,
                              hipEvent_t ev0, hipEvent_t ev1) {
-    const bool nt = a.nt_stores;
-    const dim3 grid(a.nblk + nsph);
-    if constexpr (COLL == D3Q19_COLL_BGK && (ST == D3Q19_STORE_FP16_SHIFT || ST == D3Q19_STORE_BF16_SHIFT)) {
-        if (nt) launch_maybe_timed(step_kernel_w7<ST, COLL, VT, true>, grid, dim3(threads), st, ev0, ev1, a, p, nsph, do_post);
-        else launch_maybe_timed(step_kernel_w7<ST, COLL, VT, false>, grid, dim3(threads), st, ev0, ev1, a, p, nsph, do_post);
-    } else if constexpr (COLL == D3Q19_COLL_BGK && ST == D3Q19_STORE_FP32) {
-        if (nt) launch_maybe_timed(step_kernel_w5<ST, COLL, VT, true>, grid, dim3(threads), st, ev0, ev1, a, p, nsph, do_post);
-        else launch_maybe_timed(step_kernel_w5<ST, COLL, VT, false>, grid, dim3(threads), st, ev0, ev1, a, p, nsph, do_post);
-    } else {
-        if (nt) launch_maybe_timed(step_kernel<ST, COLL, VT, true>, grid, dim3(threads), st, ev0, ev1, a, p, nsph, do_post);
-        else launch_maybe_timed(step_kernel<ST, COLL, VT, false>, grid, dim3(threads), st, ev0, ev1, a, p, nsph, do_post);
-    }
+    if (a.nt_stores == 2) launch_fused_nt<ST, COLL, VT, 2>(a, p, nsph, do_post, threads, st, ev0, ev1);
+    else if (a.nt_stores == 1) launch_fused_nt<ST, COLL, VT, 1>(a, p, nsph, do_post, threads, st, ev0, ev1);
+    else launch_fused_nt<ST, COLL, VT, 0>(a, p, nsph, do_post, threads, st, ev0, ev1);
 }
 
 template <int ST>
@@ -943,7 +949,7 @@ int launch_step_fused(Sim* s, const void* fin, void* fout, int write_u_all, int 
     return 0;
 }
 
-template <int ST, int COLL, bool NT>
+template <int ST, int COLL, int NT>
 static void collide_batch_nt(const BatchLaunch& L, int sp, int write_u_all, int probe, hipStream_t st) {
     if (L.grid_collide > 0)
         hipLaunchKernelGGL((collide_batch_kernel<ST, COLL, 0, NT>), dim3(L.grid_collide), dim3(L.tile), 0, st, L.t, L.t.cells, sp,
@@ -955,8 +961,9 @@ static void collide_batch_nt(const BatchLaunch& L, int sp, int write_u_all, int 
 
 template <int ST, int COLL>
 static void collide_batch_coll(const BatchLaunch& L, int sp, int write_u_all, int probe, hipStream_t st) {
-    if (L.nt_stores) collide_batch_nt<ST, COLL, true>(L, sp, write_u_all, probe, st);
-    else collide_batch_nt<ST, COLL, false>(L, sp, write_u_all, probe, st);
+    if (L.nt_stores == 2) collide_batch_nt<ST, COLL, 2>(L, sp, write_u_all, probe, st);
+    else if (L.nt_stores == 1) collide_batch_nt<ST, COLL, 1>(L, sp, write_u_all, probe, st);
+    else collide_batch_nt<ST, COLL, 0>(L, sp, write_u_all, probe, st);
 }
 
 template <int ST>
@@ -966,7 +973,7 @@ static void collide_batch_st(const BatchLaunch& L, int sp, int write_u_all, int 
 }
 
 // the step's collide launches with the sphere blocks in the first of them
-template <int ST, int COLL, int VT, bool NT>
+template <int ST, int COLL, int VT, int NT>
 static void step_batch_one(const BatchLaunch& L, int grid, int nturn, int sp, int write_u_all, int do_post, hipStream_t st) {
     const dim3 g(grid + nturn), blk(L.tile);
     if constexpr (COLL == D3Q19_COLL_BGK && ST == D3Q19_STORE_FP32)
@@ -977,7 +984,7 @@ static void step_batch_one(const BatchLaunch& L, int grid, int nturn, int sp, in
         hipLaunchKernelGGL((step_batch_kernel<ST, COLL, VT, NT>), g, blk, 0, st, L.t, L.t.cells, L.t.sph, sp, write_u_all, nturn, do_post);
 }
 
-template <int ST, int COLL, bool NT>
+template <int ST, int COLL, int NT>
 static void step_batch_nt(const BatchLaunch& L, int sp, int write_u_all, int do_post, hipStream_t st) {
     const int nturn = L.grid_sphere;
     if (L.grid_collide > 0 || L.grid_collide_vt == 0) step_batch_one<ST, COLL, 0, NT>(L, L.grid_collide, nturn, sp, write_u_all, do_post, st);
@@ -988,12 +995,15 @@ static void step_batch_nt(const BatchLaunch& L, int sp, int write_u_all, int do_
 template <int ST>
 static void step_batch_st(const BatchLaunch& L, int sp, int write_u_all, int do_post, hipStream_t st) {
     const bool bgk = L.collision == D3Q19_COLL_BGK;
-    if (L.nt_stores) {
-        if (bgk) step_batch_nt<ST, D3Q19_COLL_BGK, true>(L, sp, write_u_all, do_post, st);
-        else step_batch_nt<ST, D3Q19_COLL_MRT, true>(L, sp, write_u_all, do_post, st);
+    if (L.nt_stores == 2) {
+        if (bgk) step_batch_nt<ST, D3Q19_COLL_BGK, 2>(L, sp, write_u_all, do_post, st);
+        else step_batch_nt<ST, D3Q19_COLL_MRT, 2>(L, sp, write_u_all, do_post, st);
+    } else if (L.nt_stores == 1) {
+        if (bgk) step_batch_nt<ST, D3Q19_COLL_BGK, 1>(L, sp, write_u_all, do_post, st);
+        else step_batch_nt<ST, D3Q19_COLL_MRT, 1>(L, sp, write_u_all, do_post, st);
     } else {
-        if (bgk) step_batch_nt<ST, D3Q19_COLL_BGK, false>(L, sp, write_u_all, do_post, st);
-        else step_batch_nt<ST, D3Q19_COLL_MRT, false>(L, sp, write_u_all, do_post, st);
+        if (bgk) step_batch_nt<ST, D3Q19_COLL_BGK, 0>(L, sp, write_u_all, do_post, st);
+        else step_batch_nt<ST, D3Q19_COLL_MRT, 0>(L, sp, write_u_all, do_post, st);
     }
 }
 

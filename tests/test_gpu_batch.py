@@ -69,12 +69,13 @@ def test_batch_population_layouts_match_single_simulations():
     """Batch members take one of three population layouts by their row length: chunks walked in
     node order (rows of 64: runs that are whole wavefronts), chunks walked in padded row slots
     (rows of 50 and 100: padded to 64 and 128 slots, the padding lanes idle) and planes (rows of
-    20).  Each matches the same cell as a single simulation (chunked, walked in row tiles)."""
+    20); rows of 32 go two to a chunk.  Each matches the same cell as a single simulation (one row per chunk, walked in row tiles)."""
     cells = [
         shear_cell((50, 50, 24), np_=1, diam=6.0, markers=128, uw=0.02, seed=6),
         shear_cell((64, 32, 16), uw=0.01, visc=2, vp=(0.02, 0.7, 0, 0), tau=0.9),
         shear_cell((20, 20, 20), np_=1, diam=5.0, markers=128, uw=0.03, seed=7),
         shear_cell((100, 40, 16), np_=2, diam=6.0, markers=160, uw=0.015, visc=4, vp=(1e-5, 0.0667, 1e4, 0), tau=0.7, seed=8),
+        shear_cell((32, 32, 24), np_=1, diam=6.0, markers=128, uw=0.02, seed=9),
     ]
     b = d3q19.Batch(storage=0, collision=d3q19.COLL_BGK, ibm=d3q19.IBM_PESKIN4, arena_bytes=1 << 30)
     ids = [b.add(c) for c in cells]

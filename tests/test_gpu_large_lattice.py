@@ -7,8 +7,8 @@ The collide addresses a population as a scalar base (its run in the node's z-pla
 Check, independent of size: a sphere-free shear cell, periodic in x and y with walls normal to z,
 started at rest, stays a function of z alone, node for node, so its per-plane mean velocity must
 equal bitwise that of an 8 x 8 x 256 cell with the same parameters (the plane means sum 2^18 and
-2^6 identical values in FP64, exactly).  The two lattices also store their populations in
-different chunk shapes (runs of one 512-node row, and of eight 8-node rows).
+2^6 identical values in FP64, exactly).  The two lattices store their populations in runs of one
+row each, of 512 and of 8 nodes.
 """
 import numpy as np
 import pytest
