@@ -170,6 +170,15 @@ static void default_block(Sim* s) {
     s->block[0] = bx;
     s->block[1] = by;
     s->block[2] = 1;
+    // Rows of 257-511 nodes (not a multiple of 64) take two row tiles, the second mostly idle (a
+    // 288-node row: 32 of 256 lanes).  The default tile there is 256 consecutive nodes of the plane
+    // in node order, across rows.  Measured step kernel, fp32: 288^2x100 263 -> 218 us, 300^2x90
+    // 361 -> 289, 360^2x64 317 -> 258, 400^2x50 248 -> 213, 480^2x36 232 -> 219.  Rows of at most
+    // 256 nodes lost with it (200^3 231 -> 266 us, 142^3 89 -> 101), rows of 520-700 nodes 2%.
+    // Planes stay below 2^24 nodes, where the float split of the node index is exact to +-1.
+    const int nx = s->g.nx;
+    s->packed_tiles = (s->opt.block[0] <= 0 && nx > 256 && nx < 512 && nx % 64 != 0 &&
+                       (long long)nx * s->g.ny < (1LL << 24)) ? 1 : 0;
 }
 
 static void* cur_f(Sim* s) { return s->parity == 0 ? s->fA : s->fB; }

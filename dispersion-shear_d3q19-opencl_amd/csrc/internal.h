@@ -249,6 +249,8 @@ struct CollideArgs {
     int vt;                // per-node tau (runtime form of the kernels' VT template argument)
     int bx, by;            // block tile
     int nbx, nby;          // tiles per plane
+    int packed;            // 1: a tile is bx consecutive nodes of its plane in node order (rows of any length)
+    float inv_nx;          // 1 / nx, for the node-order split of packed tiles
     int nblk;              // tiles of this launch
     int z0;                // first z plane of this launch's tiles (a z-slab of the lattice)
     int nt_stores;         // non-temporal population accesses (nt_level): 1 stores, 2 stores and loads
@@ -465,6 +467,7 @@ struct Sim {
     hipStream_t aux = nullptr;         // the IBM pass's stream
     hipEvent_t sev[MAX_SLABS + 1] = {};  // slab k collided; aux done
     int block[3] = {64, 4, 1};
+    int packed_tiles = 0;              // collide tiles of 256 consecutive nodes of a plane (CollideArgs::packed)
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     // batch membership: device memory comes from the batch's arena, the batch's stream and epoch
     // counter are used, and the batch owns neither

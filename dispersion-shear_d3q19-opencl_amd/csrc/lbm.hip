@@ -639,22 +639,40 @@ __device__ __forceinline__ void collide_node(const CollideArgs& a, const void* f
     }
 }
 
+// Tile L of a single simulation's collide -> this lane's node; false past the lattice.  A tile is a
+// bx x by block of one plane (x fastest), or with a.packed bx consecutive nodes of the plane in node
+// order (a wavefront may then span two rows: its accesses are two contiguous stretches per
+// population, each row its own chunk).  The float quotient is exact to +-1 for planes below 2^24
+// nodes and corrected.
+__device__ __forceinline__ bool tile_node(const CollideArgs& a, int L, int& x, int& y, int& z) {
+    const int per_plane = a.nbx * a.nby;
+    const int zl = L / per_plane;  // plane within this launch (a z-slab from plane a.z0)
+    z = a.z0 + zl;
+    const int rem = L - zl * per_plane;
+    if (a.packed) {
+        const int i = rem * a.bx + (int)threadIdx.x;
+        int yy = (int)((float)i * a.inv_nx);
+        int xx = i - yy * a.g.nx;
+        if (xx < 0) { yy--; xx += a.g.nx; } else if (xx >= a.g.nx) { yy++; xx -= a.g.nx; }
+        x = xx;
+        y = yy;
+    } else {
+        const int tyb = rem / a.nbx;
+        const int txb = rem - tyb * a.nbx;
+        x = txb * a.bx + (int)(threadIdx.x % a.bx);
+        y = tyb * a.by + (int)(threadIdx.x / a.bx);
+    }
+    return x >= 0 && x < a.g.nx && y < a.g.ny && z < a.g.nz && z >= 0;
+}
+
 template <int ST, int COLL, int VT, int NT>
 __global__ void __launch_bounds__(256) collide_kernel(CollideArgs a) {
     // Tiles are taken in dispatch order (x fastest, then y, then z).  An XCD-contiguous remap of
     // the tiles (each XCD owning a z-slab) measured 2-7% slower on MI355X for 8-34 M node lattices:
     // with plain order all XCDs stream neighbouring addresses and the shared Infinity Cache serves
     // the y/z halo rows.
-    const int L = blockIdx.x;
-    const int per_plane = a.nbx * a.nby;
-    const int zl = L / per_plane;  // plane within this launch
-    const int z = a.z0 + zl;
-    const int rem = L - zl * per_plane;
-    const int tyb = rem / a.nbx;
-    const int txb = rem - tyb * a.nbx;
-    const int x = txb * a.bx + (int)(threadIdx.x % a.bx);
-    const int y = tyb * a.by + (int)(threadIdx.x / a.bx);
-    if (x >= a.g.nx || y >= a.g.ny || z >= a.g.nz || z < 0) return;
+    int x, y, z;
+    if (!tile_node(a, blockIdx.x, x, y, z)) return;
     collide_node<ST, COLL, VT, NT>(a, a.fin, a.fout, a.write_u_all, x, y, z, 0);
 }
 
@@ -670,16 +688,8 @@ __device__ __forceinline__ void step_body(const CollideArgs& a, const PartArgs& 
         sphere_turn(p, blockIdx.x, do_post, p.epochs[0] + 2u);
         return;
     }
-    const int L = (int)blockIdx.x - nsph;
-    const int per_plane = a.nbx * a.nby;
-    const int zl = L / per_plane;  // plane within this launch (a z-slab from plane a.z0)
-    const int z = a.z0 + zl;
-    const int rem = L - zl * per_plane;
-    const int tyb = rem / a.nbx;
-    const int txb = rem - tyb * a.nbx;
-    const int x = txb * a.bx + (int)(threadIdx.x % a.bx);
-    const int y = tyb * a.by + (int)(threadIdx.x / a.bx);
-    if (x >= a.g.nx || y >= a.g.ny || z >= a.g.nz || z < 0) return;
+    int x, y, z;
+    if (!tile_node(a, (int)blockIdx.x - nsph, x, y, z)) return;
     collide_node<ST, COLL, VT, NT>(a, a.fin, a.fout, a.write_u_all, x, y, z, 0);
 }
 
@@ -852,10 +862,19 @@ CollideArgs make_collide_args(Sim* s, const void* fin, void* fout, int write_u_a
     a.use_bcraw = s->bcraw_valid;
     a.has_force = (s->np > 0 && s->facc != nullptr && s->ftag != nullptr) ? 1 : 0;
     a.vt = s->opt.variable_viscosity ? 1 : 0;
-    a.bx = s->block[0];
-    a.by = s->block[1];
-    a.nbx = (s->g.nx + a.bx - 1) / a.bx;
-    a.nby = (s->g.ny + a.by - 1) / a.by;
+    a.packed = s->packed_tiles;
+    a.inv_nx = 1.0f / (float)s->g.nx;
+    if (a.packed) {
+        a.bx = 256;
+        a.by = 1;
+        a.nbx = (int)(((long long)s->g.nx * s->g.ny + 255) / 256);
+        a.nby = 1;
+    } else {
+        a.bx = s->block[0];
+        a.by = s->block[1];
+        a.nbx = (s->g.nx + a.bx - 1) / a.bx;
+        a.nby = (s->g.ny + a.by - 1) / a.by;
+    }
     a.nblk = a.nbx * a.nby * s->g.nz;
     a.z0 = 0;
     a.nt_stores = s->nt_stores;

@@ -244,16 +244,18 @@ def test_marked_bricks_cover_every_stencil_across_periodic_faces(ibm):
     assert ep < 1e-9 and ev < 1e-9
 
 
-@pytest.mark.parametrize("coll,storage,visc", [(0, 0, 1), (1, 0, 2), (1, 1, 2), (0, 3, 1)])
-def test_tile_shape_changes_no_result_bit(coll, storage, visc):
+@pytest.mark.parametrize("size,coll,storage,visc", [((96, 24, 24), 0, 0, 1), ((96, 24, 24), 1, 0, 2), ((96, 24, 24), 1, 1, 2),
+                                                    ((96, 24, 24), 0, 3, 1), ((300, 20, 20), 0, 0, 1), ((300, 20, 20), 1, 1, 2)])
+def test_tile_shape_changes_no_result_bit(size, coll, storage, visc):
     """No result may depend on which nodes share a wavefront or a block (the sphere sums of the fused
     step run over 256 virtual threads whatever the block size): 40 steps of a sphere workload
-    (forced and unforced nodes in the same rows) with 64x4, 128x2, 256x1 and the default 128x1
-    tiles give the same populations, velocities and sphere state bit for bit."""
+    (forced and unforced nodes in the same rows) with 64x4, 128x2, 256x1 and the default tiles
+    (rows of 96 nodes: 128x2; rows of 300 nodes: 256 consecutive nodes of a plane in node order,
+    across rows) give the same populations, velocities and sphere state bit for bit."""
     vp = (0, 0, 0, 0) if visc == 1 else (2.0e-2, 0.7, 0, 0)
     runs = []
     for block in ((64, 4), (128, 2), (256, 1), (0, 0)):
-        c = case(size=(96, 24, 24), coll=coll, ibm=1, np_=6, diam=8.0, markers=128, dist=4, seed=31, tau=0.8, uw=0.03,
+        c = case(size=size, coll=coll, ibm=1, np_=6, diam=8.0, markers=128, dist=4, seed=31, tau=0.8, uw=0.03,
                  storage=storage, visc=visc, vparams=vp, local=1 if visc != 1 else 0, clamp=(0.51, 5.0) if visc != 1 else (0, 0))
         c.opt.block[0], c.opt.block[1] = block
         s = d3q19.Sim(c)
